@@ -1,0 +1,155 @@
+/*
+ * wmd.h -- C ABI of the MI355X Sinkhorn-Knopp Word Mover's Distance engine
+ * (libwmd.so, HIP kernels for gfx950, one GPU).
+ *
+ * Reference interface replaced: IntelLabs/Optimized-Implementation-of-Word-
+ * Movers-Distance, `void sinkhorn_wmd(const double* r, const double* vecs,
+ * REAL* WMD, const REAL lamda, const int max_iter, const u64 data_vocab_size,
+ * const u64 num_docs, const u64 word2vec_word_embedding_size)` at
+ * sinkhorn_wmd_fused.cpp:112-114 (same signature at
+ * sinkhorn_wmd_nonfused.cpp:114-116), which reads its V x N word-major CSR
+ * from the global `c_csr` (sinkhorn_wmd_fused.cpp:13, common.h:38-46).  Here
+ * the CSR is passed explicitly, every call is reentrant, the callee never
+ * frees caller memory, and every entry point returns a status code.
+ *
+ * Conventions
+ *   lamda   fused convention, K = exp(lamda * M) (sinkhorn_wmd_fused.cpp:217,
+ *           common.h:26 ships lamda = -1): lamda must be < 0.  The nonfused
+ *           entry keeps its file's convention K = exp(-M * lamda)
+ *           (sinkhorn_wmd_nonfused.cpp:172-173), so there lamda must be > 0.
+ *   CSR     V x N, row = word, u64 row_ptr[V+1], u64 col_inds[nnz] (doc ids),
+ *           double vals[nnz] > 0 (the reference's reader rejects w <= 0,
+ *           sinkhorn_wmd_fused.cpp:474-478).
+ *   r       dense query histogram r[V] >= 0; its nonzeros are the query words
+ *           (select_non_empty_entries, sinkhorn_wmd_fused.cpp:45-63).
+ *   WMD     double[N] per query, row-major [query][doc] for batches.  A
+ *           document with no words yields NaN exactly as the reference
+ *           (x = 0, u = inf, inf * 0); the call then returns WMD_ERR_NONFINITE
+ *           after writing every output.
+ */
+#ifndef WMD_H
+#define WMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum wmd_status {
+  WMD_OK = 0,
+  WMD_ERR_ARG = -1,         /* null pointer, bad size, bad lamda, invalid CSR or query */
+  WMD_ERR_HIP = -2,         /* HIP runtime error (no device, allocation, launch) */
+  WMD_ERR_NONFINITE = -3,   /* outputs written, but some are NaN/inf */
+  WMD_ERR_STATE = -4,       /* context call order: corpus or queries missing */
+  WMD_ERR_UNSUPPORTED = -5  /* shape outside the selected variant's limits */
+};
+
+enum wmd_variant {
+  /* one launch per tile shape; each (query, doc) tile stays in registers for
+   * all iterations (default) */
+  WMD_VARIANT_RESIDENT = 0,
+  /* one fused SDDMM+SpMM launch per iteration (sinkhorn_wmd_fused.cpp:249-319) */
+  WMD_VARIANT_FUSED = 1,
+  /* SDDMM launch + SpMM launch per iteration (sinkhorn_wmd_nonfused.cpp:226-261) */
+  WMD_VARIANT_NONFUSED = 2
+};
+
+typedef struct wmd_options {
+  int32_t max_iter;    /* iteration cap; the reference runs exactly 15 (:12) */
+  int32_t variant;     /* enum wmd_variant */
+  double tol;          /* 0: exactly max_iter iterations (reference behaviour);
+                          > 0: each (query, doc) pair stops once its marginal
+                          residual sum_i |u_i (K v)_i - r_i| < tol */
+  int32_t check_every; /* streaming variants with tol > 0: iterations per
+                          graph launch between convergence checks (0 -> 8) */
+  int32_t shape;       /* resident variant: -1 automatic, else a tile-shape id */
+  int32_t summarize;   /* 1: fill wmd_stats iteration/residual summaries (one
+                          device-to-host copy of per-pair counters); 0: stage
+                          times only */
+  int32_t reserved[5];
+} wmd_options;
+
+typedef struct wmd_stats {
+  double ms_total;     /* device time of the whole run (HIP events) */
+  double ms_cdist;     /* distance + exp stage */
+  double ms_solve;     /* iterations + final pass */
+  int64_t pairs;       /* queries x docs */
+  int32_t iters_max;   /* largest iteration count over pairs */
+  int32_t launches;    /* kernel launches (graph launches count their nodes) */
+  double iters_mean;
+  double resid_max;    /* largest final marginal residual over pairs */
+  int32_t variant;     /* variant that ran */
+  int32_t reserved[7];
+} wmd_stats;
+
+/* Default options: max_iter 15, resident variant, tol 0, automatic shape,
+ * summarize 1. */
+void wmd_options_default(wmd_options* opt);
+
+/* Drop-in for the fused reference sinkhorn_wmd (sinkhorn_wmd_fused.cpp:112-114)
+ * with its global CSR made explicit.  Runs on device 0 and blocks until WMD is
+ * written.  lamda < 0. */
+int wmd_sinkhorn(const double* r, const double* vecs, double* WMD, double lamda,
+                 int max_iter, uint64_t data_vocab_size, uint64_t num_docs,
+                 uint64_t word2vec_word_embedding_size, const uint64_t* row_ptr,
+                 const uint64_t* col_inds, const double* vals);
+
+/* Drop-in for the nonfused reference sinkhorn_wmd (sinkhorn_wmd_nonfused.cpp:
+ * 114-116): same arguments, K = exp(-M * lamda) so lamda > 0; runs the
+ * nonfused SDDMM/SpMM kernels. */
+int wmd_sinkhorn_nonfused(const double* r, const double* vecs, double* WMD, double lamda,
+                          int max_iter, uint64_t data_vocab_size, uint64_t num_docs,
+                          uint64_t word2vec_word_embedding_size, const uint64_t* row_ptr,
+                          const uint64_t* col_inds, const double* vals);
+
+/* wmd_sinkhorn with explicit options and statistics (either may be NULL).
+ * lamda in the fused convention. */
+int wmd_sinkhorn_ex(const double* r, const double* vecs, double* WMD, double lamda,
+                    uint64_t data_vocab_size, uint64_t num_docs,
+                    uint64_t word2vec_word_embedding_size, const uint64_t* row_ptr,
+                    const uint64_t* col_inds, const double* vals,
+                    const wmd_options* opt, wmd_stats* stats);
+
+/* ---- Resident-corpus context (no reference counterpart: the reference
+ * re-reads and re-uploads nothing because it has no device).  A context owns
+ * one HIP stream and the device copies of the corpus, queries and results;
+ * it is not thread-safe, distinct contexts are independent. */
+typedef struct wmd_context wmd_context;
+
+int wmd_context_create(int device, wmd_context** out);
+void wmd_context_destroy(wmd_context* ctx);
+
+/* Upload embeddings (FP32 or FP64 host array V x D, exactly one non-NULL)
+ * and the reference's V x N word-major CSR.  The corpus is converted once
+ * into the device layout (doc-major CSR over words that occur). */
+int wmd_context_set_corpus(wmd_context* ctx, const float* vecs_f32, const double* vecs_f64,
+                           uint64_t V, uint64_t D, const uint64_t* row_ptr,
+                           const uint64_t* col_inds, const double* vals, uint64_t num_docs);
+
+/* Upload n_queries queries as CSR: q_ptr[n+1], q_words (vocabulary ids,
+ * distinct per query), q_weights > 0. */
+int wmd_context_set_queries(wmd_context* ctx, uint64_t n_queries, const int64_t* q_ptr,
+                            const int64_t* q_words, const double* q_weights);
+
+/* Solve every (query, doc) pair on the device; results stay on the device. */
+int wmd_context_run(wmd_context* ctx, double lamda, const wmd_options* opt, wmd_stats* stats);
+
+/* Copy results to the host: wmd[n_queries * N], iters and resid may be NULL. */
+int wmd_context_result(wmd_context* ctx, double* wmd, int32_t* iters, float* resid);
+
+/* Device pointer of the n_queries x N FP64 result (valid until the next
+ * set_queries / destroy), for callers that keep results on the GPU. */
+const double* wmd_context_device_result(const wmd_context* ctx);
+
+/* Number of resident tile shapes and their limits (for tuners/tests). */
+int wmd_num_shapes(void);
+int wmd_shape_info(int shape, int32_t* waves, int32_t* cap_q, int32_t* cap_n);
+
+/* Human-readable description of the last error on this thread. */
+const char* wmd_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WMD_H */

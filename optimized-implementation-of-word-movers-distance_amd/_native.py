@@ -1,0 +1,109 @@
+"""ctypes binding of libwmd.so (the C ABI declared in include/wmd.h).
+
+There is no fallback: if the library is missing or fails to load, every
+entry point raises.  The library must be built first (build.py).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libwmd.so")
+
+WMD_OK = 0
+WMD_ERR_ARG = -1
+WMD_ERR_HIP = -2
+WMD_ERR_NONFINITE = -3
+WMD_ERR_STATE = -4
+WMD_ERR_UNSUPPORTED = -5
+STATUS_NAMES = {0: "WMD_OK", -1: "WMD_ERR_ARG", -2: "WMD_ERR_HIP", -3: "WMD_ERR_NONFINITE",
+                -4: "WMD_ERR_STATE", -5: "WMD_ERR_UNSUPPORTED"}
+
+VARIANT_RESIDENT = 0
+VARIANT_FUSED = 1
+VARIANT_NONFUSED = 2
+
+# Every symbol include/wmd.h declares.
+EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
+           "wmd_context_create", "wmd_context_destroy", "wmd_context_set_corpus",
+           "wmd_context_set_queries", "wmd_context_run", "wmd_context_result",
+           "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_last_error")
+
+
+class Options(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("variant", C.c_int32), ("tol", C.c_double),
+                ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_cdist", C.c_double), ("ms_solve", C.c_double),
+                ("pairs", C.c_int64), ("iters_max", C.c_int32), ("launches", C.c_int32),
+                ("iters_mean", C.c_double), ("resid_max", C.c_double),
+                ("variant", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class WMDError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {detail}")
+
+
+_lib = None
+
+
+def load():
+    """Load libwmd.so and declare its prototypes (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run the package's build.py "
+                          "(hipcc, gfx950) -- there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    P, U64 = C.c_void_p, C.c_uint64
+    L.wmd_options_default.argtypes = [C.POINTER(Options)]
+    L.wmd_options_default.restype = None
+    ref_sig = [P, P, P, C.c_double, C.c_int, U64, U64, U64, P, P, P]
+    L.wmd_sinkhorn.argtypes = ref_sig
+    L.wmd_sinkhorn_nonfused.argtypes = ref_sig
+    L.wmd_sinkhorn_ex.argtypes = [P, P, P, C.c_double, U64, U64, U64, P, P, P,
+                                  C.POINTER(Options), C.POINTER(Stats)]
+    L.wmd_context_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    L.wmd_context_destroy.argtypes = [P]
+    L.wmd_context_destroy.restype = None
+    L.wmd_context_set_corpus.argtypes = [P, P, P, U64, U64, P, P, P, U64]
+    L.wmd_context_set_queries.argtypes = [P, U64, P, P, P]
+    L.wmd_context_run.argtypes = [P, C.c_double, C.POINTER(Options), C.POINTER(Stats)]
+    L.wmd_context_result.argtypes = [P, P, P, P]
+    L.wmd_context_device_result.argtypes = [P]
+    L.wmd_context_device_result.restype = C.c_void_p
+    L.wmd_num_shapes.argtypes = []
+    L.wmd_shape_info.argtypes = [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int32)]
+    L.wmd_last_error.argtypes = []
+    L.wmd_last_error.restype = C.c_char_p
+    for name in ("wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
+                 "wmd_context_create", "wmd_context_set_corpus", "wmd_context_set_queries",
+                 "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info"):
+        getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(status: int, where: str) -> int:
+    if status != WMD_OK:
+        raise WMDError(status, where, load().wmd_last_error().decode(errors="replace"))
+    return status
+
+
+def default_options(**over) -> Options:
+    o = Options()
+    load().wmd_options_default(C.byref(o))
+    for k, v in over.items():
+        setattr(o, k, v)
+    return o

@@ -1,0 +1,157 @@
+"""Host-side mirror of the reference's sinkhorn_wmd interface over libwmd.so.
+
+* :func:`sinkhorn_wmd` has the signature and meaning of main.py:37
+  ``sinkhorn_wmd(r, c, vecs, lamb, max_iter)``: r dense query histogram (V),
+  c the V x N doc-term matrix (scipy sparse or a (row_ptr, col_inds, vals, N)
+  tuple), vecs V x D embeddings, lamb with K = exp(M * lamb) (main.py:65, so
+  negative), max_iter fixed iterations; returns the N distances.
+* :func:`sinkhorn_wmd_fused` / :func:`sinkhorn_wmd_nonfused` are the C entry
+  points replacing sinkhorn_wmd_fused.cpp:112-114 and
+  sinkhorn_wmd_nonfused.cpp:114-116 (nonfused keeps its exp(-M*lamda) sign).
+* :class:`WMDEngine` keeps a corpus resident on the GPU and solves batches of
+  queries (no reference counterpart).
+
+Every call goes through the HIP library; errors raise WMDError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as nat
+
+
+def _csr_arrays(c):
+    """(row_ptr u64, col_inds u64, vals f64, N) of a V x N word-major matrix."""
+    if isinstance(c, tuple):
+        row_ptr, col_inds, vals, N = c
+    else:  # scipy.sparse
+        m = c.tocsr()
+        row_ptr, col_inds, vals, N = m.indptr, m.indices, m.data, m.shape[1]
+    return (np.ascontiguousarray(row_ptr, dtype=np.uint64),
+            np.ascontiguousarray(col_inds, dtype=np.uint64),
+            np.ascontiguousarray(vals, dtype=np.float64), int(N))
+
+
+def _ref_call(fn_name, r, vecs, csr, lamda, max_iter):
+    L = nat.load()
+    r = np.ascontiguousarray(np.asarray(r, dtype=np.float64).reshape(-1))
+    vecs = np.ascontiguousarray(vecs, dtype=np.float64)
+    row_ptr, col_inds, vals, N = _csr_arrays(csr)
+    V, D = vecs.shape
+    if r.shape[0] != V or row_ptr.shape[0] != V + 1:
+        raise ValueError("r, vecs and the CSR disagree on the vocabulary size")
+    out = np.empty(N, dtype=np.float64)
+    st = getattr(L, fn_name)(r.ctypes.data, vecs.ctypes.data, out.ctypes.data, float(lamda),
+                             int(max_iter), V, N, D, row_ptr.ctypes.data, col_inds.ctypes.data,
+                             vals.ctypes.data)
+    if st == nat.WMD_ERR_NONFINITE:
+        return out  # NaN columns as the reference gives (empty documents)
+    nat.check(st, fn_name)
+    return out
+
+
+def sinkhorn_wmd_fused(r, vecs, csr, lamda=-1.0, max_iter=15):
+    """C entry wmd_sinkhorn: K = exp(lamda * M), lamda < 0."""
+    return _ref_call("wmd_sinkhorn", r, vecs, csr, lamda, max_iter)
+
+
+def sinkhorn_wmd_nonfused(r, vecs, csr, lamda=1.0, max_iter=15):
+    """C entry wmd_sinkhorn_nonfused: K = exp(-M * lamda), lamda > 0."""
+    return _ref_call("wmd_sinkhorn_nonfused", r, vecs, csr, lamda, max_iter)
+
+
+def sinkhorn_wmd(r, c, vecs, lamb, max_iter):
+    """main.py:37 on the GPU: returns ndarray[N] of distances."""
+    return sinkhorn_wmd_fused(r, vecs, c, lamda=lamb, max_iter=max_iter)
+
+
+class WMDEngine:
+    """A corpus resident on one GPU; solves query batches against it."""
+
+    def __init__(self, device: int = 0):
+        self._L = nat.load()
+        h = C.c_void_p()
+        nat.check(self._L.wmd_context_create(int(device), C.byref(h)), "wmd_context_create")
+        self._h = h
+        self.V = self.D = self.N = 0
+        self.n_queries = 0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self._L.wmd_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_corpus(self, vecs, csr):
+        """vecs: V x D float32/float64; csr: V x N word-major matrix."""
+        vecs = np.asarray(vecs)
+        if vecs.dtype == np.float32:
+            v32, v64 = np.ascontiguousarray(vecs), None
+        else:
+            v32, v64 = None, np.ascontiguousarray(vecs, dtype=np.float64)
+        row_ptr, col_inds, vals, N = _csr_arrays(csr)
+        V, D = vecs.shape
+        self._keep = (v32, v64)
+        nat.check(self._L.wmd_context_set_corpus(
+            self._h, v32.ctypes.data if v32 is not None else None,
+            v64.ctypes.data if v64 is not None else None, V, D, row_ptr.ctypes.data,
+            col_inds.ctypes.data, vals.ctypes.data, N), "wmd_context_set_corpus")
+        self._keep = None
+        self.V, self.D, self.N = V, D, N
+
+    def set_queries(self, q_ptr, words, weights):
+        q_ptr = np.ascontiguousarray(q_ptr, dtype=np.int64)
+        words = np.ascontiguousarray(words, dtype=np.int64)
+        weights = np.ascontiguousarray(weights, dtype=np.float64)
+        n = q_ptr.shape[0] - 1
+        nat.check(self._L.wmd_context_set_queries(self._h, n, q_ptr.ctypes.data, words.ctypes.data,
+                                                  weights.ctypes.data), "wmd_context_set_queries")
+        self.n_queries = n
+
+    def run(self, lamda, max_iter=15, tol=0.0, variant=nat.VARIANT_RESIDENT, check_every=8,
+            shape=-1, summarize=True) -> dict:
+        """Solve every (query, doc) pair; returns the run's wmd_stats as a dict
+        (with summarize=False only stage times, no device-to-host copy)."""
+        o = nat.default_options(max_iter=int(max_iter), tol=float(tol), variant=int(variant),
+                                check_every=int(check_every), shape=int(shape),
+                                summarize=int(bool(summarize)))
+        st = nat.Stats()
+        nat.check(self._L.wmd_context_run(self._h, float(lamda), C.byref(o), C.byref(st)),
+                  "wmd_context_run")
+        return st.as_dict()
+
+    def result(self, allow_nonfinite=False):
+        """(wmd [nq, N] float64, iters [nq, N] int32, resid [nq, N] float32)."""
+        n = self.n_queries * self.N
+        w = np.empty(n, dtype=np.float64)
+        it = np.empty(n, dtype=np.int32)
+        rs = np.empty(n, dtype=np.float32)
+        st = self._L.wmd_context_result(self._h, w.ctypes.data, it.ctypes.data, rs.ctypes.data)
+        if not (allow_nonfinite and st == nat.WMD_ERR_NONFINITE):
+            nat.check(st, "wmd_context_result")
+        shp = (self.n_queries, self.N)
+        return w.reshape(shp), it.reshape(shp), rs.reshape(shp)
+
+
+def shapes():
+    """[(waves, cap_q, cap_n)] of the resident tile shapes."""
+    L = nat.load()
+    out = []
+    for s in range(L.wmd_num_shapes()):
+        w, q, n = C.c_int32(), C.c_int32(), C.c_int32()
+        nat.check(L.wmd_shape_info(s, C.byref(w), C.byref(q), C.byref(n)), "wmd_shape_info")
+        out.append((w.value, q.value, n.value))
+    return out

@@ -1,0 +1,741 @@
+// C-ABI host runtime of the Sinkhorn-Knopp WMD engine (declarations in
+// include/wmd.h).  Owns device memory, converts the reference's V x N
+// word-major CSR (common.h:38-46) into the device layout once per corpus,
+// launches the gfx950 kernels of wmd_kernels.hip, and captures the streaming
+// variants' iteration loop in a HIP graph.
+#include "../../include/wmd.h"
+#include "wmd_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+#define HIP_TRY(expr)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return WMD_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+// Grow-only device buffer.
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;  // elements
+  hipError_t ensure(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+    if (e == hipSuccess) cap = std::max<size_t>(n, 1);
+    return e;
+  }
+  hipError_t upload(const T* h, size_t n, hipStream_t s) {
+    hipError_t e = ensure(n);
+    if (e != hipSuccess || n == 0) return e;
+    return hipMemcpyAsync(p, h, n * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+int round_up4(int x) { return (x + 3) & ~3; }
+
+}  // namespace
+
+struct wmd_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t* h_status = nullptr;  // pinned
+
+  // corpus
+  bool has_corpus = false;
+  int64_t V = 0, D = 0, N = 0, nnz = 0, Vt = 0;
+  int nmax = 0;
+  DevBuf<float> E;
+  DevBuf<int32_t> touched, doc_ptr, doc_t;
+  DevBuf<float> doc_val;
+
+  // queries
+  bool has_queries = false;
+  int nq = 0;
+  int qmax = 0;
+  std::vector<int32_t> h_q_ptr, h_k_stride;
+  std::vector<int64_t> h_k_off, h_u_off;
+  int64_t k_total = 0, u_total = 0;
+  DevBuf<int32_t> q_ptr, q_words, k_stride, bucket;
+  DevBuf<float> q_r;
+  DevBuf<int64_t> k_off, u_off;
+  DevBuf<float> K, mrow;
+  DevBuf<double> wmd;
+  DevBuf<int32_t> iters;
+  DevBuf<float> resid;
+  // streaming state
+  DevBuf<float> u, ow;
+  DevBuf<uint8_t> done;
+  DevBuf<uint32_t> status;
+  bool ran = false;
+};
+
+namespace {
+
+int check_ctx(wmd_context* ctx) {
+  if (!ctx) {
+    set_err("null context");
+    return WMD_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(ctx->device));
+  return WMD_OK;
+}
+
+// Fills SolveArgs fields common to every variant.
+wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
+  wmd::SolveArgs a{};
+  a.K = c->K.p;
+  a.mrow = c->mrow.p;
+  a.k_off = c->k_off.p;
+  a.k_stride = c->k_stride.p;
+  a.q_ptr = c->q_ptr.p;
+  a.q_r = c->q_r.p;
+  a.doc_ptr = c->doc_ptr.p;
+  a.doc_t = c->doc_t.p;
+  a.doc_val = c->doc_val.p;
+  a.nnz = c->nnz;
+  a.N = (int)c->N;
+  a.Vt = (int)c->Vt;
+  a.max_iter = o.max_iter;
+  a.tol = (float)o.tol;
+  a.inv_lamda = (float)(1.0 / lamda);
+  a.wmd = c->wmd.p;
+  a.iters = c->iters.p;
+  a.resid = c->resid.p;
+  a.lds_q = round_up4(std::max(c->qmax, 1));
+  a.lds_n = round_up4(std::max(c->nmax, 1));
+  return a;
+}
+
+int pick_shape(int q, int nmax) {
+  for (int s = 0; s < wmd::kNumShapes; ++s)
+    if (wmd::kShapes[s].cap_q() >= q && wmd::kShapes[s].cap_n() >= nmax) return s;
+  return -1;
+}
+
+int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, int* launches) {
+  // Group queries by tile shape; one launch per non-empty group.
+  std::vector<std::vector<int32_t>> groups(wmd::kNumShapes);
+  for (int Q = 0; Q < c->nq; ++Q) {
+    const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
+    int s = forced_shape >= 0 ? forced_shape : pick_shape(q, c->nmax);
+    groups[s].push_back(Q);
+  }
+  std::vector<int32_t> flat;
+  std::vector<std::pair<int, size_t>> spans;  // shape, offset
+  for (int s = 0; s < wmd::kNumShapes; ++s) {
+    if (groups[s].empty()) continue;
+    spans.push_back({s, flat.size()});
+    flat.insert(flat.end(), groups[s].begin(), groups[s].end());
+  }
+  HIP_TRY(c->bucket.upload(flat.data(), flat.size(), c->stream));
+  for (size_t g = 0; g < spans.size(); ++g) {
+    const int s = spans[g].first;
+    const size_t off = spans[g].second;
+    const size_t cnt = groups[s].size();
+    wmd::SolveArgs a = base;
+    a.bucket_q = c->bucket.p + off;
+    a.n_pairs = (int64_t)cnt * c->N;
+    HIP_TRY(wmd::launch_resident(s, a, c->stream));
+    ++*launches;
+  }
+  return WMD_OK;
+}
+
+// Streaming variants: iterations launched from a captured HIP graph.
+int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_options& o,
+                  int* launches) {
+  std::vector<int32_t> ident(c->nq);
+  for (int Q = 0; Q < c->nq; ++Q) ident[Q] = Q;
+  HIP_TRY(c->bucket.upload(ident.data(), ident.size(), c->stream));
+  HIP_TRY(c->u.ensure((size_t)c->u_total));
+  HIP_TRY(c->done.ensure((size_t)c->nq * c->N));
+  HIP_TRY(c->status.ensure(2));
+  if (!fused) HIP_TRY(c->ow.ensure((size_t)c->nq * (size_t)c->nnz));
+  a.bucket_q = c->bucket.p;
+  a.n_pairs = (int64_t)c->nq * c->N;
+  a.u = c->u.p;
+  a.u_off = c->u_off.p;
+  a.done = c->done.p;
+  a.ow = c->ow.p;
+  HIP_TRY(wmd::launch_stream_init(a, c->stream));
+  ++*launches;
+
+  auto step = [&](void) -> hipError_t {
+    if (fused) return wmd::launch_fused_step(a, false, c->stream);
+    hipError_t e = wmd::launch_sddmm(a, c->stream);
+    if (e != hipSuccess) return e;
+    return wmd::launch_spmm(a, false, c->stream);
+  };
+  const int per_step = fused ? 1 : 2;
+  const bool use_tol = o.tol > 0.0;
+  const int chunk = use_tol ? std::max(1, o.check_every > 0 ? o.check_every : 8) : o.max_iter;
+  int done_iters = 0;
+  hipGraphExec_t exec = nullptr;
+  int exec_len = -1;
+  int rc = WMD_OK;
+  while (done_iters < o.max_iter) {
+    const int len = std::min(chunk, o.max_iter - done_iters);
+    if (len != exec_len) {
+      if (exec) (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+      hipGraph_t graph = nullptr;
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t cap = hipSuccess;
+      for (int k = 0; k < len && cap == hipSuccess; ++k) cap = step();
+      if (cap == hipSuccess && use_tol) cap = wmd::launch_status(a, c->status.p, c->stream);
+      hipError_t end = hipStreamEndCapture(c->stream, &graph);
+      if (cap != hipSuccess || end != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        set_err("graph capture failed: %s", hipGetErrorString(cap != hipSuccess ? cap : end));
+        return WMD_ERR_HIP;
+      }
+      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        set_err("hipGraphInstantiate failed: %s", hipGetErrorString(ie));
+        return WMD_ERR_HIP;
+      }
+      exec_len = len;
+    }
+    hipError_t le = hipGraphLaunch(exec, c->stream);
+    if (le != hipSuccess) {
+      set_err("hipGraphLaunch failed: %s", hipGetErrorString(le));
+      rc = WMD_ERR_HIP;
+      break;
+    }
+    *launches += len * per_step + (use_tol ? 2 : 0);
+    done_iters += len;
+    if (use_tol) {
+      hipError_t ce = hipMemcpyAsync(c->h_status, c->status.p, 2 * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, c->stream);
+      if (ce == hipSuccess) ce = hipStreamSynchronize(c->stream);
+      if (ce != hipSuccess) {
+        set_err("status readback failed: %s", hipGetErrorString(ce));
+        rc = WMD_ERR_HIP;
+        break;
+      }
+      if (c->h_status[0] == 0) break;  // every pair has converged
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (rc != WMD_OK) return rc;
+  if (fused) {
+    HIP_TRY(wmd::launch_fused_step(a, true, c->stream));
+    ++*launches;
+  } else {
+    // Final SDDMM ignores the stop flags: clear them first (the converged
+    // pairs' w must be recomputed from their final u).
+    HIP_TRY(hipMemsetAsync(c->done.p, 0, (size_t)c->nq * c->N, c->stream));
+    HIP_TRY(wmd::launch_sddmm(a, c->stream));
+    HIP_TRY(wmd::launch_spmm(a, true, c->stream));
+    *launches += 2;
+  }
+  return WMD_OK;
+}
+
+int validate_csr(uint64_t V, uint64_t N, const uint64_t* row_ptr, const uint64_t* col_inds,
+                 const double* vals) {
+  if (!row_ptr || V == 0 || N == 0) {
+    set_err("empty corpus or null row_ptr");
+    return WMD_ERR_ARG;
+  }
+  if (V >= (1ull << 31) || N >= (1ull << 31)) {
+    set_err("V and N must be < 2^31");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (row_ptr[0] != 0) {
+    set_err("row_ptr[0] != 0");
+    return WMD_ERR_ARG;
+  }
+  for (uint64_t w = 0; w < V; ++w)
+    if (row_ptr[w + 1] < row_ptr[w]) {
+      set_err("row_ptr decreases at row %llu", (unsigned long long)w);
+      return WMD_ERR_ARG;
+    }
+  const uint64_t nnz = row_ptr[V];
+  if (nnz >= (1ull << 31)) {
+    set_err("nnz must be < 2^31");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (nnz > 0 && (!col_inds || !vals)) {
+    set_err("null col_inds/vals");
+    return WMD_ERR_ARG;
+  }
+  for (uint64_t e = 0; e < nnz; ++e) {
+    if (col_inds[e] >= N) {
+      set_err("col_inds[%llu] = %llu out of range", (unsigned long long)e,
+              (unsigned long long)col_inds[e]);
+      return WMD_ERR_ARG;
+    }
+    if (!(vals[e] > 0.0) || !std::isfinite(vals[e])) {
+      set_err("vals[%llu] must be finite and > 0", (unsigned long long)e);
+      return WMD_ERR_ARG;
+    }
+  }
+  return WMD_OK;
+}
+
+int one_shot(const double* r, const double* vecs, double* WMD, double lamda, uint64_t V,
+             uint64_t N, uint64_t D, const uint64_t* row_ptr, const uint64_t* col_inds,
+             const double* vals, const wmd_options* opt, wmd_stats* stats) {
+  if (!r || !vecs || !WMD) {
+    set_err("null r/vecs/WMD");
+    return WMD_ERR_ARG;
+  }
+  // select_non_empty_entries (sinkhorn_wmd_fused.cpp:45-63), ascending word order.
+  std::vector<int64_t> qp{0}, qw;
+  std::vector<double> qv;
+  for (uint64_t w = 0; w < V; ++w) {
+    if (!(r[w] >= 0.0)) {
+      set_err("r[%llu] is negative or NaN", (unsigned long long)w);
+      return WMD_ERR_ARG;
+    }
+    if (r[w] > 0.0) {
+      qw.push_back((int64_t)w);
+      qv.push_back(r[w]);
+    }
+  }
+  if (qw.empty()) {
+    set_err("r has no positive entry");
+    return WMD_ERR_ARG;
+  }
+  qp.push_back((int64_t)qw.size());
+  wmd_context* ctx = nullptr;
+  int rc = wmd_context_create(0, &ctx);
+  if (rc == WMD_OK)
+    rc = wmd_context_set_corpus(ctx, nullptr, vecs, V, D, row_ptr, col_inds, vals, N);
+  if (rc == WMD_OK) rc = wmd_context_set_queries(ctx, 1, qp.data(), qw.data(), qv.data());
+  if (rc == WMD_OK) rc = wmd_context_run(ctx, lamda, opt, stats);
+  if (rc == WMD_OK) rc = wmd_context_result(ctx, WMD, nullptr, nullptr);
+  std::string keep = g_err;
+  wmd_context_destroy(ctx);
+  g_err = keep;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+void wmd_options_default(wmd_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->max_iter = 15;
+  opt->variant = WMD_VARIANT_RESIDENT;
+  opt->tol = 0.0;
+  opt->check_every = 8;
+  opt->shape = -1;
+  opt->summarize = 1;
+}
+
+const char* wmd_last_error(void) { return g_err.c_str(); }
+
+int wmd_num_shapes(void) { return wmd::kNumShapes; }
+
+int wmd_shape_info(int shape, int32_t* waves, int32_t* cap_q, int32_t* cap_n) {
+  if (shape < 0 || shape >= wmd::kNumShapes) {
+    set_err("shape %d out of range", shape);
+    return WMD_ERR_ARG;
+  }
+  if (waves) *waves = wmd::kShapes[shape].W;
+  if (cap_q) *cap_q = wmd::kShapes[shape].cap_q();
+  if (cap_n) *cap_n = wmd::kShapes[shape].cap_n();
+  return WMD_OK;
+}
+
+int wmd_context_create(int device, wmd_context** out) {
+  if (!out) {
+    set_err("null output pointer");
+    return WMD_ERR_ARG;
+  }
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    set_err("device %d not present (%d devices)", device, ndev);
+    return WMD_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(device));
+  wmd_context* c = new wmd_context();
+  c->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 4 * sizeof(uint32_t), hipHostMallocDefault);
+  if (e != hipSuccess) {
+    set_err("context setup failed: %s", hipGetErrorString(e));
+    wmd_context_destroy(c);
+    return WMD_ERR_HIP;
+  }
+  *out = c;
+  return WMD_OK;
+}
+
+void wmd_context_destroy(wmd_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->E.release(); c->touched.release(); c->doc_ptr.release(); c->doc_t.release();
+  c->doc_val.release(); c->q_ptr.release(); c->q_words.release(); c->k_stride.release();
+  c->bucket.release(); c->q_r.release(); c->k_off.release(); c->u_off.release();
+  c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
+  c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
+  c->status.release();
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_status) (void)hipHostFree(c->h_status);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* vecs_f64,
+                           uint64_t V, uint64_t D, const uint64_t* row_ptr,
+                           const uint64_t* col_inds, const double* vals, uint64_t N) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if ((vecs_f32 == nullptr) == (vecs_f64 == nullptr)) {
+    set_err("exactly one of vecs_f32 / vecs_f64 must be given");
+    return WMD_ERR_ARG;
+  }
+  if (D == 0 || D > 65536) {
+    set_err("embedding size D=%llu unsupported", (unsigned long long)D);
+    return WMD_ERR_ARG;
+  }
+  rc = validate_csr(V, N, row_ptr, col_inds, vals);
+  if (rc != WMD_OK) return rc;
+  const uint64_t nnz = row_ptr[V];
+  // Compact word ids of words that occur, and the doc-major CSR over them.
+  std::vector<int32_t> compact(V, -1), touched;
+  for (uint64_t w = 0; w < V; ++w)
+    if (row_ptr[w + 1] > row_ptr[w]) {
+      compact[w] = (int32_t)touched.size();
+      touched.push_back((int32_t)w);
+    }
+  std::vector<int32_t> dptr(N + 1, 0), fill(N, 0), dt(nnz);
+  std::vector<float> dv(nnz);
+  for (uint64_t e = 0; e < nnz; ++e) dptr[col_inds[e] + 1]++;
+  int nmax = 0;
+  for (uint64_t j = 0; j < N; ++j) {
+    nmax = std::max(nmax, dptr[j + 1]);
+    dptr[j + 1] += dptr[j];
+  }
+  for (uint64_t w = 0; w < V; ++w)
+    for (uint64_t e = row_ptr[w]; e < row_ptr[w + 1]; ++e) {
+      const uint64_t j = col_inds[e];
+      const int32_t pos = dptr[j] + fill[j]++;
+      dt[pos] = compact[w];
+      dv[pos] = (float)vals[e];
+    }
+  std::vector<float> ef;
+  const float* eptr = vecs_f32;
+  if (!eptr) {
+    ef.resize(V * D);
+    for (uint64_t k = 0; k < V * D; ++k) ef[k] = (float)vecs_f64[k];
+    eptr = ef.data();
+  }
+  for (uint64_t k = 0; k < V * D; ++k)
+    if (!std::isfinite(eptr[k])) {
+      set_err("embedding element %llu is not finite", (unsigned long long)k);
+      return WMD_ERR_ARG;
+    }
+  HIP_TRY(c->E.upload(eptr, V * D, c->stream));
+  HIP_TRY(c->touched.upload(touched.data(), touched.size(), c->stream));
+  HIP_TRY(c->doc_ptr.upload(dptr.data(), dptr.size(), c->stream));
+  HIP_TRY(c->doc_t.upload(dt.data(), dt.size(), c->stream));
+  HIP_TRY(c->doc_val.upload(dv.data(), dv.size(), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->V = (int64_t)V;
+  c->D = (int64_t)D;
+  c->N = (int64_t)N;
+  c->nnz = (int64_t)nnz;
+  c->Vt = (int64_t)touched.size();
+  c->nmax = nmax;
+  c->has_corpus = true;
+  c->has_queries = false;  // query blocks depend on Vt and N
+  c->ran = false;
+  return WMD_OK;
+}
+
+int wmd_context_set_queries(wmd_context* c, uint64_t n, const int64_t* qp, const int64_t* qw,
+                            const double* qv) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->has_corpus) {
+    set_err("set_corpus must be called before set_queries");
+    return WMD_ERR_STATE;
+  }
+  if (n == 0 || n >= (1u << 20) || !qp || !qw || !qv) {
+    set_err("bad query batch (n=%llu)", (unsigned long long)n);
+    return WMD_ERR_ARG;
+  }
+  if (qp[0] != 0) {
+    set_err("q_ptr[0] != 0");
+    return WMD_ERR_ARG;
+  }
+  std::vector<int32_t> hq(n + 1), hs(n);
+  std::vector<int64_t> ko(n), uo(n);
+  int64_t ktot = 0, utot = 0;
+  int qmax = 0;
+  for (uint64_t Q = 0; Q < n; ++Q) {
+    const int64_t q = qp[Q + 1] - qp[Q];
+    if (q <= 0 || q > 65536) {
+      set_err("query %llu has %lld words", (unsigned long long)Q, (long long)q);
+      return WMD_ERR_ARG;
+    }
+    hq[Q] = (int32_t)qp[Q];
+    hs[Q] = round_up4((int)q);
+    ko[Q] = ktot;
+    uo[Q] = utot;
+    ktot += c->Vt * hs[Q];
+    utot += c->N * hs[Q];
+    qmax = std::max<int>(qmax, (int)q);
+  }
+  const int64_t total = qp[n];
+  if (total >= (1ll << 31)) {
+    set_err("too many query words");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  hq[n] = (int32_t)total;
+  std::vector<int32_t> words(total);
+  std::vector<float> weights(total);
+  for (int64_t k = 0; k < total; ++k) {
+    if (qw[k] < 0 || qw[k] >= c->V) {
+      set_err("query word %lld out of range", (long long)qw[k]);
+      return WMD_ERR_ARG;
+    }
+    if (!(qv[k] > 0.0) || !std::isfinite(qv[k])) {
+      set_err("query weight %lld must be finite and > 0", (long long)k);
+      return WMD_ERR_ARG;
+    }
+    words[k] = (int32_t)qw[k];
+    weights[k] = (float)qv[k];
+  }
+  HIP_TRY(c->q_ptr.upload(hq.data(), hq.size(), c->stream));
+  HIP_TRY(c->q_words.upload(words.data(), words.size(), c->stream));
+  HIP_TRY(c->q_r.upload(weights.data(), weights.size(), c->stream));
+  HIP_TRY(c->k_stride.upload(hs.data(), hs.size(), c->stream));
+  HIP_TRY(c->k_off.upload(ko.data(), ko.size(), c->stream));
+  HIP_TRY(c->u_off.upload(uo.data(), uo.size(), c->stream));
+  HIP_TRY(c->K.ensure((size_t)ktot));
+  HIP_TRY(c->mrow.ensure((size_t)n * c->Vt));
+  HIP_TRY(c->wmd.ensure((size_t)n * c->N));
+  HIP_TRY(c->iters.ensure((size_t)n * c->N));
+  HIP_TRY(c->resid.ensure((size_t)n * c->N));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->nq = (int)n;
+  c->qmax = qmax;
+  c->h_q_ptr = hq;
+  c->h_k_stride = hs;
+  c->h_k_off = ko;
+  c->h_u_off = uo;
+  c->k_total = ktot;
+  c->u_total = utot;
+  c->has_queries = true;
+  c->ran = false;
+  return WMD_OK;
+}
+
+int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd_stats* stats) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->has_corpus || !c->has_queries) {
+    set_err("corpus and queries must be set before run");
+    return WMD_ERR_STATE;
+  }
+  wmd_options o;
+  if (opt_in) o = *opt_in; else wmd_options_default(&o);
+  if (!(lamda < 0.0) || !std::isfinite(lamda)) {
+    set_err("lamda must be finite and < 0 (K = exp(lamda * M))");
+    return WMD_ERR_ARG;
+  }
+  if (o.max_iter < 0 || o.max_iter > (1 << 24) || !(o.tol >= 0.0)) {
+    set_err("bad max_iter/tol");
+    return WMD_ERR_ARG;
+  }
+  int variant = o.variant;
+  if (variant < WMD_VARIANT_RESIDENT || variant > WMD_VARIANT_NONFUSED) {
+    set_err("unknown variant %d", variant);
+    return WMD_ERR_ARG;
+  }
+  if (o.shape >= wmd::kNumShapes) {
+    set_err("shape %d out of range", o.shape);
+    return WMD_ERR_ARG;
+  }
+  if (variant == WMD_VARIANT_RESIDENT) {
+    bool fits = true;
+    for (int Q = 0; Q < c->nq && fits; ++Q) {
+      const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
+      if (o.shape >= 0)
+        fits = wmd::kShapes[o.shape].cap_q() >= q && wmd::kShapes[o.shape].cap_n() >= c->nmax;
+      else
+        fits = pick_shape(q, c->nmax) >= 0;
+    }
+    if (!fits) {
+      if (o.shape >= 0) {
+        set_err("tile shape %d cannot hold q or the longest document (%d words)", o.shape, c->nmax);
+        return WMD_ERR_UNSUPPORTED;
+      }
+      variant = WMD_VARIANT_FUSED;  // documents or queries beyond the register tiles
+    }
+  }
+  if (variant != WMD_VARIANT_RESIDENT) {
+    const size_t lds = sizeof(float) * ((size_t)round_up4(c->qmax) + 3 * (size_t)round_up4(c->nmax));
+    if (lds > 64 * 1024) {
+      set_err("query (%d words) or document (%d words) too long for the streaming kernels",
+              c->qmax, c->nmax);
+      return WMD_ERR_UNSUPPORTED;
+    }
+  }
+
+  wmd::CdistArgs ca{};
+  ca.E = c->E.p;
+  ca.touched = c->touched.p;
+  ca.q_ptr = c->q_ptr.p;
+  ca.q_words = c->q_words.p;
+  ca.k_off = c->k_off.p;
+  ca.k_stride = c->k_stride.p;
+  ca.K = c->K.p;
+  ca.mrow = c->mrow.p;
+  ca.D = (int)c->D;
+  ca.Vt = (int)c->Vt;
+  ca.lamda = (float)lamda;
+
+  int launches = 0;
+  HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+  HIP_TRY(wmd::launch_cdist(ca, c->nq, c->stream));
+  ++launches;
+  HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+  wmd::SolveArgs a = base_args(c, lamda, o);
+  if (variant == WMD_VARIANT_RESIDENT)
+    rc = run_resident(c, a, o.shape, &launches);
+  else
+    rc = run_streaming(c, a, variant == WMD_VARIANT_FUSED, o, &launches);
+  if (rc != WMD_OK) return rc;
+  HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev[2]));
+  c->ran = true;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    float t_all = 0.f, t_cd = 0.f, t_sv = 0.f;
+    HIP_TRY(hipEventElapsedTime(&t_all, c->ev[0], c->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t_cd, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&t_sv, c->ev[1], c->ev[2]));
+    stats->ms_total = t_all;
+    stats->ms_cdist = t_cd;
+    stats->ms_solve = t_sv;
+    stats->pairs = (int64_t)c->nq * c->N;
+    stats->launches = launches;
+    stats->variant = variant;
+    if (!o.summarize) return WMD_OK;
+    const size_t np = (size_t)c->nq * c->N;
+    std::vector<int32_t> it(np);
+    std::vector<float> rs(np);
+    HIP_TRY(hipMemcpy(it.data(), c->iters.p, np * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(rs.data(), c->resid.p, np * sizeof(float), hipMemcpyDeviceToHost));
+    double sum = 0.0, rmax = 0.0;
+    int imax = 0;
+    for (size_t k = 0; k < np; ++k) {
+      sum += it[k];
+      imax = std::max(imax, it[k]);
+      if (std::isnan(rs[k]) || rs[k] > rmax) rmax = rs[k];
+    }
+    stats->iters_max = imax;
+    stats->iters_mean = np ? sum / (double)np : 0.0;
+    stats->resid_max = rmax;
+  }
+  return WMD_OK;
+}
+
+int wmd_context_result(wmd_context* c, double* out, int32_t* iters, float* resid) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->ran) {
+    set_err("run must complete before result");
+    return WMD_ERR_STATE;
+  }
+  if (!out) {
+    set_err("null output");
+    return WMD_ERR_ARG;
+  }
+  const size_t np = (size_t)c->nq * c->N;
+  HIP_TRY(hipMemcpy(out, c->wmd.p, np * sizeof(double), hipMemcpyDeviceToHost));
+  if (iters) HIP_TRY(hipMemcpy(iters, c->iters.p, np * sizeof(int32_t), hipMemcpyDeviceToHost));
+  if (resid) HIP_TRY(hipMemcpy(resid, c->resid.p, np * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < np; ++k)
+    if (!std::isfinite(out[k])) {
+      set_err("result %zu (query %zu, doc %zu) is not finite", k, k / (size_t)c->N,
+              k % (size_t)c->N);
+      return WMD_ERR_NONFINITE;
+    }
+  return WMD_OK;
+}
+
+const double* wmd_context_device_result(const wmd_context* c) {
+  return (c && c->ran) ? c->wmd.p : nullptr;
+}
+
+int wmd_sinkhorn_ex(const double* r, const double* vecs, double* WMD, double lamda, uint64_t V,
+                    uint64_t N, uint64_t D, const uint64_t* row_ptr, const uint64_t* col_inds,
+                    const double* vals, const wmd_options* opt, wmd_stats* stats) {
+  return one_shot(r, vecs, WMD, lamda, V, N, D, row_ptr, col_inds, vals, opt, stats);
+}
+
+int wmd_sinkhorn(const double* r, const double* vecs, double* WMD, double lamda, int max_iter,
+                 uint64_t V, uint64_t N, uint64_t D, const uint64_t* row_ptr,
+                 const uint64_t* col_inds, const double* vals) {
+  wmd_options o;
+  wmd_options_default(&o);
+  o.max_iter = max_iter;
+  return one_shot(r, vecs, WMD, lamda, V, N, D, row_ptr, col_inds, vals, &o, nullptr);
+}
+
+int wmd_sinkhorn_nonfused(const double* r, const double* vecs, double* WMD, double lamda,
+                          int max_iter, uint64_t V, uint64_t N, uint64_t D,
+                          const uint64_t* row_ptr, const uint64_t* col_inds, const double* vals) {
+  if (!(lamda > 0.0)) {
+    set_err("nonfused convention K = exp(-M * lamda) needs lamda > 0");
+    return WMD_ERR_ARG;
+  }
+  wmd_options o;
+  wmd_options_default(&o);
+  o.max_iter = max_iter;
+  o.variant = WMD_VARIANT_NONFUSED;
+  return one_shot(r, vecs, WMD, -lamda, V, N, D, row_ptr, col_inds, vals, &o, nullptr);
+}
+
+}  // extern "C"

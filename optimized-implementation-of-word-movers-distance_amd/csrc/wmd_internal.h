@@ -1,0 +1,83 @@
+// Internal interface between the C-ABI host runtime (wmd_capi.hip) and the
+// gfx950 kernels (wmd_kernels.hip).  Not installed; include/wmd.h is the ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace wmd {
+
+// Distance/kernel stage.  For every query Q (words q_words[q_ptr[Q]..]) and
+// every touched word t (vocabulary id touched[t]) it writes
+//   Kt[Q][t][i] = exp(lamda * (M[t,i] - mrow[Q][t])),  mrow[Q][t] = min_i M[t,i]
+// with M the Euclidean distance of the FP32 embeddings.  Row rescaling by
+// exp(-lamda * mrow) leaves every Sinkhorn iterate and the final distance
+// unchanged (a diagonal scaling of K's rows is absorbed by v), and keeps each
+// row's largest entry at exactly 1 so no row underflows.
+struct CdistArgs {
+  const float* E;          // V x D embeddings, row-major
+  const int32_t* touched;  // Vt vocabulary ids of words that occur in the corpus
+  const int32_t* q_ptr;    // nq + 1
+  const int32_t* q_words;  // sum q vocabulary ids
+  const int64_t* k_off;    // nq offsets (floats) of each query's Vt x ks block
+  const int32_t* k_stride; // nq row strides ks (q rounded up to a multiple of 4)
+  float* K;
+  float* mrow;             // nq x Vt
+  int D;
+  int Vt;
+  float lamda;             // K = exp(lamda * M), lamda < 0
+};
+
+// Sinkhorn stage.  A pair is (query, document); pairs of one launch are
+// bucket_q[p / N] x (p % N), query-major.
+struct SolveArgs {
+  const float* K;
+  const float* mrow;
+  const int64_t* k_off;
+  const int32_t* k_stride;
+  const int32_t* q_ptr;
+  const float* q_r;        // query word weights (r restricted to the query)
+  const int32_t* doc_ptr;  // N + 1, doc-major CSR
+  const int32_t* doc_t;    // nnz compact word ids (index into touched)
+  const float* doc_val;    // nnz
+  const int32_t* bucket_q; // queries of this launch
+  int64_t n_pairs;
+  int64_t nnz;
+  int N;
+  int Vt;
+  int max_iter;
+  float tol;               // 0: fixed max_iter iterations
+  float inv_lamda;         // 1 / lamda
+  double* wmd;             // nq x N
+  int32_t* iters;          // nq x N
+  float* resid;            // nq x N
+  // streaming variants only
+  float* u;                // per query block of N x ks scaling vectors
+  const int64_t* u_off;    // nq
+  uint8_t* done;           // nq x N
+  float* ow;               // nq x nnz SDDMM output (nonfused)
+  int lds_q;               // LDS floats reserved for u (>= max q)
+  int lds_n;               // LDS floats reserved per document-length array (>= max n)
+};
+
+// Register-resident tile shapes of the resident solver: W waves per pair,
+// a B-lane group per document word row, NI query-word slots and NE
+// document-word slots per lane.  Capacity: q <= B*NI, n <= (64W/B)*NE.
+struct TileShape {
+  int W, B, NI, NE;
+  int cap_q() const { return B * NI; }
+  int cap_n() const { return (64 * W / B) * NE; }
+};
+constexpr int kNumShapes = 7;
+extern const TileShape kShapes[kNumShapes];
+
+hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
+hipError_t launch_resident(int shape, const SolveArgs& a, hipStream_t s);
+hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);
+hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
+hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);
+hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s);
+hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s);
+
+}  // namespace wmd

@@ -1,0 +1,590 @@
+// gfx950 kernels of the Sinkhorn-Knopp WMD engine.
+//
+// Stages (reference: sinkhorn_wmd_fused.cpp:112-412, sinkhorn_wmd_nonfused.cpp:114-318):
+//   k_cdist          distances M = cdist(query words, touched words) in FP32
+//                    direct differences, row-min rescale, exp epilogue -> Kt, mrow
+//   k_resident<...>  one (query, doc) pair per wave/workgroup: the pair's q x n
+//                    tile of Kt is gathered once into registers and every
+//                    Sinkhorn iteration plus the final (K o M) pass runs on-chip
+//   k_fused_step     streaming variant, one iteration per launch (SDDMM + SpMM
+//                    fused per pair, u kept in HBM between launches)
+//   k_sddmm/k_spmm   nonfused streaming variant (w = c / K^T u written to HBM)
+//
+// Layouts: Kt is query-major blocks [Q][t][ks] (word rows contiguous, so a
+// document word's q weights are one contiguous row); documents are a doc-major
+// CSR over compact word ids t, which makes the SpMM of every pair local
+// (no atomics, deterministic).
+#include "wmd_internal.h"
+
+#include <cfloat>
+
+namespace wmd {
+
+const TileShape kShapes[kNumShapes] = {
+    // W  B  NI NE      cap_q cap_n
+    {1, 2, 4, 5},    //   8    160
+    {1, 4, 5, 10},   //  20    160
+    {1, 8, 5, 20},   //  40    160
+    {1, 8, 8, 20},   //  64    160
+    {2, 16, 8, 20},  // 128    160
+    {4, 32, 8, 20},  // 256    160
+    {8, 64, 8, 20},  // 512    160
+};
+
+namespace {
+
+constexpr float kFltMin = 1.17549435e-38f;
+
+// Butterfly all-reduce over xor masks LO, 2LO, ..., HI.  Every lane of a
+// group ends with the bitwise-same sum (each step adds the same two values).
+template <int LO, int HI>
+__device__ __forceinline__ float bfly_sum(float v) {
+  if constexpr (LO <= HI) {
+    v += __shfl_xor(v, LO, 64);
+    return bfly_sum<LO * 2, HI>(v);
+  } else {
+    return v;
+  }
+}
+
+template <int LO, int HI>
+__device__ __forceinline__ double bfly_sum_d(double v) {
+  if constexpr (LO <= HI) {
+    v += __shfl_xor(v, LO, 64);
+    return bfly_sum_d<LO * 2, HI>(v);
+  } else {
+    return v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cdist + exp epilogue.  Block: 64 touched words x one query, query words in
+// chunks of 32, embedding components staged through LDS 32 at a time.  Thread
+// (tx, ty) owns words tx*4..tx*4+3 and chunk query words ty*2, ty*2+1.
+// Follows sinkhorn_wmd_fused.cpp:197-223 (euclidean_distance :35-43, exp :217).
+constexpr int CD_WT = 64;
+constexpr int CD_QC = 32;
+constexpr int CD_DC = 32;
+
+__global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
+  const int Q = blockIdx.y;
+  const int t0 = blockIdx.x * CD_WT;
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  float* Kq = a.K + a.k_off[Q];
+  float* mq = a.mrow + (int64_t)Q * a.Vt;
+
+  __shared__ __align__(16) float Ws[CD_DC][CD_WT + 4];
+  __shared__ __align__(16) float Qs[CD_DC][CD_QC + 4];
+  __shared__ float red[CD_WT][17];
+
+  const int tid = threadIdx.x;
+  const int tx = tid & 15;
+  const int ty = tid >> 4;
+  const int sw = tid >> 2, sd = (tid & 3) * 8;   // staging: word row, 8 components
+  const int sq = tid >> 3, sqd = (tid & 7) * 4;  // staging: query row, 4 components
+  const int tw = t0 + sw;
+  const float* wsrc = (tw < a.Vt) ? a.E + (int64_t)a.touched[tw] * a.D : nullptr;
+  const int nqc = (q + CD_QC - 1) / CD_QC;
+
+  float rmin[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  float dist[4][2];
+  for (int qc = 0; qc < nqc; ++qc) {
+    const int qi = qc * CD_QC + sq;
+    const float* qsrc = (qi < q) ? a.E + (int64_t)a.q_words[qb + qi] * a.D : nullptr;
+    float acc[4][2] = {};
+    for (int d0 = 0; d0 < a.D; d0 += CD_DC) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int d = d0 + sd + k;
+        Ws[sd + k][sw] = (wsrc && d < a.D) ? wsrc[d] : 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int d = d0 + sqd + k;
+        Qs[sqd + k][sq] = (qsrc && d < a.D) ? qsrc[d] : 0.f;
+      }
+      __syncthreads();
+      // Per-stage partial sums keep the FP32 accumulation error of the
+      // 300-term sum of squares near pairwise-summation level.
+      float part[4][2] = {};
+#pragma unroll 8
+      for (int d = 0; d < CD_DC; ++d) {
+        const float4 w = *reinterpret_cast<const float4*>(&Ws[d][tx * 4]);
+        const float2 qv = *reinterpret_cast<const float2*>(&Qs[d][ty * 2]);
+        const float wv[4] = {w.x, w.y, w.z, w.w};
+        const float qq[2] = {qv.x, qv.y};
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            const float df = wv[x] - qq[y];
+            part[x][y] = fmaf(df, df, part[x][y]);
+          }
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] += part[x][y];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int i = qc * CD_QC + ty * 2 + y;
+        const int t = t0 + tx * 4 + x;
+        const float dd = sqrtf(acc[x][y]);
+        dist[x][y] = dd;
+        if (i < q && t < a.Vt) {
+          rmin[x] = fminf(rmin[x], dd);
+          if (nqc > 1) Kq[(int64_t)t * ks + i] = dd;  // raw distance, rescaled below
+        }
+      }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) red[tx * 4 + x][ty] = rmin[x];
+  __syncthreads();
+  float m[4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    float v = red[tx * 4 + x][0];
+    for (int y = 1; y < 16; ++y) v = fminf(v, red[tx * 4 + x][y]);
+    m[x] = v;
+    const int t = t0 + tx * 4 + x;
+    if (ty == 0 && t < a.Vt) mq[t] = v;
+  }
+  // Epilogue: Kt = exp(lamda * (M - rowmin)); padding columns [q, ks) = 0.
+  for (int qc = 0; qc < nqc; ++qc) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int i = qc * CD_QC + ty * 2 + y;
+        const int t = t0 + tx * 4 + x;
+        if (t < a.Vt && i < ks) {
+          float kv = 0.f;
+          if (i < q) {
+            // Multi-chunk rows re-read the raw distance this thread stored.
+            const float dd = (nqc == 1) ? dist[x][y] : Kq[(int64_t)t * ks + i];
+            kv = expf(a.lamda * (dd - m[x]));
+          }
+          Kq[(int64_t)t * ks + i] = kv;
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
+// query-word lanes.  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
+// i = b + B*m (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
+//   s_e = sum_i K[e,i] u_i      (partial over own i, butterfly over the B lanes)
+//   o_e = c_e / s_e
+//   x_i = sum_e K[e,i] o_e      (partial over own e, butterfly over the A lanes,
+//                                LDS across waves when W > 1)
+//   res = sum_i |u_i x_i - r_i| (row-marginal residual of the current plan)
+//   u_i = r_i / x_i             (= 1 / x of the reference's K_over_r form)
+// then the final half-step and sum_{i,e} u_i K M o_e in FP64 (:339-393), with
+// M recovered from the rescaled kernel as mrow + log(Kt) / lamda.
+template <int W, int B, int NI, int NE>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
+  constexpr int AW = 64 / B;
+  constexpr int A = W * AW;
+  __shared__ float xsh[(W > 1) ? 2 * W * NI * B : 1];
+  __shared__ double dsh[(W > 1) ? W : 1];
+
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
+  if (pair >= a.n_pairs) return;  // W == 1: per wave (no block barriers); W > 1: whole block
+  const int b = lane & (B - 1);
+  const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
+
+  const int Q = a.bucket_q[pair / a.N];
+  const int j = (int)(pair % a.N);
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  const float* Kq = a.K + a.k_off[Q];
+  const int db = a.doc_ptr[j];
+  const int n = a.doc_ptr[j + 1] - db;
+  const int nk = (n + A - 1) / A;  // wave-uniform slot counts
+  const int nm = (q + B - 1) / B;
+
+  float Kt[NE][NI];
+  float c[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = ea + A * k;
+    const bool ve = (k < nk) && (e < n);
+    int t = 0;
+    float cv = 0.f;
+    if (ve) {
+      t = a.doc_t[db + e];
+      cv = a.doc_val[db + e];
+    }
+    c[k] = cv;
+    const float* row = Kq + (int64_t)t * ks;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      const int i = b + B * m;
+      Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
+    }
+  }
+  float u[NI], r[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) {
+    const int i = b + B * m;
+    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
+    u[m] = (i < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
+  }
+
+  const bool use_tol = a.tol > 0.f;
+  int it = 0;
+  float res = 0.f;
+  float o[NE];
+  while (it < a.max_iter) {
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      if (k < nk) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+        s = bfly_sum<1, B / 2>(s);
+        o[k] = (c[k] > 0.f) ? c[k] / s : 0.f;
+      } else {
+        o[k] = 0.f;
+      }
+    }
+    float xs[NI];
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      float x = 0.f;
+#pragma unroll
+      for (int k = 0; k < NE; ++k)
+        if (k < nk) x = fmaf(Kt[k][m], o[k], x);
+      xs[m] = bfly_sum<B, 32>(x);
+    }
+    if constexpr (W > 1) {
+      float* buf = xsh + (it & 1) * (W * NI * B);
+      if (lane < B) {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) buf[(wv * NI + m) * B + b] = xs[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        float x = 0.f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) x += buf[(w * NI + m) * B + b];
+        xs[m] = x;
+      }
+      // Double buffering: the next write to this half happens two
+      // iterations later, after every wave has passed the next barrier.
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m)
+      if (r[m] > 0.f) part += fabsf(u[m] * xs[m] - r[m]);
+    res = bfly_sum<1, B / 2>(part);
+#pragma unroll
+    for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? r[m] / xs[m] : 0.f;
+    ++it;
+    if (use_tol && res < a.tol) break;
+  }
+
+  // Final half-step: v = c / (K^T u), then sum u o ((K o M) v) in FP64.
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    if (k < nk) {
+      float s = 0.f;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+      s = bfly_sum<1, B / 2>(s);
+      const int e = ea + A * k;
+      if (e < n) {
+        const float oe = c[k] / s;
+        const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+#pragma unroll
+        for (int m = 0; m < NI; ++m) {
+          const float kv = Kt[k][m];
+          if (kv >= kFltMin) {
+            const float Mv = mr + __logf(kv) * a.inv_lamda;
+            acc += (double)u[m] * (double)kv * (double)Mv * (double)oe;
+          }
+        }
+      }
+    }
+  }
+  acc = bfly_sum_d<1, 32>(acc);
+  if constexpr (W > 1) {
+    if (lane == 0) dsh[wv] = acc;
+    __syncthreads();
+    acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc += dsh[w];
+  }
+  if (lane == 0 && (W == 1 || wv == 0)) {
+    const int64_t out = (int64_t)Q * a.N + j;
+    a.wmd[out] = (n > 0) ? acc : NAN;  // empty column: the reference's 0 * inf
+    a.iters[out] = it;
+    a.resid[out] = res;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Streaming variants: one wave (block of 64) per pair per launch.
+struct PairRef {
+  int Q, j, qb, q, ks, db, n;
+  int64_t out;
+};
+
+__device__ __forceinline__ PairRef pair_ref(const SolveArgs& a, int64_t pair) {
+  PairRef p;
+  p.Q = a.bucket_q[pair / a.N];
+  p.j = (int)(pair % a.N);
+  p.qb = a.q_ptr[p.Q];
+  p.q = a.q_ptr[p.Q + 1] - p.qb;
+  p.ks = a.k_stride[p.Q];
+  p.db = a.doc_ptr[p.j];
+  p.n = a.doc_ptr[p.j + 1] - p.db;
+  p.out = (int64_t)p.Q * a.N + p.j;
+  return p;
+}
+
+__device__ __forceinline__ float wave_sum(float v) { return bfly_sum<1, 32>(v); }
+
+__global__ __launch_bounds__(64) void k_stream_init(SolveArgs a) {
+  const PairRef p = pair_ref(a, blockIdx.x);
+  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  for (int i = threadIdx.x; i < p.ks; i += 64) ug[i] = (i < p.q) ? (float)p.q : 0.f;
+  if (threadIdx.x == 0) {
+    a.done[p.out] = 0;
+    a.iters[p.out] = 0;
+    a.resid[p.out] = 0.f;
+  }
+}
+
+// SpMM half of an iteration for one pair, lanes over query words: reads o (or
+// the nonfused w) from LDS, updates u in HBM, residual and stop flag.
+__device__ __forceinline__ void spmm_update(const SolveArgs& a, const PairRef& p,
+                                            const float* Kq, const float* u_sh,
+                                            const float* o_sh, const int* t_sh, float* ug) {
+  float part = 0.f;
+  for (int i = threadIdx.x; i < p.q; i += 64) {
+    float x = 0.f;
+    for (int e = 0; e < p.n; ++e) x = fmaf(Kq[(int64_t)t_sh[e] * p.ks + i], o_sh[e], x);
+    const float ri = a.q_r[p.qb + i];
+    part += fabsf(u_sh[i] * x - ri);
+    ug[i] = ri / x;
+  }
+  part = wave_sum(part);
+  if (threadIdx.x == 0) {
+    a.resid[p.out] = part;
+    a.iters[p.out] += 1;
+    if (a.tol > 0.f && part < a.tol) a.done[p.out] = 1;
+  }
+}
+
+// Final pass for one pair: sum_i u_i sum_e K M o_e in FP64.
+__device__ __forceinline__ void final_sum(const SolveArgs& a, const PairRef& p,
+                                          const float* Kq, const float* u_sh,
+                                          const float* o_sh, const int* t_sh,
+                                          const float* m_sh) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < p.q; i += 64) {
+    double x = 0.0;
+    for (int e = 0; e < p.n; ++e) {
+      const float kv = Kq[(int64_t)t_sh[e] * p.ks + i];
+      if (kv >= kFltMin) {
+        const float Mv = m_sh[e] + __logf(kv) * a.inv_lamda;
+        x += (double)kv * (double)Mv * (double)o_sh[e];
+      }
+    }
+    acc += (double)u_sh[i] * x;
+  }
+  acc = bfly_sum_d<1, 32>(acc);
+  if (threadIdx.x == 0) a.wmd[p.out] = (p.n > 0) ? acc : NAN;
+}
+
+// Fused SDDMM + SpMM for one pair (the GPU form of sinkhorn_wmd_fused.cpp:249-319,
+// doc-major so the x accumulation needs no atomics).
+__global__ __launch_bounds__(64) void k_fused_step(SolveArgs a, int final_pass) {
+  extern __shared__ float sh[];
+  const PairRef p = pair_ref(a, blockIdx.x);
+  if (!final_pass && a.done[p.out]) return;  // block-uniform
+  float* u_sh = sh;
+  float* o_sh = u_sh + a.lds_q;
+  float* m_sh = o_sh + a.lds_n;
+  int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
+  const float* Kq = a.K + a.k_off[p.Q];
+  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  for (int i = threadIdx.x; i < p.q; i += 64) u_sh[i] = ug[i];
+  for (int e = threadIdx.x; e < p.n; e += 64) {
+    const int t = a.doc_t[p.db + e];
+    t_sh[e] = t;
+    if (final_pass) m_sh[e] = a.mrow[(int64_t)p.Q * a.Vt + t];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < p.n; e += 64) {
+    const float* row = Kq + (int64_t)t_sh[e] * p.ks;
+    float s = 0.f;
+    for (int i = 0; i < p.q; ++i) s = fmaf(row[i], u_sh[i], s);
+    o_sh[e] = a.doc_val[p.db + e] / s;
+  }
+  __syncthreads();
+  if (final_pass)
+    final_sum(a, p, Kq, u_sh, o_sh, t_sh, m_sh);
+  else
+    spmm_update(a, p, Kq, u_sh, o_sh, t_sh, ug);
+}
+
+// Nonfused SDDMM (sinkhorn_wmd_nonfused.cpp:72-90): w_e = c_e / (K[t_e,:] . u).
+__global__ __launch_bounds__(64) void k_sddmm(SolveArgs a) {
+  extern __shared__ float sh[];
+  const PairRef p = pair_ref(a, blockIdx.x);
+  if (a.done[p.out]) return;
+  float* u_sh = sh;
+  const float* Kq = a.K + a.k_off[p.Q];
+  const float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  for (int i = threadIdx.x; i < p.q; i += 64) u_sh[i] = ug[i];
+  __syncthreads();
+  float* w = a.ow + (int64_t)p.Q * a.nnz + p.db;
+  for (int e = threadIdx.x; e < p.n; e += 64) {
+    const float* row = Kq + (int64_t)a.doc_t[p.db + e] * p.ks;
+    float s = 0.f;
+    for (int i = 0; i < p.q; ++i) s = fmaf(row[i], u_sh[i], s);
+    w[e] = a.doc_val[p.db + e] / s;
+  }
+}
+
+// Nonfused SpMM (sinkhorn_wmd_nonfused.cpp:237-261, final :279-306).
+__global__ __launch_bounds__(64) void k_spmm(SolveArgs a, int final_pass) {
+  extern __shared__ float sh[];
+  const PairRef p = pair_ref(a, blockIdx.x);
+  if (!final_pass && a.done[p.out]) return;
+  float* u_sh = sh;
+  float* o_sh = u_sh + a.lds_q;
+  float* m_sh = o_sh + a.lds_n;
+  int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
+  const float* Kq = a.K + a.k_off[p.Q];
+  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  const float* w = a.ow + (int64_t)p.Q * a.nnz + p.db;
+  for (int i = threadIdx.x; i < p.q; i += 64) u_sh[i] = ug[i];
+  for (int e = threadIdx.x; e < p.n; e += 64) {
+    const int t = a.doc_t[p.db + e];
+    t_sh[e] = t;
+    o_sh[e] = w[e];
+    if (final_pass) m_sh[e] = a.mrow[(int64_t)p.Q * a.Vt + t];
+  }
+  __syncthreads();
+  if (final_pass)
+    final_sum(a, p, Kq, u_sh, o_sh, t_sh, m_sh);
+  else
+    spmm_update(a, p, Kq, u_sh, o_sh, t_sh, ug);
+}
+
+// status[0] += number of pairs still iterating; status[1] = max residual bits.
+__global__ __launch_bounds__(256) void k_status(const uint8_t* done, const float* resid,
+                                                int64_t n, uint32_t* status) {
+  __shared__ uint32_t s_cnt;
+  __shared__ uint32_t s_max;
+  if (threadIdx.x == 0) { s_cnt = 0; s_max = 0; }
+  __syncthreads();
+  uint32_t cnt = 0, mx = 0;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    cnt += done[p] ? 0u : 1u;
+    const uint32_t bits = __float_as_uint(resid[p]);  // residuals are >= 0 (or NaN)
+    mx = bits > mx ? bits : mx;
+  }
+  atomicAdd(&s_cnt, cnt);
+  atomicMax(&s_max, mx);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&status[0], s_cnt);
+    atomicMax(&status[1], s_max);
+  }
+}
+
+__global__ void k_zero_status(uint32_t* status) {
+  if (threadIdx.x < 2) status[threadIdx.x] = 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s) {
+  if (a.Vt <= 0 || nq <= 0) return hipSuccess;
+  dim3 grid((a.Vt + CD_WT - 1) / CD_WT, nq);
+  hipLaunchKernelGGL(k_cdist, grid, dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+template <int W, int B, int NI, int NE>
+static hipError_t launch_res(const SolveArgs& a, hipStream_t s) {
+  const int64_t blocks = (W == 1) ? (a.n_pairs + 3) / 4 : a.n_pairs;
+  hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
+                     dim3(W == 1 ? 256 : 64 * W), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_resident(int shape, const SolveArgs& a, hipStream_t s) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  switch (shape) {
+    case 0: return launch_res<1, 2, 4, 5>(a, s);
+    case 1: return launch_res<1, 4, 5, 10>(a, s);
+    case 2: return launch_res<1, 8, 5, 20>(a, s);
+    case 3: return launch_res<1, 8, 8, 20>(a, s);
+    case 4: return launch_res<2, 16, 8, 20>(a, s);
+    case 5: return launch_res<4, 32, 8, 20>(a, s);
+    case 6: return launch_res<8, 64, 8, 20>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+static size_t stream_lds(const SolveArgs& a) {
+  return sizeof(float) * ((size_t)a.lds_q + 3 * (size_t)a.lds_n);
+}
+
+hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_stream_init, dim3((unsigned)a.n_pairs), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fused_step, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
+                     final_pass ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sddmm, dim3((unsigned)a.n_pairs), dim3(64),
+                     sizeof(float) * (size_t)a.lds_q, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_spmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
+                     final_pass ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, status);
+  const int64_t n = a.n_pairs;
+  int blocks = (int)((n + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_status, dim3(blocks), dim3(256), 0, s, a.done, a.resid, n, status);
+  return hipGetLastError();
+}
+
+}  // namespace wmd

@@ -1,0 +1,286 @@
+"""Parity of the HIP engine (through the C ABI) with the reference and the
+FP64 oracle.  Tolerances: fixed-iteration FP32 results within 1e-5 relative
+of the reference's FP64 outputs; tolerance-mode results within 1e-4 of the
+FP64 oracle stopped by the same rule (an FP32 residual can cross 1e-4 one
+iteration apart from FP64), and within 1e-5 where the iteration counts agree."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, pkg, rel_err
+
+pytestmark = pytest.mark.gpu
+
+RES, FUS, NON = 0, 1, 2
+VARIANTS = [RES, FUS, NON]
+VNAME = {RES: "resident", FUS: "fused", NON: "nonfused"}
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def P():
+    return pkg()
+
+
+def engine(P, corpus):
+    e = P.WMDEngine(0)
+    e.set_corpus(corpus.vecs, corpus.word_major())
+    return e
+
+
+@pytest.fixture(scope="module")
+def plain(dg, P):
+    corpus, qs, p = dg.workload("plain", seed=0)
+    e = engine(P, corpus)
+    e.set_queries(qs.q_ptr, qs.words, qs.weights)
+    yield corpus, qs, p, e
+    e.close()
+
+
+@pytest.mark.parametrize("name", ["tiny", "cpu_small", "plain"])
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_fixed_iterations_match_reference(P, dg, name, variant):
+    say(f"[case] fixed15 {name} {VNAME[variant]}")
+    corpus, qs, p = dg.workload(name, seed=0)
+    gold = load_golden(name)["wmd_fused"]
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        st = e.run(p["lamda"], max_iter=15, variant=variant)
+        w, it, _ = e.result()
+    assert st["variant"] == variant
+    assert (it == 15).all()
+    err = rel_err(w, gold)
+    say(f"  max rel err vs reference = {err:.3e}")
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["cpu_small", "plain"])
+def test_reference_drop_in_entries(P, dg, name):
+    say(f"[case] drop-in wmd_sinkhorn / wmd_sinkhorn_nonfused {name}")
+    corpus, qs, p = dg.workload(name, seed=0)
+    gold = load_golden(name)
+    r = qs.dense_r(0, corpus.V)
+    vecs = corpus.vecs.astype(np.float64)
+    csr = corpus.word_major()
+    wf = P.sinkhorn_wmd_fused(r, vecs, csr, lamda=p["lamda"], max_iter=15)
+    wn = P.sinkhorn_wmd_nonfused(r, vecs, csr, lamda=-p["lamda"], max_iter=15)
+    assert rel_err(wf, gold["wmd_fused"][0]) <= 1e-5
+    assert rel_err(wn, gold["wmd_nonfused"][0]) <= 1e-5
+    # main.py:37 signature
+    import scipy.sparse as sp
+    row_ptr, col, vals, N = csr
+    c = sp.csr_matrix((vals, col.astype(np.int64), row_ptr.astype(np.int64)), shape=(corpus.V, N))
+    wm = P.sinkhorn_wmd(r, c, vecs, p["lamda"], 15)
+    assert rel_err(wm, gold["wmd_fused"][0]) <= 1e-5
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_tolerance_mode_matches_fp64_oracle(plain, oracle, variant):
+    say(f"[case] tol1e-4 plain {VNAME[variant]}")
+    corpus, qs, p, e = plain
+    st = e.run(p["lamda"], max_iter=1000, tol=1e-4, variant=variant, check_every=4)
+    w, it, res = e.result()
+    wo, ito, _ = oracle.perdoc(*qs.query(0), corpus.vecs.astype(np.float64), corpus.doc_ptr,
+                               corpus.doc_words, corpus.doc_vals, p["lamda"], 1000, tol=1e-4)
+    say(f"  iters mean {it.mean():.2f} max {it.max()} (oracle {ito.mean():.2f}); "
+        f"launches {st['launches']}")
+    assert np.all(res < 1e-4)
+    assert np.all(np.abs(it[0] - ito) <= 1)
+    assert np.mean(it[0] == ito) > 0.95
+    same = it[0] == ito
+    assert rel_err(w[0][same], wo[same]) <= 1e-5
+    assert rel_err(w[0], wo) <= 1e-4
+
+
+def test_variants_agree_and_are_deterministic(plain):
+    say("[case] variants agree + run-twice bitwise")
+    corpus, qs, p, e = plain
+    outs = {}
+    for v in VARIANTS:
+        e.run(p["lamda"], max_iter=15, variant=v)
+        outs[v] = e.result()[0].copy()
+        e.run(p["lamda"], max_iter=15, variant=v)
+        assert np.array_equal(e.result()[0], outs[v]), VNAME[v]
+    assert rel_err(outs[FUS], outs[RES]) < 2e-6
+    assert rel_err(outs[NON], outs[FUS]) < 2e-6
+
+
+def _batch_queries(dg, V, lengths, seed=11):
+    rng = np.random.default_rng(seed)
+    return dg.make_queries(rng, V, lengths)
+
+
+def test_batched_every_tile_shape(P, dg, oracle):
+    # one query per shape boundary (1..300 words) against a 1500-doc corpus
+    corpus, _, _ = dg.workload("batched", seed=1, N=1500, n_queries=1)
+    lengths = [1, 5, 8, 9, 20, 21, 40, 41, 64, 65, 128, 129, 256, 257, 300]
+    qs = _batch_queries(dg, corpus.V, lengths)
+    vecs = corpus.vecs.astype(np.float64)
+    for tol, max_iter in ((0.0, 15), (1e-4, 10000)):
+        say(f"[case] batched shapes tol={tol}")
+        with engine(P, corpus) as e:
+            e.set_queries(qs.q_ptr, qs.words, qs.weights)
+            e.run(-10.0, max_iter=max_iter, tol=tol)
+            w, it, res = e.result()
+        wo, ito, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, vecs, corpus.doc_ptr,
+                                         corpus.doc_words, corpus.doc_vals, -10.0, max_iter, tol=tol)
+        for k, q in enumerate(lengths):
+            same = it[k] == ito[k]
+            err = rel_err(w[k][same], wo[k][same])
+            say(f"  q={q:3d} err {err:.2e} iters {it[k].mean():.1f}/{ito[k].mean():.1f}")
+            assert err <= 1e-5, q
+            if tol > 0:
+                assert np.all(res[k] < tol)
+                assert rel_err(w[k], wo[k]) <= 1e-4
+
+
+def test_forced_shapes_agree(P, dg):
+    say("[case] forced tile shapes")
+    corpus, _, _ = dg.workload("batched", seed=2, N=800, n_queries=1)
+    qs = _batch_queries(dg, corpus.V, [7, 7, 7], seed=3)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        ref = None
+        for s in range(len(P.shapes())):
+            e.run(-10.0, max_iter=15, shape=s)
+            w = e.result()[0]
+            if ref is None:
+                ref = w.copy()
+            assert rel_err(w, ref) < 2e-6, s
+
+
+def test_closed_forms_on_gpu(P, dg):
+    say("[case] closed forms q=1 and single-word documents")
+    corpus, _, _ = dg.workload("tiny", seed=3)
+    V, N = corpus.V, corpus.N
+    vecs = corpus.vecs.astype(np.float64)
+    with engine(P, corpus) as e:
+        e.set_queries(np.array([0, 1]), np.array([17]), np.array([1.0]))
+        e.run(-10.0, max_iter=7)
+        w = e.result()[0][0]
+    M = np.sqrt(((vecs[corpus.doc_words] - vecs[17]) ** 2).sum(1))
+    assert rel_err(w, np.add.reduceat(corpus.doc_vals * M, corpus.doc_ptr[:-1])) <= 1e-5
+    # single-word documents: WMD_j = sum_i r_i M[i, w_j] / sum r
+    rng = np.random.default_rng(5)
+    words = np.sort(rng.choice(V, size=40, replace=False))
+    row_ptr = np.zeros(V + 1, dtype=np.uint64)
+    row_ptr[1:] = np.cumsum(np.bincount(words, minlength=V))
+    csr = (row_ptr, np.arange(40, dtype=np.uint64), np.ones(40), 40)
+    qw = np.array([3, 50, 77, 120])
+    r = np.array([0.1, 0.2, 0.3, 0.4])
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, csr)
+        e.set_queries(np.array([0, 4]), qw, r)
+        for v in VARIANTS:
+            e.run(-10.0, max_iter=9, variant=v)
+            w = e.result()[0][0]
+            Mq = np.sqrt(((vecs[qw][:, None, :] - vecs[words][None]) ** 2).sum(-1))
+            assert rel_err(w, r @ Mq) <= 1e-5, VNAME[v]
+
+
+def test_empty_document_gives_nan_like_reference(P, dg, oracle):
+    say("[case] empty document column")
+    corpus, qs, p = dg.workload("tiny", seed=0)
+    row_ptr, col, vals, N = corpus.word_major()
+    keep = col != 5  # drop every word of doc 5
+    counts = np.bincount(np.repeat(np.arange(corpus.V), np.diff(row_ptr.astype(np.int64)))[keep],
+                         minlength=corpus.V)
+    rp = np.zeros(corpus.V + 1, dtype=np.uint64)
+    rp[1:] = np.cumsum(counts)
+    csr = (rp, col[keep], vals[keep], N)
+    r = qs.dense_r(1, corpus.V)
+    vecs = corpus.vecs.astype(np.float64)
+    ref = oracle.fused(r, vecs, csr, -10.0, 15)
+    assert np.isnan(ref[5])
+    for v in VARIANTS:
+        with P.WMDEngine(0) as e:
+            e.set_corpus(corpus.vecs, csr)
+            e.set_queries(*[np.array(a) for a in ([0, len(qs.query(1)[0])], *qs.query(1))])
+            e.run(-10.0, max_iter=15, variant=v)
+            with pytest.raises(P.WMDError) as ei:
+                e.result()
+            assert ei.value.status == P._native.WMD_ERR_NONFINITE
+            w = e.result(allow_nonfinite=True)[0][0]
+        assert np.isnan(w[5])
+        ok = np.arange(N) != 5
+        assert rel_err(w[ok], ref[ok]) <= 1e-5
+    w1 = P.sinkhorn_wmd_fused(r, vecs, csr, lamda=-10.0)
+    assert np.isnan(w1[5]) and rel_err(w1[ok], ref[ok]) <= 1e-5
+
+
+def test_long_documents_fall_back_to_streaming(P, dg, oracle):
+    say("[case] documents longer than the register tile")
+    corpus, qs, p = dg.workload("batched", seed=4, N=300, n_queries=2, len_lo=150, len_hi=220)
+    vecs = corpus.vecs.astype(np.float64)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        st = e.run(-10.0, max_iter=15)
+        w = e.result()[0]
+    assert st["variant"] == FUS
+    wo, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, vecs, corpus.doc_ptr,
+                                   corpus.doc_words, corpus.doc_vals, -10.0, 15)
+    assert rel_err(w, wo) <= 1e-5
+
+
+def test_invalid_arguments(P, dg):
+    say("[case] invalid arguments")
+    corpus, qs, p = dg.workload("tiny", seed=0)
+    nat = P._native
+    r = qs.dense_r(0, corpus.V)
+    vecs = corpus.vecs.astype(np.float64)
+    csr = corpus.word_major()
+
+    def status(fn):
+        with pytest.raises(P.WMDError) as ei:
+            fn()
+        return ei.value.status
+
+    assert status(lambda: P.sinkhorn_wmd_fused(r, vecs, csr, lamda=0.0)) == nat.WMD_ERR_ARG
+    assert status(lambda: P.sinkhorn_wmd_fused(r, vecs, csr, lamda=2.0)) == nat.WMD_ERR_ARG
+    assert status(lambda: P.sinkhorn_wmd_nonfused(r, vecs, csr, lamda=-2.0)) == nat.WMD_ERR_ARG
+    bad_r = r.copy()
+    bad_r[3] = -0.5
+    assert status(lambda: P.sinkhorn_wmd_fused(bad_r, vecs, csr, lamda=-1.0)) == nat.WMD_ERR_ARG
+    assert status(lambda: P.sinkhorn_wmd_fused(np.zeros_like(r), vecs, csr)) == nat.WMD_ERR_ARG
+    rp, col, vals, N = csr
+    bad_col = col.copy()
+    bad_col[0] = N
+    assert status(lambda: P.sinkhorn_wmd_fused(r, vecs, (rp, bad_col, vals, N))) == nat.WMD_ERR_ARG
+    bad_vals = vals.copy()
+    bad_vals[1] = 0.0
+    assert status(lambda: P.sinkhorn_wmd_fused(r, vecs, (rp, col, bad_vals, N))) == nat.WMD_ERR_ARG
+    with P.WMDEngine(0) as e:
+        assert status(lambda: e.set_queries([0, 1], [1], [1.0])) == nat.WMD_ERR_STATE
+        e.set_corpus(corpus.vecs, csr)
+        assert status(lambda: e.set_queries([0, 1], [corpus.V], [1.0])) == nat.WMD_ERR_ARG
+        assert status(lambda: e.set_queries([0, 1], [2], [0.0])) == nat.WMD_ERR_ARG
+        assert status(lambda: e.run(-1.0)) == nat.WMD_ERR_STATE
+        e.set_queries([0, 1], [2], [1.0])
+        assert status(lambda: e.run(-1.0, variant=7)) == nat.WMD_ERR_ARG
+        assert status(lambda: e.run(-1.0, shape=0, max_iter=3)) in (nat.WMD_OK,
+                                                                     nat.WMD_ERR_UNSUPPORTED)
+
+
+def test_batched_full_size_properties(P, dg, oracle):
+    # the batched workload at its full size: 256 queries x 50,000 documents
+    say("[case] batched full size, residual 1e-4")
+    corpus, qs, p = dg.workload("batched", seed=0)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        st = e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"])
+        w, it, res = e.result()
+    say(f"  {st}")
+    assert st["variant"] == RES
+    assert np.all(np.isfinite(w)) and np.all(w > 0)
+    assert np.all(res < 1e-4)
+    vecs = corpus.vecs.astype(np.float64)
+    for k in (0, int(np.argmax(qs.lengths())), qs.n - 1):
+        wo, ito, _ = oracle.perdoc(*qs.query(k), vecs, corpus.doc_ptr, corpus.doc_words,
+                                   corpus.doc_vals, p["lamda"], p["max_iter"], tol=p["tol"])
+        err = rel_err(w[k], wo)
+        say(f"  query {k} (q={qs.lengths()[k]}): err {err:.2e}")
+        assert err <= 1e-4
