@@ -82,6 +82,8 @@ struct wmd_context {
   int nmax = 0;
   DevBuf<float> E;
   DevBuf<int32_t> touched, doc_ptr, doc_t;
+  DevBuf<int32_t> doc_perm;             // documents sorted by length class
+  std::vector<int64_t> h_class_off;     // kNumDocClasses + 1 offsets into doc_perm
   DevBuf<float> doc_val;
 
   // queries
@@ -149,7 +151,9 @@ int pick_shape(int q, int nmax) {
 }
 
 int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, int* launches) {
-  // Group queries by tile shape; one launch per non-empty group.
+  // Group queries by tile shape; one launch per (shape, document-length
+  // class) with both non-empty.  Pairs are query-major inside a launch so the
+  // concurrently running waves share one query's kernel rows in cache.
   std::vector<std::vector<int32_t>> groups(wmd::kNumShapes);
   for (int Q = 0; Q < c->nq; ++Q) {
     const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
@@ -168,11 +172,17 @@ int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, i
     const int s = spans[g].first;
     const size_t off = spans[g].second;
     const size_t cnt = groups[s].size();
-    wmd::SolveArgs a = base;
-    a.bucket_q = c->bucket.p + off;
-    a.n_pairs = (int64_t)cnt * c->N;
-    HIP_TRY(wmd::launch_resident(s, a, c->stream));
-    ++*launches;
+    for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
+      const int64_t d0 = c->h_class_off[cls], d1 = c->h_class_off[cls + 1];
+      if (d1 == d0) continue;
+      wmd::SolveArgs a = base;
+      a.bucket_q = c->bucket.p + off;
+      a.doc_list = c->doc_perm.p + d0;
+      a.n_list = (int)(d1 - d0);
+      a.n_pairs = (int64_t)cnt * (d1 - d0);
+      HIP_TRY(wmd::launch_resident(s, cls, a, c->stream));
+      ++*launches;
+    }
   }
   return WMD_OK;
 }
@@ -415,6 +425,7 @@ void wmd_context_destroy(wmd_context* c) {
   c->bucket.release(); c->q_r.release(); c->k_off.release(); c->u_off.release();
   c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
+  c->doc_perm.release();
   c->status.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -478,7 +489,24 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   HIP_TRY(c->doc_ptr.upload(dptr.data(), dptr.size(), c->stream));
   HIP_TRY(c->doc_t.upload(dt.data(), dt.size(), c->stream));
   HIP_TRY(c->doc_val.upload(dv.data(), dv.size(), c->stream));
+  // Documents grouped by length class (stable, so ids ascend inside a
+  // class); documents beyond the last class are left out, and the resident
+  // variant is then not used for this corpus.
+  std::vector<int64_t> class_off(wmd::kNumDocClasses + 1, 0);
+  std::vector<int32_t> perm;
+  perm.reserve(N);
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
+    class_off[cls] = (int64_t)perm.size();
+    for (uint64_t j = 0; j < N; ++j) {
+      const int len = dptr[j + 1] - dptr[j];
+      const int cj = len == 0 ? 0 : (len - 1) / wmd::kDocClassWidth;
+      if (cj == cls) perm.push_back((int32_t)j);
+    }
+  }
+  class_off[wmd::kNumDocClasses] = (int64_t)perm.size();
+  HIP_TRY(c->doc_perm.upload(perm.data(), perm.size(), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  c->h_class_off = class_off;
   c->V = (int64_t)V;
   c->D = (int64_t)D;
   c->N = (int64_t)N;

@@ -42,6 +42,8 @@ struct SolveArgs {
   const int32_t* doc_t;    // nnz compact word ids (index into touched)
   const float* doc_val;    // nnz
   const int32_t* bucket_q; // queries of this launch
+  const int32_t* doc_list; // resident launches: documents of this launch
+  int n_list;              // resident launches: number of documents in doc_list
   int64_t n_pairs;
   int64_t nnz;
   int N;
@@ -64,16 +66,20 @@ struct SolveArgs {
 // Register-resident tile shapes of the resident solver: W waves per pair,
 // a B-lane group per document word row, NI query-word slots and NE
 // document-word slots per lane.  Capacity: q <= B*NI, n <= (64W/B)*NE.
+// Each launch also fixes a document-length class (documents of at most
+// kDocClassWidth*(c+1) words), which sets the slot count it compiles for.
 struct TileShape {
   int W, B, NI, NE;
   int cap_q() const { return B * NI; }
   int cap_n() const { return (64 * W / B) * NE; }
 };
 constexpr int kNumShapes = 7;
+constexpr int kNumDocClasses = 5;
+constexpr int kDocClassWidth = 32;
 extern const TileShape kShapes[kNumShapes];
 
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
-hipError_t launch_resident(int shape, const SolveArgs& a, hipStream_t s);
+hipError_t launch_resident(int shape, int doc_class, const SolveArgs& a, hipStream_t s);
 hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);

@@ -35,17 +35,68 @@ namespace {
 
 constexpr float kFltMin = 1.17549435e-38f;
 
-// Butterfly all-reduce over xor masks LO, 2LO, ..., HI.  Every lane of a
-// group ends with the bitwise-same sum (each step adds the same two values).
+// Cross-lane sums without LDS round trips.  Each step adds a lane's value to
+// its partner's; both lanes of a pair compute the same two operands, so every
+// lane of a group ends with the bitwise-same sum.
+//   xor 1, 2   DPP quad_perm            xor 8   DPP row_ror:8 (16-lane rows)
+//   xor 4      ds_swizzle xor mode      xor 16  v_permlane16_swap (gfx950)
+//   xor 32     v_permlane32_swap (gfx950)
+// A swap of two copies of v returns (row pair low half, row pair high half)
+// in its two results, so their sum is the pair sum in one fixed order.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+template <int MASK>
+__device__ __forceinline__ float xor_step(float v) {
+  if constexpr (MASK == 1) {
+    return v + dpp_mov<0xB1>(v);  // quad_perm [1,0,3,2]
+  } else if constexpr (MASK == 2) {
+    return v + dpp_mov<0x4E>(v);  // quad_perm [2,3,0,1]
+  } else if constexpr (MASK == 4) {
+    return v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+  } else if constexpr (MASK == 8) {
+    return v + dpp_mov<0x128>(v);  // row_ror:8 == xor 8 inside a 16-lane row
+  } else if constexpr (MASK == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  } else {
+    static_assert(MASK == 32, "xor masks are powers of two below 64");
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
+                                                    false, false);
+    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  }
+}
+
+// Butterfly all-reduce over xor masks LO, 2LO, ..., HI (no-op when LO > HI).
 template <int LO, int HI>
 __device__ __forceinline__ float bfly_sum(float v) {
   if constexpr (LO <= HI) {
-    v += __shfl_xor(v, LO, 64);
-    return bfly_sum<LO * 2, HI>(v);
+    return bfly_sum<LO * 2, HI>(xor_step<LO>(v));
   } else {
     return v;
   }
 }
+
+// All-reduce over aligned groups of G consecutive lanes.  After the xor-1 and
+// xor-2 steps every lane of a quad holds the same value, so the 8-lane step
+// can pair the two quads with DPP row_half_mirror (lane i <-> 7-i) instead of
+// an exact xor-4 swizzle through the LDS unit.
+template <int G>
+__device__ __forceinline__ float group_sum(float v) {
+  if constexpr (G >= 2) v = xor_step<1>(v);
+  if constexpr (G >= 4) v = xor_step<2>(v);
+  if constexpr (G >= 8) v = v + dpp_mov<0x141>(v);  // row_half_mirror
+  if constexpr (G >= 16) v = xor_step<8>(v);
+  if constexpr (G >= 32) v = xor_step<16>(v);
+  if constexpr (G >= 64) v = xor_step<32>(v);
+  return v;
+}
+
+// Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
 template <int LO, int HI>
 __device__ __forceinline__ double bfly_sum_d(double v) {
@@ -188,6 +239,10 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
 //   u_i = r_i / x_i             (= 1 / x of the reference's K_over_r form)
 // then the final half-step and sum_{i,e} u_i K M o_e in FP64 (:339-393), with
 // M recovered from the rescaled kernel as mrow + log(Kt) / lamda.
+//
+// A launch covers one tile shape and one document-length class: NE is the
+// class's slot count, so the iteration loop has no slot guards.  Slots past
+// the document's or the query's end hold K = 0 and contribute nothing.
 template <int W, int B, int NI, int NE>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
   constexpr int AW = 64 / B;
@@ -196,29 +251,29 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   __shared__ double dsh[(W > 1) ? W : 1];
 
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  // The wave index is uniform; readfirstlane keeps everything derived from it
+  // (pair, q, n, offsets) in scalar registers.
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
   if (pair >= a.n_pairs) return;  // W == 1: per wave (no block barriers); W > 1: whole block
   const int b = lane & (B - 1);
   const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
 
-  const int Q = a.bucket_q[pair / a.N];
-  const int j = (int)(pair % a.N);
+  const int Q = a.bucket_q[pair / a.n_list];
+  const int j = a.doc_list[pair % a.n_list];
   const int qb = a.q_ptr[Q];
   const int q = a.q_ptr[Q + 1] - qb;
   const int ks = a.k_stride[Q];
   const float* Kq = a.K + a.k_off[Q];
   const int db = a.doc_ptr[j];
   const int n = a.doc_ptr[j + 1] - db;
-  const int nk = (n + A - 1) / A;  // wave-uniform slot counts
-  const int nm = (q + B - 1) / B;
 
   float Kt[NE][NI];
   float c[NE];
 #pragma unroll
   for (int k = 0; k < NE; ++k) {
     const int e = ea + A * k;
-    const bool ve = (k < nk) && (e < n);
+    const bool ve = e < n;
     int t = 0;
     float cv = 0.f;
     if (ve) {
@@ -248,23 +303,18 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   while (it < a.max_iter) {
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
-      if (k < nk) {
-        float s = 0.f;
+      float s = 0.f;
 #pragma unroll
-        for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
-        s = bfly_sum<1, B / 2>(s);
-        o[k] = (c[k] > 0.f) ? c[k] / s : 0.f;
-      } else {
-        o[k] = 0.f;
-      }
+      for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+      s = group_sum<B>(s);
+      o[k] = (c[k] > 0.f) ? fdiv(c[k], s) : 0.f;
     }
     float xs[NI];
 #pragma unroll
     for (int m = 0; m < NI; ++m) {
       float x = 0.f;
 #pragma unroll
-      for (int k = 0; k < NE; ++k)
-        if (k < nk) x = fmaf(Kt[k][m], o[k], x);
+      for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
       xs[m] = bfly_sum<B, 32>(x);
     }
     if constexpr (W > 1) {
@@ -288,9 +338,9 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 #pragma unroll
     for (int m = 0; m < NI; ++m)
       if (r[m] > 0.f) part += fabsf(u[m] * xs[m] - r[m]);
-    res = bfly_sum<1, B / 2>(part);
+    res = group_sum<B>(part);
 #pragma unroll
-    for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? r[m] / xs[m] : 0.f;
+    for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? fdiv(r[m], xs[m]) : 0.f;
     ++it;
     if (use_tol && res < a.tol) break;
   }
@@ -299,22 +349,20 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < NE; ++k) {
-    if (k < nk) {
-      float s = 0.f;
+    float s = 0.f;
 #pragma unroll
-      for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
-      s = bfly_sum<1, B / 2>(s);
-      const int e = ea + A * k;
-      if (e < n) {
-        const float oe = c[k] / s;
-        const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+    for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+    s = group_sum<B>(s);
+    const int e = ea + A * k;
+    if (e < n) {
+      const float oe = fdiv(c[k], s);
+      const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
 #pragma unroll
-        for (int m = 0; m < NI; ++m) {
-          const float kv = Kt[k][m];
-          if (kv >= kFltMin) {
-            const float Mv = mr + __logf(kv) * a.inv_lamda;
-            acc += (double)u[m] * (double)kv * (double)Mv * (double)oe;
-          }
+      for (int m = 0; m < NI; ++m) {
+        const float kv = Kt[k][m];
+        if (kv >= kFltMin) {
+          const float Mv = mr + __logf(kv) * a.inv_lamda;
+          acc += (double)u[m] * (double)kv * (double)Mv * (double)oe;
         }
       }
     }
@@ -334,7 +382,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     a.resid[out] = res;
   }
 }
-
 // ---------------------------------------------------------------------------
 // Streaming variants: one wave (block of 64) per pair per launch.
 struct PairRef {
@@ -532,16 +579,33 @@ static hipError_t launch_res(const SolveArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_resident(int shape, const SolveArgs& a, hipStream_t s) {
+// Document-length class c holds documents of at most kDocClassWidth*(c+1)
+// words: NE = that length over the shape's A document-word lanes.
+template <int W, int B, int NI>
+static hipError_t launch_shape(int cls, const SolveArgs& a, hipStream_t s) {
+  constexpr int A = 64 * W / B;
+  constexpr int C = kDocClassWidth;
+  static_assert(kNumDocClasses == 5, "one case per document-length class");
+  switch (cls) {
+    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A>(a, s);
+    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A>(a, s);
+    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A>(a, s);
+    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A>(a, s);
+    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A>(a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_resident(int shape, int doc_class, const SolveArgs& a, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
   switch (shape) {
-    case 0: return launch_res<1, 2, 4, 5>(a, s);
-    case 1: return launch_res<1, 4, 5, 10>(a, s);
-    case 2: return launch_res<1, 8, 5, 20>(a, s);
-    case 3: return launch_res<1, 8, 8, 20>(a, s);
-    case 4: return launch_res<2, 16, 8, 20>(a, s);
-    case 5: return launch_res<4, 32, 8, 20>(a, s);
-    case 6: return launch_res<8, 64, 8, 20>(a, s);
+    case 0: return launch_shape<1, 2, 4>(doc_class, a, s);
+    case 1: return launch_shape<1, 4, 5>(doc_class, a, s);
+    case 2: return launch_shape<1, 8, 5>(doc_class, a, s);
+    case 3: return launch_shape<1, 8, 8>(doc_class, a, s);
+    case 4: return launch_shape<2, 16, 8>(doc_class, a, s);
+    case 5: return launch_shape<4, 32, 8>(doc_class, a, s);
+    case 6: return launch_shape<8, 64, 8>(doc_class, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -261,8 +261,10 @@ def test_invalid_arguments(P, dg):
         assert status(lambda: e.run(-1.0)) == nat.WMD_ERR_STATE
         e.set_queries([0, 1], [2], [1.0])
         assert status(lambda: e.run(-1.0, variant=7)) == nat.WMD_ERR_ARG
-        assert status(lambda: e.run(-1.0, shape=0, max_iter=3)) in (nat.WMD_OK,
-                                                                     nat.WMD_ERR_UNSUPPORTED)
+        e.run(-1.0, shape=0, max_iter=3)  # q = 1 fits the smallest tile
+        big = np.arange(0, 2 * 9, 2)      # 9 words: beyond shape 0's 8-word capacity
+        e.set_queries([0, 9], big, np.full(9, 1.0 / 9))
+        assert status(lambda: e.run(-1.0, shape=0, max_iter=3)) == nat.WMD_ERR_UNSUPPORTED
 
 
 def test_batched_full_size_properties(P, dg, oracle):
