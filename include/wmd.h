@@ -55,6 +55,14 @@ enum wmd_variant {
   WMD_VARIANT_NONFUSED = 2
 };
 
+enum wmd_domain {
+  /* FP32 scalings u, v on the row-rescaled kernel exp(lamda (M - min_i M)) */
+  WMD_DOMAIN_LINEAR = 0,
+  /* FP32 base-2 log potentials with log-sum-exp steps: for large |lamda|,
+   * where the kernel underflows FP32 (resident variant only) */
+  WMD_DOMAIN_LOG = 1
+};
+
 typedef struct wmd_options {
   int32_t max_iter;    /* iteration cap; the reference runs exactly 15 (:12) */
   int32_t variant;     /* enum wmd_variant */
@@ -67,7 +75,8 @@ typedef struct wmd_options {
   int32_t summarize;   /* 1: fill wmd_stats iteration/residual summaries (one
                           device-to-host copy of per-pair counters); 0: stage
                           times only */
-  int32_t reserved[5];
+  int32_t domain;      /* enum wmd_domain */
+  int32_t reserved[4];
 } wmd_options;
 
 typedef struct wmd_stats {

@@ -24,6 +24,9 @@ VARIANT_RESIDENT = 0
 VARIANT_FUSED = 1
 VARIANT_NONFUSED = 2
 
+DOMAIN_LINEAR = 0
+DOMAIN_LOG = 1
+
 # Every symbol include/wmd.h declares.
 EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
            "wmd_context_create", "wmd_context_destroy", "wmd_context_set_corpus",
@@ -34,7 +37,7 @@ EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("variant", C.c_int32), ("tol", C.c_double),
                 ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
-                ("reserved", C.c_int32 * 5)]
+                ("domain", C.c_int32), ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):

@@ -136,6 +136,7 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   a.max_iter = o.max_iter;
   a.tol = (float)o.tol;
   a.inv_lamda = (float)(1.0 / lamda);
+  a.lamda = (float)lamda;
   a.wmd = c->wmd.p;
   a.iters = c->iters.p;
   a.resid = c->resid.p;
@@ -150,7 +151,8 @@ int pick_shape(int q, int nmax) {
   return -1;
 }
 
-int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, int* launches) {
+int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, bool log_domain,
+                 int* launches) {
   // Group queries by tile shape; one launch per (shape, document-length
   // class) with both non-empty.  Pairs are query-major inside a launch so the
   // concurrently running waves share one query's kernel rows in cache.
@@ -180,7 +182,7 @@ int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, i
       a.doc_list = c->doc_perm.p + d0;
       a.n_list = (int)(d1 - d0);
       a.n_pairs = (int64_t)cnt * (d1 - d0);
-      HIP_TRY(wmd::launch_resident(s, cls, a, c->stream));
+      HIP_TRY(wmd::launch_resident(s, cls, log_domain, a, c->stream));
       ++*launches;
     }
   }
@@ -372,6 +374,7 @@ void wmd_options_default(wmd_options* opt) {
   opt->check_every = 8;
   opt->shape = -1;
   opt->summarize = 1;
+  opt->domain = WMD_DOMAIN_LINEAR;
 }
 
 const char* wmd_last_error(void) { return g_err.c_str(); }
@@ -624,6 +627,11 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     set_err("shape %d out of range", o.shape);
     return WMD_ERR_ARG;
   }
+  if (o.domain != WMD_DOMAIN_LINEAR && o.domain != WMD_DOMAIN_LOG) {
+    set_err("unknown domain %d", o.domain);
+    return WMD_ERR_ARG;
+  }
+  const bool log_domain = o.domain == WMD_DOMAIN_LOG;
   if (variant == WMD_VARIANT_RESIDENT) {
     bool fits = true;
     for (int Q = 0; Q < c->nq && fits; ++Q) {
@@ -640,6 +648,12 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
       }
       variant = WMD_VARIANT_FUSED;  // documents or queries beyond the register tiles
     }
+  }
+  if (log_domain && variant != WMD_VARIANT_RESIDENT) {
+    set_err("the log domain runs on the resident variant only (documents up to %d words, "
+            "queries up to %d words)", wmd::kShapes[0].cap_n(),
+            wmd::kShapes[wmd::kNumShapes - 1].cap_q());
+    return WMD_ERR_UNSUPPORTED;
   }
   if (variant != WMD_VARIANT_RESIDENT) {
     const size_t lds = sizeof(float) * ((size_t)round_up4(c->qmax) + 3 * (size_t)round_up4(c->nmax));
@@ -662,6 +676,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   ca.D = (int)c->D;
   ca.Vt = (int)c->Vt;
   ca.lamda = (float)lamda;
+  ca.log_out = log_domain ? 1 : 0;
 
   int launches = 0;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
@@ -670,7 +685,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   wmd::SolveArgs a = base_args(c, lamda, o);
   if (variant == WMD_VARIANT_RESIDENT)
-    rc = run_resident(c, a, o.shape, &launches);
+    rc = run_resident(c, a, o.shape, log_domain, &launches);
   else
     rc = run_streaming(c, a, variant == WMD_VARIANT_FUSED, o, &launches);
   if (rc != WMD_OK) return rc;

@@ -27,6 +27,7 @@ struct CdistArgs {
   int D;
   int Vt;
   float lamda;             // K = exp(lamda * M), lamda < 0
+  int log_out;             // 1: store base-2 logits lamda (M - mrow) log2(e) instead
 };
 
 // Sinkhorn stage.  A pair is (query, document); pairs of one launch are
@@ -51,6 +52,7 @@ struct SolveArgs {
   int max_iter;
   float tol;               // 0: fixed max_iter iterations
   float inv_lamda;         // 1 / lamda
+  float lamda;
   double* wmd;             // nq x N
   int32_t* iters;          // nq x N
   float* resid;            // nq x N
@@ -79,7 +81,8 @@ constexpr int kDocClassWidth = 32;
 extern const TileShape kShapes[kNumShapes];
 
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
-hipError_t launch_resident(int shape, int doc_class, const SolveArgs& a, hipStream_t s);
+hipError_t launch_resident(int shape, int doc_class, bool log_domain, const SolveArgs& a,
+                           hipStream_t s);
 hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);

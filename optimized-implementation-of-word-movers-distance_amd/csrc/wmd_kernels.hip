@@ -34,6 +34,9 @@ const TileShape kShapes[kNumShapes] = {
 namespace {
 
 constexpr float kFltMin = 1.17549435e-38f;
+// Base-2 logit of an absent kernel entry: exp2 of it is 0, and sums with
+// finite logits stay finite (no inf - inf).
+constexpr float kLogZero = -1.0e30f;
 
 // Cross-lane sums without LDS round trips.  Each step adds a lane's value to
 // its partner's; both lanes of a pair compute the same two operands, so every
@@ -48,52 +51,63 @@ __device__ __forceinline__ float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-template <int MASK>
+template <bool MAX>
+__device__ __forceinline__ float combine(float a, float b) {
+  if constexpr (MAX) return fmaxf(a, b); else return a + b;
+}
+
+template <int MASK, bool MAX = false>
 __device__ __forceinline__ float xor_step(float v) {
   if constexpr (MASK == 1) {
-    return v + dpp_mov<0xB1>(v);  // quad_perm [1,0,3,2]
+    return combine<MAX>(v, dpp_mov<0xB1>(v));  // quad_perm [1,0,3,2]
   } else if constexpr (MASK == 2) {
-    return v + dpp_mov<0x4E>(v);  // quad_perm [2,3,0,1]
+    return combine<MAX>(v, dpp_mov<0x4E>(v));  // quad_perm [2,3,0,1]
   } else if constexpr (MASK == 4) {
-    return v + __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F));
+    return combine<MAX>(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F)));
   } else if constexpr (MASK == 8) {
-    return v + dpp_mov<0x128>(v);  // row_ror:8 == xor 8 inside a 16-lane row
+    return combine<MAX>(v, dpp_mov<0x128>(v));  // row_ror:8 == xor 8 inside a 16-lane row
   } else if constexpr (MASK == 16) {
     const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
                                                     false, false);
-    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    return combine<MAX>(__uint_as_float(p[0]), __uint_as_float(p[1]));
   } else {
     static_assert(MASK == 32, "xor masks are powers of two below 64");
     const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
                                                     false, false);
-    return __uint_as_float(p[0]) + __uint_as_float(p[1]);
+    return combine<MAX>(__uint_as_float(p[0]), __uint_as_float(p[1]));
   }
 }
 
 // Butterfly all-reduce over xor masks LO, 2LO, ..., HI (no-op when LO > HI).
-template <int LO, int HI>
-__device__ __forceinline__ float bfly_sum(float v) {
+template <int LO, int HI, bool MAX = false>
+__device__ __forceinline__ float bfly_red(float v) {
   if constexpr (LO <= HI) {
-    return bfly_sum<LO * 2, HI>(xor_step<LO>(v));
+    return bfly_red<LO * 2, HI, MAX>(xor_step<LO, MAX>(v));
   } else {
     return v;
   }
 }
 
+template <int LO, int HI>
+__device__ __forceinline__ float bfly_sum(float v) { return bfly_red<LO, HI, false>(v); }
+
 // All-reduce over aligned groups of G consecutive lanes.  After the xor-1 and
 // xor-2 steps every lane of a quad holds the same value, so the 8-lane step
 // can pair the two quads with DPP row_half_mirror (lane i <-> 7-i) instead of
 // an exact xor-4 swizzle through the LDS unit.
-template <int G>
-__device__ __forceinline__ float group_sum(float v) {
-  if constexpr (G >= 2) v = xor_step<1>(v);
-  if constexpr (G >= 4) v = xor_step<2>(v);
-  if constexpr (G >= 8) v = v + dpp_mov<0x141>(v);  // row_half_mirror
-  if constexpr (G >= 16) v = xor_step<8>(v);
-  if constexpr (G >= 32) v = xor_step<16>(v);
-  if constexpr (G >= 64) v = xor_step<32>(v);
+template <int G, bool MAX = false>
+__device__ __forceinline__ float group_red(float v) {
+  if constexpr (G >= 2) v = xor_step<1, MAX>(v);
+  if constexpr (G >= 4) v = xor_step<2, MAX>(v);
+  if constexpr (G >= 8) v = combine<MAX>(v, dpp_mov<0x141>(v));  // row_half_mirror
+  if constexpr (G >= 16) v = xor_step<8, MAX>(v);
+  if constexpr (G >= 32) v = xor_step<16, MAX>(v);
+  if constexpr (G >= 64) v = xor_step<32, MAX>(v);
   return v;
 }
+
+template <int G>
+__device__ __forceinline__ float group_sum(float v) { return group_red<G, false>(v); }
 
 // Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -206,7 +220,10 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
     const int t = t0 + tx * 4 + x;
     if (ty == 0 && t < a.Vt) mq[t] = v;
   }
-  // Epilogue: Kt = exp(lamda * (M - rowmin)); padding columns [q, ks) = 0.
+  // Epilogue, linear form: Kt = exp(lamda * (M - rowmin)), padding 0.
+  // Log form: Kt = lamda * (M - rowmin) * log2(e) (base-2 logits, <= 0),
+  // padding kLogZero.
+  const float l2 = a.lamda * 1.4426950408889634f;
   for (int qc = 0; qc < nqc; ++qc) {
 #pragma unroll
     for (int x = 0; x < 4; ++x)
@@ -215,11 +232,11 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
         const int i = qc * CD_QC + ty * 2 + y;
         const int t = t0 + tx * 4 + x;
         if (t < a.Vt && i < ks) {
-          float kv = 0.f;
+          float kv = a.log_out ? kLogZero : 0.f;
           if (i < q) {
             // Multi-chunk rows re-read the raw distance this thread stored.
             const float dd = (nqc == 1) ? dist[x][y] : Kq[(int64_t)t * ks + i];
-            kv = expf(a.lamda * (dd - m[x]));
+            kv = a.log_out ? l2 * (dd - m[x]) : expf(a.lamda * (dd - m[x]));
           }
           Kq[(int64_t)t * ks + i] = kv;
         }
@@ -382,6 +399,191 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     a.resid[out] = res;
   }
 }
+// ---------------------------------------------------------------------------
+// Log-domain resident solver (large lambda, where exp(lamda M) underflows
+// FP32 even after the row rescale).  Same lane grid and launch structure as
+// k_resident; the tile holds base-2 logits L[e][i] = lamda (M - mrow) log2(e)
+// and the scalings are kept as base-2 potentials f_i = log2 u_i,
+// g_e = log2 v_e.  Per iteration (the log-sum-exp form of the same step,
+// main.py:75-77; the oracle's perdoc_log):
+//   g_e = log2 c_e - LSE2_i(L[e,i] + f_i)
+//   lx_i = LSE2_e(L[e,i] + g_e)
+//   res = sum_i |2^(f_i + lx_i) - r_i|,  f_i = log2 r_i - lx_i
+// where LSE2 is a two-pass (max, then sum of exp2) reduction over the lane
+// groups.  The final sum of P_ie M_ie is formed in FP64 with
+// M = mrow + L ln2 / lamda.
+template <int W, int B, int NI, int NE>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArgs a) {
+  constexpr int AW = 64 / B;
+  constexpr int A = W * AW;
+  // Cross-wave maxima and sums.  Two barriers per iteration make single
+  // buffers safe: every reader of one buffer is done before the other
+  // buffer's barrier, which precedes the next write.
+  __shared__ float xsh[(W > 1) ? 2 * W * NI * B : 1];
+  __shared__ double dsh[(W > 1) ? W : 1];
+
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
+  if (pair >= a.n_pairs) return;
+  const int b = lane & (B - 1);
+  const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
+
+  const int Q = a.bucket_q[pair / a.n_list];
+  const int j = a.doc_list[pair % a.n_list];
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  const float* Lq = a.K + a.k_off[Q];
+  const int db = a.doc_ptr[j];
+  const int n = a.doc_ptr[j + 1] - db;
+
+  float L[NE][NI];
+  float lc[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = ea + A * k;
+    const bool ve = e < n;
+    int t = 0;
+    float l = kLogZero;
+    if (ve) {
+      t = a.doc_t[db + e];
+      l = __log2f(a.doc_val[db + e]);
+    }
+    lc[k] = l;
+    const float* row = Lq + (int64_t)t * ks;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      const int i = b + B * m;
+      L[k][m] = (ve && i < q) ? row[i] : kLogZero;
+    }
+  }
+  float f[NI], lr[NI], r[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) {
+    const int i = b + B * m;
+    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
+    lr[m] = (i < q) ? __log2f(r[m]) : kLogZero;
+    f[m] = (i < q) ? __log2f((float)q) : kLogZero;  // u = q
+  }
+
+  const bool use_tol = a.tol > 0.f;
+  int it = 0;
+  float res = 0.f;
+  float g[NE];
+  while (it < a.max_iter) {
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      float mx = kLogZero;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
+      mx = group_red<B, true>(mx);
+      float s = 0.f;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
+      s = group_sum<B>(s);
+      g[k] = (lc[k] > 0.5f * kLogZero) ? lc[k] - mx - __log2f(s) : kLogZero;
+    }
+    float lx[NI];
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      float mx = kLogZero;
+#pragma unroll
+      for (int k = 0; k < NE; ++k) mx = fmaxf(mx, L[k][m] + g[k]);
+      lx[m] = bfly_red<B, 32, true>(mx);
+    }
+    float* bufm = xsh;
+    float* bufs = xsh + W * NI * B;
+    if constexpr (W > 1) {
+      if (lane < B) {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) bufm[(wv * NI + m) * B + b] = lx[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        float x = kLogZero;
+#pragma unroll
+        for (int w = 0; w < W; ++w) x = fmaxf(x, bufm[(w * NI + m) * B + b]);
+        lx[m] = x;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      float t = 0.f;
+#pragma unroll
+      for (int k = 0; k < NE; ++k) t += exp2f(L[k][m] + g[k] - lx[m]);
+      t = bfly_sum<B, 32>(t);
+      if constexpr (W > 1) {
+        if (lane < B) bufs[(wv * NI + m) * B + b] = t;
+      } else {
+        lx[m] += __log2f(t);
+      }
+    }
+    if constexpr (W > 1) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) t += bufs[(w * NI + m) * B + b];
+        lx[m] += __log2f(t);
+      }
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m)
+      if (r[m] > 0.f) part += fabsf(exp2f(f[m] + lx[m]) - r[m]);
+    res = group_sum<B>(part);
+#pragma unroll
+    for (int m = 0; m < NI; ++m) f[m] = (r[m] > 0.f) ? lr[m] - lx[m] : kLogZero;
+    ++it;
+    if (use_tol && res < a.tol) break;
+  }
+
+  // Final half-step and sum_{i,e} P_ie M_ie in FP64.
+  const double ln2 = 0.6931471805599453;
+  const double m_per_l = ln2 / (double)a.lamda;
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    float mx = kLogZero;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
+    mx = group_red<B, true>(mx);
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
+    s = group_sum<B>(s);
+    const int e = ea + A * k;
+    if (e < n) {
+      const double ge = (double)lc[k] - (double)mx - (double)__log2f(s);
+      const double mr = (double)a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        if (r[m] > 0.f && L[k][m] > 0.5f * kLogZero) {
+          const double lp = (double)L[k][m] + (double)f[m] + ge;
+          acc += exp(lp * ln2) * (mr + (double)L[k][m] * m_per_l);
+        }
+      }
+    }
+  }
+  acc = bfly_sum_d<1, 32>(acc);
+  if constexpr (W > 1) {
+    if (lane == 0) dsh[wv] = acc;
+    __syncthreads();
+    acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc += dsh[w];
+  }
+  if (lane == 0 && (W == 1 || wv == 0)) {
+    const int64_t out = (int64_t)Q * a.N + j;
+    a.wmd[out] = (n > 0) ? acc : NAN;
+    a.iters[out] = it;
+    a.resid[out] = res;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Streaming variants: one wave (block of 64) per pair per launch.
 struct PairRef {
@@ -572,40 +774,45 @@ hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s) {
 }
 
 template <int W, int B, int NI, int NE>
-static hipError_t launch_res(const SolveArgs& a, hipStream_t s) {
+static hipError_t launch_res(bool log_domain, const SolveArgs& a, hipStream_t s) {
   const int64_t blocks = (W == 1) ? (a.n_pairs + 3) / 4 : a.n_pairs;
-  hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
-                     dim3(W == 1 ? 256 : 64 * W), 0, s, a);
+  if (log_domain)
+    hipLaunchKernelGGL((k_resident_log<W, B, NI, NE>), dim3((unsigned)blocks),
+                       dim3(W == 1 ? 256 : 64 * W), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
+                       dim3(W == 1 ? 256 : 64 * W), 0, s, a);
   return hipGetLastError();
 }
 
 // Document-length class c holds documents of at most kDocClassWidth*(c+1)
 // words: NE = that length over the shape's A document-word lanes.
 template <int W, int B, int NI>
-static hipError_t launch_shape(int cls, const SolveArgs& a, hipStream_t s) {
+static hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
   constexpr int A = 64 * W / B;
   constexpr int C = kDocClassWidth;
   static_assert(kNumDocClasses == 5, "one case per document-length class");
   switch (cls) {
-    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A>(a, s);
-    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A>(a, s);
-    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A>(a, s);
-    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A>(a, s);
-    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A>(a, s);
+    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A>(lg, a, s);
+    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A>(lg, a, s);
+    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A>(lg, a, s);
+    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A>(lg, a, s);
+    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A>(lg, a, s);
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t launch_resident(int shape, int doc_class, const SolveArgs& a, hipStream_t s) {
+hipError_t launch_resident(int shape, int doc_class, bool log_domain, const SolveArgs& a,
+                           hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
   switch (shape) {
-    case 0: return launch_shape<1, 2, 4>(doc_class, a, s);
-    case 1: return launch_shape<1, 4, 5>(doc_class, a, s);
-    case 2: return launch_shape<1, 8, 5>(doc_class, a, s);
-    case 3: return launch_shape<1, 8, 8>(doc_class, a, s);
-    case 4: return launch_shape<2, 16, 8>(doc_class, a, s);
-    case 5: return launch_shape<4, 32, 8>(doc_class, a, s);
-    case 6: return launch_shape<8, 64, 8>(doc_class, a, s);
+    case 0: return launch_shape<1, 2, 4>(doc_class, log_domain, a, s);
+    case 1: return launch_shape<1, 4, 5>(doc_class, log_domain, a, s);
+    case 2: return launch_shape<1, 8, 5>(doc_class, log_domain, a, s);
+    case 3: return launch_shape<1, 8, 8>(doc_class, log_domain, a, s);
+    case 4: return launch_shape<2, 16, 8>(doc_class, log_domain, a, s);
+    case 5: return launch_shape<4, 32, 8>(doc_class, log_domain, a, s);
+    case 6: return launch_shape<8, 64, 8>(doc_class, log_domain, a, s);
     default: return hipErrorInvalidValue;
   }
 }

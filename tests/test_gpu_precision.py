@@ -1,0 +1,105 @@
+"""Precision form: lambda = 50 with near-duplicate query words, iterated in
+the log domain to marginal residual 1e-4, against the FP64 log-domain
+oracle stopped by the same per-pair rule.  Tolerance: 1e-4 relative per
+document (FP32 potentials of magnitude up to ~100 carry ~1e-5 absolute
+error into the plan's exponents)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, pkg, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@pytest.fixture(scope="module")
+def P():
+    return pkg()
+
+
+@pytest.fixture(scope="module")
+def precision(dg, P):
+    corpus, qs, p = dg.workload("precision", seed=0)
+    e = P.WMDEngine(0)
+    e.set_corpus(corpus.vecs, corpus.word_major())
+    e.set_queries(qs.q_ptr, qs.words, qs.weights)
+    yield corpus, qs, p, e
+    e.close()
+
+
+def test_precision_workload_log_domain(precision, oracle):
+    corpus, qs, p, e = precision
+    say("[case] precision workload lambda=50, log domain, residual 1e-4")
+    st = e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"], log_domain=True)
+    w, it, res = e.result()
+    wo, ito, _ = oracle.perdoc(*qs.query(0), corpus.vecs.astype(np.float64), corpus.doc_ptr,
+                               corpus.doc_words, corpus.doc_vals, p["lamda"], p["max_iter"],
+                               tol=p["tol"], log_domain=True)
+    err = np.abs(w[0] - wo) / wo
+    same = it[0] == ito
+    say(f"  iters mean {it.mean():.1f} max {it.max()} (oracle {ito.mean():.1f}); "
+        f"per-doc rel err max {err.max():.2e} median {np.median(err):.2e}; "
+        f"same-iteration pairs {same.mean():.3f} err {err[same].max():.2e}; {st['ms_total']:.2f} ms")
+    assert np.all(np.isfinite(w))
+    assert np.all(res < p["tol"])
+    assert np.all(np.abs(it[0] - ito) <= 2)
+    assert err.max() <= 1e-4
+
+
+def test_near_duplicates_are_in_the_query(precision):
+    corpus, qs, _, _ = precision
+    words, _ = qs.query(0)
+    v = corpus.vecs.astype(np.float64)[words]
+    d = np.sqrt(((v[:, None, :] - v[None, :, :]) ** 2).sum(-1)) + np.eye(len(words)) * 1e9
+    assert np.sum(d.min(1) < 1e-2) >= 2  # duplicated pairs at ~1.7e-3
+
+
+def test_log_domain_matches_reference_at_lambda10(P, dg):
+    say("[case] log domain, plain workload, 15 fixed iterations vs reference")
+    corpus, qs, p = dg.workload("plain", seed=0)
+    gold = load_golden("plain")["wmd_fused"]
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(p["lamda"], max_iter=15, log_domain=True)
+        w, it, _ = e.result()
+        e.run(p["lamda"], max_iter=15)
+        wl = e.result()[0]
+    say(f"  log vs reference {rel_err(w, gold):.2e}; linear vs reference {rel_err(wl, gold):.2e}")
+    assert (it == 15).all()
+    assert rel_err(w, gold) <= 1e-5
+
+
+def test_log_domain_every_tile_shape(P, dg, oracle):
+    say("[case] log domain across tile shapes, lambda=30")
+    corpus, _, _ = dg.workload("batched", seed=1, N=400, n_queries=1)
+    rng = np.random.default_rng(12)
+    lengths = [3, 8, 20, 40, 64, 100, 200, 300]
+    qs = dg.make_queries(rng, corpus.V, lengths)
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-30.0, max_iter=15, log_domain=True)
+        w = e.result()[0]
+    wo, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, corpus.vecs.astype(np.float64),
+                                   corpus.doc_ptr, corpus.doc_words, corpus.doc_vals, -30.0, 15,
+                                   log_domain=True)
+    for k, q in enumerate(lengths):
+        err = rel_err(w[k], wo[k])
+        say(f"  q={q:3d} err {err:.2e}")
+        assert err <= 2e-5, q
+
+
+def test_log_domain_needs_resident_variant(P, dg):
+    corpus, qs, p = dg.workload("tiny", seed=0)
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        with pytest.raises(P.WMDError) as ei:
+            e.run(-50.0, variant=P.VARIANT_FUSED, log_domain=True)
+        assert ei.value.status == P._native.WMD_ERR_UNSUPPORTED
