@@ -63,6 +63,15 @@ enum wmd_domain {
   WMD_DOMAIN_LOG = 1
 };
 
+enum wmd_cdist {
+  /* FP32 direct differences sum_k (a_k - b_k)^2 on the vector ALUs */
+  WMD_CDIST_FP32 = 0,
+  /* |a|^2 + |b|^2 - 2 a.b with a.b on the matrix cores (three-way BF16 split,
+   * FP32 accumulation), FP64 norms and cancellation, FP64 direct differences
+   * for near-identical pairs */
+  WMD_CDIST_MFMA = 1
+};
+
 typedef struct wmd_options {
   int32_t max_iter;    /* iteration cap; the reference runs exactly 15 (:12) */
   int32_t variant;     /* enum wmd_variant */
@@ -76,7 +85,8 @@ typedef struct wmd_options {
                           device-to-host copy of per-pair counters); 0: stage
                           times only */
   int32_t domain;      /* enum wmd_domain */
-  int32_t reserved[4];
+  int32_t cdist;       /* enum wmd_cdist */
+  int32_t reserved[3];
 } wmd_options;
 
 typedef struct wmd_stats {

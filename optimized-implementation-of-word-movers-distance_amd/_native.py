@@ -27,6 +27,9 @@ VARIANT_NONFUSED = 2
 DOMAIN_LINEAR = 0
 DOMAIN_LOG = 1
 
+CDIST_FP32 = 0
+CDIST_MFMA = 1
+
 # Every symbol include/wmd.h declares.
 EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
            "wmd_context_create", "wmd_context_destroy", "wmd_context_set_corpus",
@@ -37,7 +40,7 @@ EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("variant", C.c_int32), ("tol", C.c_double),
                 ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
-                ("domain", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("domain", C.c_int32), ("cdist", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):

@@ -122,13 +122,14 @@ class WMDEngine:
         self.n_queries = n
 
     def run(self, lamda, max_iter=15, tol=0.0, variant=nat.VARIANT_RESIDENT, check_every=8,
-            shape=-1, summarize=True, log_domain=False) -> dict:
+            shape=-1, summarize=True, log_domain=False, mfma_cdist=False) -> dict:
         """Solve every (query, doc) pair; returns the run's wmd_stats as a dict
         (with summarize=False only stage times, no device-to-host copy)."""
         o = nat.default_options(max_iter=int(max_iter), tol=float(tol), variant=int(variant),
                                 check_every=int(check_every), shape=int(shape),
                                 summarize=int(bool(summarize)),
-                                domain=nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR)
+                                domain=nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR,
+                                cdist=nat.CDIST_MFMA if mfma_cdist else nat.CDIST_FP32)
         st = nat.Stats()
         nat.check(self._L.wmd_context_run(self._h, float(lamda), C.byref(o), C.byref(st)),
                   "wmd_context_run")

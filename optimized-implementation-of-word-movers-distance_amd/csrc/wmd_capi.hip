@@ -83,6 +83,11 @@ struct wmd_context {
   DevBuf<float> E;
   DevBuf<int32_t> touched, doc_ptr, doc_t;
   DevBuf<int32_t> doc_perm;             // documents sorted by length class
+  DevBuf<uint16_t> Es;                  // matrix-core stage: BF16 planes of touched rows
+  DevBuf<double> nt;                    // FP64 squared norms of touched rows
+  bool split_ready = false;
+  DevBuf<uint16_t> Qs;                  // BF16 planes of query words
+  DevBuf<double> nqw;                   // FP64 squared norms of query words
   std::vector<int64_t> h_class_off;     // kNumDocClasses + 1 offsets into doc_perm
   DevBuf<float> doc_val;
 
@@ -375,6 +380,7 @@ void wmd_options_default(wmd_options* opt) {
   opt->shape = -1;
   opt->summarize = 1;
   opt->domain = WMD_DOMAIN_LINEAR;
+  opt->cdist = WMD_CDIST_FP32;
 }
 
 const char* wmd_last_error(void) { return g_err.c_str(); }
@@ -429,6 +435,7 @@ void wmd_context_destroy(wmd_context* c) {
   c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
   c->doc_perm.release();
+  c->Es.release(); c->nt.release(); c->Qs.release(); c->nqw.release();
   c->status.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -516,6 +523,7 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   c->nnz = (int64_t)nnz;
   c->Vt = (int64_t)touched.size();
   c->nmax = nmax;
+  c->split_ready = false;
   c->has_corpus = true;
   c->has_queries = false;  // query blocks depend on Vt and N
   c->ran = false;
@@ -632,6 +640,29 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     return WMD_ERR_ARG;
   }
   const bool log_domain = o.domain == WMD_DOMAIN_LOG;
+  if (o.cdist != WMD_CDIST_FP32 && o.cdist != WMD_CDIST_MFMA) {
+    set_err("unknown cdist mode %d", o.cdist);
+    return WMD_ERR_ARG;
+  }
+  const bool mfma = o.cdist == WMD_CDIST_MFMA;
+  const int Dp = ((int)c->D + 15) & ~15;
+  if (mfma && !c->split_ready) {
+    // Once per corpus, outside the timed stages: BF16 planes and FP64 norms
+    // of every touched word.
+    HIP_TRY(c->Es.ensure((size_t)3 * c->Vt * Dp));
+    HIP_TRY(c->nt.ensure((size_t)c->Vt));
+    wmd::SplitArgs sa{};
+    sa.E = c->E.p;
+    sa.ids = c->touched.p;
+    sa.rows = (int)c->Vt;
+    sa.D = (int)c->D;
+    sa.Dp = Dp;
+    sa.out = c->Es.p;
+    sa.norms = c->nt.p;
+    HIP_TRY(wmd::launch_split_rows(sa, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->split_ready = true;
+  }
   if (variant == WMD_VARIANT_RESIDENT) {
     bool fits = true;
     for (int Q = 0; Q < c->nq && fits; ++Q) {
@@ -680,8 +711,44 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
 
   int launches = 0;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-  HIP_TRY(wmd::launch_cdist(ca, c->nq, c->stream));
-  ++launches;
+  if (mfma) {
+    const int sq = c->h_q_ptr[c->nq];
+    HIP_TRY(c->Qs.ensure((size_t)3 * sq * Dp));
+    HIP_TRY(c->nqw.ensure((size_t)sq));
+    wmd::SplitArgs sa{};
+    sa.E = c->E.p;
+    sa.ids = c->q_words.p;
+    sa.rows = sq;
+    sa.D = (int)c->D;
+    sa.Dp = Dp;
+    sa.out = c->Qs.p;
+    sa.norms = c->nqw.p;
+    HIP_TRY(wmd::launch_split_rows(sa, c->stream));
+    wmd::MfmaCdistArgs ma{};
+    ma.Es = c->Es.p;
+    ma.Qs = c->Qs.p;
+    ma.nt = c->nt.p;
+    ma.nq = c->nqw.p;
+    ma.E = c->E.p;
+    ma.touched = c->touched.p;
+    ma.q_words = c->q_words.p;
+    ma.q_ptr = c->q_ptr.p;
+    ma.k_off = c->k_off.p;
+    ma.k_stride = c->k_stride.p;
+    ma.K = c->K.p;
+    ma.mrow = c->mrow.p;
+    ma.D = (int)c->D;
+    ma.Dp = Dp;
+    ma.Vt = (int)c->Vt;
+    ma.Sq = sq;
+    ma.lamda = (float)lamda;
+    ma.log_out = log_domain ? 1 : 0;
+    HIP_TRY(wmd::launch_cdist_mfma(ma, c->nq, c->stream));
+    launches += 2;
+  } else {
+    HIP_TRY(wmd::launch_cdist(ca, c->nq, c->stream));
+    ++launches;
+  }
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   wmd::SolveArgs a = base_args(c, lamda, o);
   if (variant == WMD_VARIANT_RESIDENT)

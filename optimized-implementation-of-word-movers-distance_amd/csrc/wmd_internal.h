@@ -30,6 +30,36 @@ struct CdistArgs {
   int log_out;             // 1: store base-2 logits lamda (M - mrow) log2(e) instead
 };
 
+// BF16 three-way split of embedding rows for the matrix-core distance stage.
+struct SplitArgs {
+  const float* E;          // V x D
+  const int32_t* ids;      // rows: vocabulary ids
+  int rows;
+  int D;
+  int Dp;                  // D rounded up to 16 (zero padding)
+  uint16_t* out;           // [3][rows][Dp] BF16 bits: hi, mid, lo
+  double* norms;           // rows: FP64 |x|^2
+};
+
+// Matrix-core distance stage: same outputs as CdistArgs' stage.
+struct MfmaCdistArgs {
+  const uint16_t* Es;      // [3][Vt][Dp] touched-word planes
+  const uint16_t* Qs;      // [3][Sq][Dp] query-word planes (query-major, as q_words)
+  const double* nt;        // Vt
+  const double* nq;        // Sq
+  const float* E;          // V x D (direct FP64 recomputation of near pairs)
+  const int32_t* touched;
+  const int32_t* q_words;
+  const int32_t* q_ptr;
+  const int64_t* k_off;
+  const int32_t* k_stride;
+  float* K;
+  float* mrow;
+  int D, Dp, Vt, Sq;
+  float lamda;
+  int log_out;
+};
+
 // Sinkhorn stage.  A pair is (query, document); pairs of one launch are
 // bucket_q[p / N] x (p % N), query-major.
 struct SolveArgs {
@@ -81,6 +111,8 @@ constexpr int kDocClassWidth = 32;
 extern const TileShape kShapes[kNumShapes];
 
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
+hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s);
+hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s);
 hipError_t launch_resident(int shape, int doc_class, bool log_domain, const SolveArgs& a,
                            hipStream_t s);
 hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);

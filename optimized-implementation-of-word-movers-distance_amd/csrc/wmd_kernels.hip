@@ -37,6 +37,10 @@ constexpr float kFltMin = 1.17549435e-38f;
 // Base-2 logit of an absent kernel entry: exp2 of it is 0, and sums with
 // finite logits stay finite (no inf - inf).
 constexpr float kLogZero = -1.0e30f;
+// Matrix-core distances: d^2 below this fraction of |a|^2 + |b|^2 is
+// recomputed directly in FP64 (the GEMM form's cancellation error is
+// ~1e-7 (|a|^2 + |b|^2)).
+constexpr double kNearTau = 0.02;
 
 // Cross-lane sums without LDS round trips.  Each step adds a lane's value to
 // its partner's; both lanes of a pair compute the same two operands, so every
@@ -51,63 +55,68 @@ __device__ __forceinline__ float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-template <bool MAX>
+// Reduction operators of the cross-lane helpers.
+enum RedOp { kSum = 0, kMax = 1, kMin = 2 };
+
+template <int OP>
 __device__ __forceinline__ float combine(float a, float b) {
-  if constexpr (MAX) return fmaxf(a, b); else return a + b;
+  if constexpr (OP == kMax) return fmaxf(a, b);
+  else if constexpr (OP == kMin) return fminf(a, b);
+  else return a + b;
 }
 
-template <int MASK, bool MAX = false>
+template <int MASK, int OP = kSum>
 __device__ __forceinline__ float xor_step(float v) {
   if constexpr (MASK == 1) {
-    return combine<MAX>(v, dpp_mov<0xB1>(v));  // quad_perm [1,0,3,2]
+    return combine<OP>(v, dpp_mov<0xB1>(v));  // quad_perm [1,0,3,2]
   } else if constexpr (MASK == 2) {
-    return combine<MAX>(v, dpp_mov<0x4E>(v));  // quad_perm [2,3,0,1]
+    return combine<OP>(v, dpp_mov<0x4E>(v));  // quad_perm [2,3,0,1]
   } else if constexpr (MASK == 4) {
-    return combine<MAX>(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F)));
+    return combine<OP>(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F)));
   } else if constexpr (MASK == 8) {
-    return combine<MAX>(v, dpp_mov<0x128>(v));  // row_ror:8 == xor 8 inside a 16-lane row
+    return combine<OP>(v, dpp_mov<0x128>(v));  // row_ror:8 == xor 8 inside a 16-lane row
   } else if constexpr (MASK == 16) {
     const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
                                                     false, false);
-    return combine<MAX>(__uint_as_float(p[0]), __uint_as_float(p[1]));
+    return combine<OP>(__uint_as_float(p[0]), __uint_as_float(p[1]));
   } else {
     static_assert(MASK == 32, "xor masks are powers of two below 64");
     const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
                                                     false, false);
-    return combine<MAX>(__uint_as_float(p[0]), __uint_as_float(p[1]));
+    return combine<OP>(__uint_as_float(p[0]), __uint_as_float(p[1]));
   }
 }
 
 // Butterfly all-reduce over xor masks LO, 2LO, ..., HI (no-op when LO > HI).
-template <int LO, int HI, bool MAX = false>
+template <int LO, int HI, int OP = kSum>
 __device__ __forceinline__ float bfly_red(float v) {
   if constexpr (LO <= HI) {
-    return bfly_red<LO * 2, HI, MAX>(xor_step<LO, MAX>(v));
+    return bfly_red<LO * 2, HI, OP>(xor_step<LO, OP>(v));
   } else {
     return v;
   }
 }
 
 template <int LO, int HI>
-__device__ __forceinline__ float bfly_sum(float v) { return bfly_red<LO, HI, false>(v); }
+__device__ __forceinline__ float bfly_sum(float v) { return bfly_red<LO, HI, kSum>(v); }
 
 // All-reduce over aligned groups of G consecutive lanes.  After the xor-1 and
 // xor-2 steps every lane of a quad holds the same value, so the 8-lane step
 // can pair the two quads with DPP row_half_mirror (lane i <-> 7-i) instead of
 // an exact xor-4 swizzle through the LDS unit.
-template <int G, bool MAX = false>
+template <int G, int OP = kSum>
 __device__ __forceinline__ float group_red(float v) {
-  if constexpr (G >= 2) v = xor_step<1, MAX>(v);
-  if constexpr (G >= 4) v = xor_step<2, MAX>(v);
-  if constexpr (G >= 8) v = combine<MAX>(v, dpp_mov<0x141>(v));  // row_half_mirror
-  if constexpr (G >= 16) v = xor_step<8, MAX>(v);
-  if constexpr (G >= 32) v = xor_step<16, MAX>(v);
-  if constexpr (G >= 64) v = xor_step<32, MAX>(v);
+  if constexpr (G >= 2) v = xor_step<1, OP>(v);
+  if constexpr (G >= 4) v = xor_step<2, OP>(v);
+  if constexpr (G >= 8) v = combine<OP>(v, dpp_mov<0x141>(v));  // row_half_mirror
+  if constexpr (G >= 16) v = xor_step<8, OP>(v);
+  if constexpr (G >= 32) v = xor_step<16, OP>(v);
+  if constexpr (G >= 64) v = xor_step<32, OP>(v);
   return v;
 }
 
 template <int G>
-__device__ __forceinline__ float group_sum(float v) { return group_red<G, false>(v); }
+__device__ __forceinline__ float group_sum(float v) { return group_red<G, kSum>(v); }
 
 // Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
@@ -477,7 +486,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
       float mx = kLogZero;
 #pragma unroll
       for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
-      mx = group_red<B, true>(mx);
+      mx = group_red<B, kMax>(mx);
       float s = 0.f;
 #pragma unroll
       for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
@@ -490,7 +499,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
       float mx = kLogZero;
 #pragma unroll
       for (int k = 0; k < NE; ++k) mx = fmaxf(mx, L[k][m] + g[k]);
-      lx[m] = bfly_red<B, 32, true>(mx);
+      lx[m] = bfly_red<B, 32, kMax>(mx);
     }
     float* bufm = xsh;
     float* bufs = xsh + W * NI * B;
@@ -550,7 +559,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
     float mx = kLogZero;
 #pragma unroll
     for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
-    mx = group_red<B, true>(mx);
+    mx = group_red<B, kMax>(mx);
     float s = 0.f;
 #pragma unroll
     for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
@@ -582,6 +591,160 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
     a.iters[out] = it;
     a.resid[out] = res;
   }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA distance stage.  d^2 = |a|^2 + |b|^2 - 2 a.b with a.b on the matrix
+// cores: each FP32 component is split into three BF16 parts
+// x = hi + mid + lo (hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi -
+// mid); both differences are exact in FP32), and the six products with
+// weight >= 2^-16 (hh, hm, mh, hl, mm, lh) are accumulated in FP32 by
+// v_mfma_f32_32x32x16_bf16, which keeps a.b to ~2^-24 relative.  Norms are
+// FP64 sums of squares, the cancellation |a|^2 + |b|^2 - 2 a.b is formed in
+// FP64, and pairs whose d^2 falls below kNearTau (|a|^2 + |b|^2) — near-
+// identical words, where the cancellation eats the precision — are
+// recomputed as FP64 direct differences.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// One wave per row: three BF16 planes [3][rows][Dp] (zero past D) and the
+// FP64 squared norm of each row.
+__global__ __launch_bounds__(256) void k_split_rows(SplitArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (row >= a.rows) return;
+  const float* src = a.E + (int64_t)a.ids[row] * a.D;
+  const size_t plane = (size_t)a.rows * a.Dp;
+  uint16_t* dst = a.out + (size_t)row * a.Dp;
+  double ss = 0.0;
+  for (int k = lane; k < a.Dp; k += 64) {
+    const float x = (k < a.D) ? src[k] : 0.f;
+    const uint16_t hi = bf16_rne(x);
+    const float r1 = x - bf16_to_f32(hi);
+    const uint16_t mid = bf16_rne(r1);
+    const uint16_t lo = bf16_rne(r1 - bf16_to_f32(mid));
+    dst[k] = hi;
+    dst[plane + k] = mid;
+    dst[2 * plane + k] = lo;
+    ss += (double)x * (double)x;
+  }
+  ss = bfly_sum_d<1, 32>(ss);
+  if (lane == 0) a.norms[row] = ss;
+}
+
+__device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p, bool valid) {
+  u16x8 v = {};
+  if (valid) v = *reinterpret_cast<const u16x8*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// Block = 4 waves = 64 touched words x 64 query words of one query; wave
+// (wy, wx) owns the 32 x 32 sub-tile.  v_mfma_f32_32x32x16_bf16 operand
+// layout: lane l supplies row (l % 32) of A and column (l % 32) of B with
+// the 8 consecutive k values 8 (l / 32) .. +7 of each 16-wide k step;
+// accumulator r of lane l is C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+// Queries longer than 64 words loop over 64-column chunks; raw distances go
+// to Kt first and the row-minimum rescale runs after the last chunk, each
+// thread re-reading only what it wrote.
+__global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
+  const int Q = blockIdx.y;
+  const int t0 = blockIdx.x * 64;
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  float* Kq = a.K + a.k_off[Q];
+  float* mq = a.mrow + (int64_t)Q * a.Vt;
+  __shared__ unsigned int smin[64];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wy = w >> 1, wx = w & 1;
+  if (tid < 64) smin[tid] = 0x7F800000u;  // +inf
+  __syncthreads();
+
+  const int Dp = a.Dp;
+  const size_t tplane = (size_t)a.Vt * Dp;
+  const size_t qplane = (size_t)a.Sq * Dp;
+  const int ta = t0 + wy * 32 + (lane & 31);  // A row this lane supplies
+  const bool tav = ta < a.Vt;
+  const int kh = (lane >> 5) * 8;
+  const uint16_t* ap = a.Es + (size_t)(tav ? ta : 0) * Dp + kh;
+
+  for (int qc = 0; qc < q; qc += 64) {
+    const int jb = qc + wx * 32 + (lane & 31);  // B column (query word) this lane supplies
+    const bool jbv = jb < q;
+    const uint16_t* bp = a.Qs + (size_t)(qb + (jbv ? jb : 0)) * Dp + kh;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < Dp; k0 += 16) {
+      const bf16x8 a0 = load_bf16x8(ap + k0, tav);
+      const bf16x8 a1 = load_bf16x8(ap + tplane + k0, tav);
+      const bf16x8 a2 = load_bf16x8(ap + 2 * tplane + k0, tav);
+      const bf16x8 b0 = load_bf16x8(bp + k0, jbv);
+      const bf16x8 b1 = load_bf16x8(bp + qplane + k0, jbv);
+      const bf16x8 b2 = load_bf16x8(bp + 2 * qplane + k0, jbv);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b2, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc, 0, 0, 0);
+    }
+    const int jo = qc + wx * 32 + (lane & 31);  // output column of this lane
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int t = t0 + wy * 32 + i;
+      float d = INFINITY;
+      if (t < a.Vt && jo < q) {
+        const double na = a.nt[t], nb = a.nq[qb + jo];
+        double d2 = na + nb - 2.0 * (double)acc[r];
+        if (d2 < kNearTau * (na + nb)) {
+          const float* x = a.E + (int64_t)a.touched[t] * a.D;
+          const float* y = a.E + (int64_t)a.q_words[qb + jo] * a.D;
+          double s = 0.0;
+          for (int k = 0; k < a.D; ++k) {
+            const double df = (double)x[k] - (double)y[k];
+            s += df * df;
+          }
+          d2 = s;
+        }
+        d = (float)sqrt(d2 > 0.0 ? d2 : 0.0);
+        Kq[(int64_t)t * ks + jo] = d;  // raw distance, rescaled below
+      }
+      // Row minimum over this wave's 32 columns, then across waves/chunks.
+      const float mn = bfly_red<1, 16, kMin>(d);
+      if ((lane & 31) == 0) atomicMin(&smin[wy * 32 + i], __float_as_uint(mn));
+    }
+  }
+  __syncthreads();
+  const float l2 = a.lamda * 1.4426950408889634f;
+  for (int qc = 0; qc < ks; qc += 64) {
+    const int jo = qc + wx * 32 + (lane & 31);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int t = t0 + wy * 32 + i;
+      if (t < a.Vt && jo < ks) {
+        float kv = a.log_out ? kLogZero : 0.f;
+        if (jo < q) {
+          const float dd = Kq[(int64_t)t * ks + jo];
+          const float m = __uint_as_float(smin[wy * 32 + i]);
+          kv = a.log_out ? l2 * (dd - m) : expf(a.lamda * (dd - m));
+        }
+        Kq[(int64_t)t * ks + jo] = kv;
+      }
+    }
+  }
+  if (tid < 64 && t0 + tid < a.Vt) mq[t0 + tid] = __uint_as_float(smin[tid]);
 }
 
 // ---------------------------------------------------------------------------
@@ -766,6 +929,18 @@ __global__ void k_zero_status(uint32_t* status) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s) {
+  if (a.rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_split_rows, dim3((a.rows + 3) / 4), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
+  if (a.Vt <= 0 || nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cdist_mfma, dim3((a.Vt + 63) / 64, nq), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s) {
   if (a.Vt <= 0 || nq <= 0) return hipSuccess;
   dim3 grid((a.Vt + CD_WT - 1) / CD_WT, nq);

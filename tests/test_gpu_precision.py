@@ -103,3 +103,82 @@ def test_log_domain_needs_resident_variant(P, dg):
         with pytest.raises(P.WMDError) as ei:
             e.run(-50.0, variant=P.VARIANT_FUSED, log_domain=True)
         assert ei.value.status == P._native.WMD_ERR_UNSUPPORTED
+
+
+# ---------------------------------------------------------------------------
+# Matrix-core distance stage (three-way BF16 split, FP32 accumulation, FP64
+# cancellation and near-pair correction).
+
+def test_mfma_cdist_matches_reference(P, dg):
+    say("[case] MFMA cdist, plain workload, 15 fixed iterations vs reference")
+    corpus, qs, p = dg.workload("plain", seed=0)
+    gold = load_golden("plain")["wmd_fused"]
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(p["lamda"], max_iter=15, mfma_cdist=True)
+        w = e.result()[0]
+    say(f"  MFMA cdist vs reference {rel_err(w, gold):.2e}")
+    assert rel_err(w, gold) <= 1e-5
+
+
+def test_mfma_cdist_agrees_with_fp32_every_shape(P, dg):
+    say("[case] MFMA vs FP32 distance stage across query lengths 1..300")
+    corpus, _, _ = dg.workload("batched", seed=1, N=1200, n_queries=1)
+    rng = np.random.default_rng(21)
+    lengths = [1, 7, 31, 33, 63, 64, 65, 127, 129, 200, 300]
+    qs = dg.make_queries(rng, corpus.V, lengths)
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=15)
+        w32 = e.result()[0]
+        st = e.run(-10.0, max_iter=15, mfma_cdist=True)
+        wmf = e.result()[0]
+        e.run(-10.0, max_iter=15, mfma_cdist=True)
+        assert np.array_equal(e.result()[0], wmf)  # deterministic
+    for k, q in enumerate(lengths):
+        err = rel_err(wmf[k], w32[k])
+        say(f"  q={q:3d} MFMA vs FP32 {err:.2e}")
+        assert err <= 5e-6, q
+    say(f"  cdist {st['ms_cdist']:.3f} ms")
+
+
+def test_mfma_cdist_near_duplicates(precision, oracle):
+    corpus, qs, p, e = precision
+    say("[case] MFMA cdist on the near-duplicate precision workload (log domain)")
+    e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"], log_domain=True, mfma_cdist=True)
+    w, it, res = e.result()
+    wo, ito, _ = oracle.perdoc(*qs.query(0), corpus.vecs.astype(np.float64), corpus.doc_ptr,
+                               corpus.doc_words, corpus.doc_vals, p["lamda"], p["max_iter"],
+                               tol=p["tol"], log_domain=True)
+    err = np.abs(w[0] - wo) / wo
+    say(f"  per-doc rel err max {err.max():.2e} median {np.median(err):.2e}")
+    assert np.all(res < p["tol"])
+    assert err.max() <= 1e-4
+
+
+def test_reference_sample_d3_on_gpu(P, oracle):
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "sample_d3.json")) as f:
+        s = json.load(f)
+    V, N = s["V"], s["N"]
+    t = np.array(s["triples"], dtype=np.int64) - [1, 1, 0]
+    t = t[np.lexsort((t[:, 1], t[:, 0]))]
+    row_ptr = np.zeros(V + 1, dtype=np.uint64)
+    row_ptr[1:] = np.cumsum(np.bincount(t[:, 0], minlength=V))
+    csr = (row_ptr, t[:, 1].astype(np.uint64), t[:, 2].astype(np.float64), N)
+    r = np.array(s["r"])
+    vecs = np.array(s["vecs"])
+    ref = oracle.fused(r, vecs, csr, -1.0, 15)
+    sel = np.nonzero(r)[0]
+    for mf in (False, True):
+        with P.WMDEngine(0) as e:
+            e.set_corpus(vecs, csr)
+            e.set_queries([0, len(sel)], sel, r[sel])
+            e.run(-1.0, max_iter=15, mfma_cdist=mf)
+            w = e.result()[0][0]
+        say(f"  D=3 sample, mfma={mf}: {rel_err(w, ref):.2e}")
+        assert rel_err(w, ref) <= 1e-5
