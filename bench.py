@@ -136,7 +136,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--plain-queries", type=int, default=8)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cdist", choices=("mfma", "fp32"), default="mfma")
     args = ap.parse_args()
+    mfma = args.cdist == "mfma"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} (WORLD_SIZE={world}) is not "
@@ -159,13 +161,13 @@ def main() -> int:
 
     warm = None
     for w in range(max(1, args.warmup)):
-        warm = eng.run(lamda, max_iter=max_iter, tol=args.tol, summarize=True)
+        warm = eng.run(lamda, max_iter=max_iter, tol=args.tol, summarize=True, mfma_cdist=mfma)
         log(f"warmup {w}: {warm['ms_total']:.2f} ms (cdist {warm['ms_cdist']:.2f}, solve "
             f"{warm['ms_solve']:.2f}), iters mean {warm['iters_mean']:.2f} max {warm['iters_max']}")
     cd = sv = 0.0
     t_start = time.perf_counter()
     for s in range(args.steps):
-        st = eng.run(lamda, max_iter=max_iter, tol=args.tol, summarize=False)
+        st = eng.run(lamda, max_iter=max_iter, tol=args.tol, summarize=False, mfma_cdist=mfma)
         cd += st["ms_cdist"]
         sv += st["ms_solve"]
     t_end = time.perf_counter()
@@ -184,6 +186,20 @@ def main() -> int:
     it_q = iters.mean(axis=1)
     bytes_alg = sinkhorn_bytes(corpus.V, corpus.nnz, vt, qs.lengths(), corpus.N, it_q)
 
+    cd_ms = cd / args.steps
+    dp = (corpus.D + 15) // 16 * 16
+    q_len = qs.lengths()
+    cdist_alg = 2.0 * vt * float(q_len.sum()) * corpus.D  # one FMA per component
+    cdist_issued = 6 * 2.0 * vt * float((np.ceil(q_len / 64.0) * 64).sum()) * dp
+    cdist_obj = {"mode": args.cdist, "ms_per_step": cd_ms,
+                 "alg_tflops": cdist_alg / (cd_ms * 1e-3) / 1e12}
+    if mfma:
+        cdist_obj.update({"bound": "mfma", "mfma_issued_tflops":
+                          cdist_issued / (cd_ms * 1e-3) / 1e12, "peak": 2500.0,
+                          "unit": "TFLOP/s (BF16 dense)",
+                          "frac": cdist_issued / (cd_ms * 1e-3) / 1e12 / 2500.0,
+                          "note": "6 BF16 products per FP32 dot (three-way split); issued "
+                                  "FLOPs count 64-column query tiles and D padded to 16"})
     plain = None
     if not args.no_plain and args.plain_queries > 0:
         nq = min(args.plain_queries, qs.n)
@@ -192,10 +208,10 @@ def main() -> int:
             wk, vk = qs.query(k)
             eng.set_queries(np.array([0, wk.shape[0]]), wk, vk)
             eng.run(lamda, max_iter=max_iter, tol=args.tol, variant=P.VARIANT_FUSED,
-                    check_every=4, summarize=False)  # graphs built, caches warm
+                    check_every=4, summarize=False, mfma_cdist=mfma)  # graphs built, caches warm
             t1 = time.perf_counter()
             eng.run(lamda, max_iter=max_iter, tol=args.tol, variant=P.VARIANT_FUSED,
-                    check_every=4, summarize=False)
+                    check_every=4, summarize=False, mfma_cdist=mfma)
             tp += time.perf_counter() - t1
         plain = {"value": nq * corpus.N / tp, "unit": "pairs/s",
                  "method": "one query per call, fused SDDMM+SpMM launch per iteration from a "
@@ -238,6 +254,7 @@ def main() -> int:
             "sinkhorn_alg_gbs": bytes_alg / (solve_ms * 1e-3) / 1e9,
         },
         "stages_ms": {"cdist": cd / args.steps, "solve": solve_ms},
+        "cdist_stage": cdist_obj,
         "cpu_baseline": cpu,
         "cpu_port_per_pair_stop": cpu_port,
         "plain_gpu": plain,

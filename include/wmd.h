@@ -103,7 +103,7 @@ typedef struct wmd_stats {
 } wmd_stats;
 
 /* Default options: max_iter 15, resident variant, tol 0, automatic shape,
- * summarize 1. */
+ * summarize 1, linear domain, matrix-core distance stage. */
 void wmd_options_default(wmd_options* opt);
 
 /* Drop-in for the fused reference sinkhorn_wmd (sinkhorn_wmd_fused.cpp:112-114)

@@ -122,7 +122,7 @@ class WMDEngine:
         self.n_queries = n
 
     def run(self, lamda, max_iter=15, tol=0.0, variant=nat.VARIANT_RESIDENT, check_every=8,
-            shape=-1, summarize=True, log_domain=False, mfma_cdist=False) -> dict:
+            shape=-1, summarize=True, log_domain=False, mfma_cdist=True) -> dict:
         """Solve every (query, doc) pair; returns the run's wmd_stats as a dict
         (with summarize=False only stage times, no device-to-host copy)."""
         o = nat.default_options(max_iter=int(max_iter), tol=float(tol), variant=int(variant),

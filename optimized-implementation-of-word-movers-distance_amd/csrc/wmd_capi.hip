@@ -380,7 +380,7 @@ void wmd_options_default(wmd_options* opt) {
   opt->shape = -1;
   opt->summarize = 1;
   opt->domain = WMD_DOMAIN_LINEAR;
-  opt->cdist = WMD_CDIST_FP32;
+  opt->cdist = WMD_CDIST_MFMA;
 }
 
 const char* wmd_last_error(void) { return g_err.c_str(); }

@@ -42,14 +42,15 @@ def plain(dg, P):
 
 
 @pytest.mark.parametrize("name", ["tiny", "cpu_small", "plain"])
-@pytest.mark.parametrize("variant", VARIANTS)
-def test_fixed_iterations_match_reference(P, dg, name, variant):
-    say(f"[case] fixed15 {name} {VNAME[variant]}")
+@pytest.mark.parametrize("variant,mfma", [(RES, True), (FUS, True), (NON, True), (RES, False),
+                                          (NON, False)])
+def test_fixed_iterations_match_reference(P, dg, name, variant, mfma):
+    say(f"[case] fixed15 {name} {VNAME[variant]} cdist={'mfma' if mfma else 'fp32'}")
     corpus, qs, p = dg.workload(name, seed=0)
     gold = load_golden(name)["wmd_fused"]
     with engine(P, corpus) as e:
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
-        st = e.run(p["lamda"], max_iter=15, variant=variant)
+        st = e.run(p["lamda"], max_iter=15, variant=variant, mfma_cdist=mfma)
         w, it, _ = e.result()
     assert st["variant"] == variant
     assert (it == 15).all()
