@@ -72,6 +72,18 @@ enum wmd_cdist {
   WMD_CDIST_MFMA = 1
 };
 
+enum wmd_warm {
+  /* u = q for every pair (the reference's start, sinkhorn_wmd_fused.cpp:188-191) */
+  WMD_WARM_UNIFORM = 0,
+  /* v from the context's trained MLP; a pair whose residual does not decrease
+   * over a 64-iteration window restarts once from the uniform start
+   * (resident variant, linear domain) */
+  WMD_WARM_LEARNED = 1,
+  /* v from values given with wmd_context_set_warm (e.g. a previous solve's
+   * duals); same stall fallback */
+  WMD_WARM_GIVEN = 2
+};
+
 typedef struct wmd_options {
   int32_t max_iter;    /* iteration cap; the reference runs exactly 15 (:12) */
   int32_t variant;     /* enum wmd_variant */
@@ -86,7 +98,16 @@ typedef struct wmd_options {
                           times only */
   int32_t domain;      /* enum wmd_domain */
   int32_t cdist;       /* enum wmd_cdist */
-  int32_t reserved[3];
+  int32_t warm_start;  /* enum wmd_warm */
+  int32_t export_duals;/* 1: keep the converged log v of every (query, nonzero)
+                          for wmd_context_duals / training (resident variant) */
+  int32_t reserved0;
+  double omega;        /* resident variant: over-relaxation of the u-update,
+                          u <- u (r / (u K v))^omega, 1 = plain Sinkhorn, in
+                          [1, 2).  The fixed point and the residual test are
+                          unchanged (v is always the exact update); a pair whose
+                          residual rises falls back to omega = 1. */
+  int32_t reserved[4];
 } wmd_options;
 
 typedef struct wmd_stats {
@@ -99,11 +120,14 @@ typedef struct wmd_stats {
   double iters_mean;
   double resid_max;    /* largest final marginal residual over pairs */
   int32_t variant;     /* variant that ran */
-  int32_t reserved[7];
+  int32_t fallbacks;   /* warm-started pairs restarted from the uniform start */
+  double ms_warm;      /* learned warm start: MLP inference stage */
+  int32_t reserved[4];
 } wmd_stats;
 
 /* Default options: max_iter 15, resident variant, tol 0, automatic shape,
- * summarize 1, linear domain, matrix-core distance stage. */
+ * summarize 1, linear domain, matrix-core distance stage, uniform start,
+ * omega 1. */
 void wmd_options_default(wmd_options* opt);
 
 /* Drop-in for the fused reference sinkhorn_wmd (sinkhorn_wmd_fused.cpp:112-114)
@@ -160,6 +184,52 @@ int wmd_context_result(wmd_context* ctx, double* wmd, int32_t* iters, float* res
 /* Device pointer of the n_queries x N FP64 result (valid until the next
  * set_queries / destroy), for callers that keep results on the GPU. */
 const double* wmd_context_device_result(const wmd_context* ctx);
+
+/* ---- Learned warm start (no reference counterpart).  A 3D -> 256 -> 256 -> 1
+ * MLP with tanh-form GELU hidden layers predicts, for every (query, document
+ * word), the converged log v_e from [word embedding; document mean embedding;
+ * query mean embedding] (means weighted by c and r).  Parameters are one flat
+ * FP32 array: W1 [256][3D] row-major, b1 [256], W2 [256][256], b2 [256],
+ * w3 [256], b3 [1].  The targets are centred per (query, document) pair: the
+ * scaling pair (a u, v / a) gives the same plan, so only differences of log v
+ * within a document matter. */
+#define WMD_MLP_HIDDEN 256
+
+uint64_t wmd_mlp_param_count(uint64_t D);
+int wmd_context_mlp_set(wmd_context* ctx, const float* params, uint64_t n_params);
+int wmd_context_mlp_get(wmd_context* ctx, float* params, uint64_t n_params);
+
+typedef struct wmd_train_options {
+  int32_t epochs;          /* passes over every (query, document) pair */
+  int32_t pairs_per_batch; /* pairs per Adam step */
+  double lr;               /* Adam learning rate (beta 0.9 / 0.999, eps 1e-8) */
+  double weight_decay;     /* L2 coefficient added to the gradient */
+  uint64_t seed;           /* pair shuffling */
+  int32_t reserved[8];
+} wmd_train_options;
+
+void wmd_train_options_default(wmd_train_options* opt);
+
+/* Train the context's MLP on the GPU against the duals of the last run that
+ * exported them (same corpus and queries): every (query, document) pair of
+ * the current batch is one training group.  epoch_loss[epochs] (may be NULL)
+ * receives the mean centred squared error of each epoch. */
+int wmd_context_mlp_train(wmd_context* ctx, const wmd_train_options* opt, double* epoch_loss);
+
+/* Caller-supplied starting log v (n_queries x nnz, the layout of
+ * wmd_context_duals) for runs with warm_start = WMD_WARM_GIVEN. */
+int wmd_context_set_warm(wmd_context* ctx, const float* log_v);
+
+/* Converged log v of the last run with export_duals: n_queries x nnz, in the
+ * document-major nonzero order (documents ascending, words ascending). */
+int wmd_context_duals(wmd_context* ctx, float* out);
+
+/* The training step's loss and gradient on the GPU for n samples X [n][3D]
+ * with targets y [n] in n_groups contiguous groups (group_ptr [n_groups + 1]);
+ * params and grad use the flat layout above.  Stateless (device `device`). */
+int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const float* y,
+                      uint64_t n_groups, const int32_t* group_ptr, const float* params,
+                      double* loss, float* grad);
 
 /* Number of resident tile shapes and their limits (for tuners/tests). */
 int wmd_num_shapes(void);

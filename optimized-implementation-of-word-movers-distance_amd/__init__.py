@@ -8,7 +8,8 @@ synthetic workload generators and the reference's on-disk formats.
 from . import datagen  # noqa: F401
 from ._native import (VARIANT_FUSED, VARIANT_NONFUSED, VARIANT_RESIDENT,  # noqa: F401
                       WMDError, load)
-from .api import (WMDEngine, shapes, sinkhorn_wmd, sinkhorn_wmd_fused,  # noqa: F401
+from .api import (WMDEngine, mlp_init, mlp_layout, mlp_loss_grad,  # noqa: F401
+                  mlp_param_count, shapes, sinkhorn_wmd, sinkhorn_wmd_fused,
                   sinkhorn_wmd_nonfused)
 
 PACKAGE = __name__

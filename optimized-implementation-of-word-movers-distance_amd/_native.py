@@ -30,27 +30,43 @@ DOMAIN_LOG = 1
 CDIST_FP32 = 0
 CDIST_MFMA = 1
 
+WARM_UNIFORM = 0
+WARM_LEARNED = 1
+WARM_GIVEN = 2
+MLP_HIDDEN = 256
+
 # Every symbol include/wmd.h declares.
 EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
            "wmd_context_create", "wmd_context_destroy", "wmd_context_set_corpus",
            "wmd_context_set_queries", "wmd_context_run", "wmd_context_result",
-           "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_last_error")
+           "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_last_error",
+           "wmd_mlp_param_count", "wmd_context_mlp_set", "wmd_context_mlp_get",
+           "wmd_train_options_default", "wmd_context_mlp_train", "wmd_context_duals",
+           "wmd_mlp_loss_grad", "wmd_context_set_warm")
 
 
 class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("variant", C.c_int32), ("tol", C.c_double),
                 ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
-                ("domain", C.c_int32), ("cdist", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("domain", C.c_int32), ("cdist", C.c_int32), ("warm_start", C.c_int32),
+                ("export_duals", C.c_int32), ("reserved0", C.c_int32), ("omega", C.c_double),
+                ("reserved", C.c_int32 * 4)]
 
 
 class Stats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_cdist", C.c_double), ("ms_solve", C.c_double),
                 ("pairs", C.c_int64), ("iters_max", C.c_int32), ("launches", C.c_int32),
                 ("iters_mean", C.c_double), ("resid_max", C.c_double),
-                ("variant", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("variant", C.c_int32), ("fallbacks", C.c_int32), ("ms_warm", C.c_double),
+                ("reserved", C.c_int32 * 4)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class TrainOptions(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("pairs_per_batch", C.c_int32), ("lr", C.c_double),
+                ("weight_decay", C.c_double), ("seed", C.c_uint64), ("reserved", C.c_int32 * 8)]
 
 
 class WMDError(RuntimeError):
@@ -93,9 +109,21 @@ def load():
                                  C.POINTER(C.c_int32)]
     L.wmd_last_error.argtypes = []
     L.wmd_last_error.restype = C.c_char_p
+    L.wmd_mlp_param_count.argtypes = [U64]
+    L.wmd_mlp_param_count.restype = C.c_uint64
+    L.wmd_context_mlp_set.argtypes = [P, P, U64]
+    L.wmd_context_mlp_get.argtypes = [P, P, U64]
+    L.wmd_train_options_default.argtypes = [C.POINTER(TrainOptions)]
+    L.wmd_train_options_default.restype = None
+    L.wmd_context_mlp_train.argtypes = [P, C.POINTER(TrainOptions), P]
+    L.wmd_context_duals.argtypes = [P, P]
+    L.wmd_context_set_warm.argtypes = [P, P]
+    L.wmd_mlp_loss_grad.argtypes = [C.c_int, U64, U64, P, P, U64, P, P, P, P]
     for name in ("wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
                  "wmd_context_create", "wmd_context_set_corpus", "wmd_context_set_queries",
-                 "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info"):
+                 "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info",
+                 "wmd_context_mlp_set", "wmd_context_mlp_get", "wmd_context_mlp_train",
+                 "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

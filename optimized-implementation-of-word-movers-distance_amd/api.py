@@ -122,18 +122,55 @@ class WMDEngine:
         self.n_queries = n
 
     def run(self, lamda, max_iter=15, tol=0.0, variant=nat.VARIANT_RESIDENT, check_every=8,
-            shape=-1, summarize=True, log_domain=False, mfma_cdist=True) -> dict:
+            shape=-1, summarize=True, log_domain=False, mfma_cdist=True, warm_start=False,
+            export_duals=False, omega=1.0) -> dict:
         """Solve every (query, doc) pair; returns the run's wmd_stats as a dict
         (with summarize=False only stage times, no device-to-host copy)."""
         o = nat.default_options(max_iter=int(max_iter), tol=float(tol), variant=int(variant),
                                 check_every=int(check_every), shape=int(shape),
                                 summarize=int(bool(summarize)),
                                 domain=nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR,
-                                cdist=nat.CDIST_MFMA if mfma_cdist else nat.CDIST_FP32)
+                                cdist=nat.CDIST_MFMA if mfma_cdist else nat.CDIST_FP32,
+                                warm_start=(nat.WARM_GIVEN if warm_start == "given" else
+                                            nat.WARM_LEARNED if warm_start else nat.WARM_UNIFORM),
+                                export_duals=int(bool(export_duals)), omega=float(omega))
         st = nat.Stats()
         nat.check(self._L.wmd_context_run(self._h, float(lamda), C.byref(o), C.byref(st)),
                   "wmd_context_run")
         return st.as_dict()
+
+    # ---- learned warm start ----
+    def mlp_set(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        nat.check(self._L.wmd_context_mlp_set(self._h, params.ctypes.data, params.size),
+                  "wmd_context_mlp_set")
+
+    def mlp_get(self):
+        n = mlp_param_count(self.D)
+        out = np.empty(n, dtype=np.float32)
+        nat.check(self._L.wmd_context_mlp_get(self._h, out.ctypes.data, n), "wmd_context_mlp_get")
+        return out
+
+    def mlp_train(self, epochs=4, pairs_per_batch=32, lr=1e-3, weight_decay=0.0, seed=0):
+        """Train on the duals of the last run with export_duals; returns the
+        per-epoch mean centred squared error."""
+        o = nat.TrainOptions()
+        self._L.wmd_train_options_default(C.byref(o))
+        o.epochs, o.pairs_per_batch, o.lr = int(epochs), int(pairs_per_batch), float(lr)
+        o.weight_decay, o.seed = float(weight_decay), int(seed)
+        losses = np.zeros(max(int(epochs), 1), dtype=np.float64)
+        nat.check(self._L.wmd_context_mlp_train(self._h, C.byref(o), losses.ctypes.data),
+                  "wmd_context_mlp_train")
+        return losses[:int(epochs)]
+
+    def set_warm(self, log_v):
+        log_v = np.ascontiguousarray(log_v, dtype=np.float32)
+        nat.check(self._L.wmd_context_set_warm(self._h, log_v.ctypes.data), "wmd_context_set_warm")
+
+    def duals(self, nnz):
+        out = np.empty(self.n_queries * int(nnz), dtype=np.float32)
+        nat.check(self._L.wmd_context_duals(self._h, out.ctypes.data), "wmd_context_duals")
+        return out.reshape(self.n_queries, int(nnz))
 
     def result(self, allow_nonfinite=False):
         """(wmd [nq, N] float64, iters [nq, N] int32, resid [nq, N] float32)."""
@@ -146,6 +183,51 @@ class WMDEngine:
             nat.check(st, "wmd_context_result")
         shp = (self.n_queries, self.N)
         return w.reshape(shp), it.reshape(shp), rs.reshape(shp)
+
+
+def mlp_param_count(D):
+    return int(nat.load().wmd_mlp_param_count(int(D)))
+
+
+def mlp_layout(D):
+    """Offsets of W1 [256, 3D], b1, W2 [256, 256], b2, w3, b3 in the flat parameters."""
+    H, F = nat.MLP_HIDDEN, 3 * int(D)
+    off = {"W1": 0, "b1": H * F}
+    off["W2"] = off["b1"] + H
+    off["b2"] = off["W2"] + H * H
+    off["w3"] = off["b2"] + H
+    off["b3"] = off["w3"] + H
+    off["total"] = off["b3"] + 1
+    return off
+
+
+def mlp_init(D, seed=0):
+    """He-normal weights, zero biases, small output layer."""
+    rng = np.random.default_rng(seed)
+    H, F = nat.MLP_HIDDEN, 3 * int(D)
+    L = mlp_layout(D)
+    p = np.zeros(L["total"], dtype=np.float32)
+    p[L["W1"]:L["b1"]] = rng.normal(0, np.sqrt(2.0 / F), H * F)
+    p[L["W2"]:L["b2"]] = rng.normal(0, np.sqrt(2.0 / H), H * H)
+    p[L["w3"]:L["b3"]] = rng.normal(0, np.sqrt(1.0 / H), H)
+    return p
+
+
+def mlp_loss_grad(X, y, group_ptr, params, device=0):
+    """Centred squared-error loss and its gradient, computed by the GPU
+    training kernels (stateless)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    group_ptr = np.ascontiguousarray(group_ptr, dtype=np.int32)
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    n, F = X.shape
+    grad = np.empty_like(params)
+    loss = C.c_double()
+    nat.check(nat.load().wmd_mlp_loss_grad(int(device), n, F // 3, X.ctypes.data, y.ctypes.data,
+                                           group_ptr.shape[0] - 1, group_ptr.ctypes.data,
+                                           params.ctypes.data, C.byref(loss), grad.ctypes.data),
+              "wmd_mlp_loss_grad")
+    return loss.value, grad
 
 
 def shapes():

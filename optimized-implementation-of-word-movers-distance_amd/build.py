@@ -15,7 +15,7 @@ import sys
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libwmd.so")
-SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in ("wmd_kernels.hip", "wmd_capi.hip")]
+SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in ("wmd_kernels.hip", "wmd_mlp.hip", "wmd_capi.hip")]
 ARCH = os.environ.get("WMD_OFFLOAD_ARCH", "gfx950")
 
 

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -110,6 +111,24 @@ struct wmd_context {
   DevBuf<uint8_t> done;
   DevBuf<uint32_t> status;
   bool ran = false;
+
+  // learned warm start
+  int n_cu = 0;
+  std::vector<int32_t> h_doc_ptr, h_doc_t, h_touched;  // host copies for training batches
+  DevBuf<int32_t> doc_of;               // nnz: document of each nonzero
+  DevBuf<float> mlp_p, mlp_g, mlp_m, mlp_v;  // parameters, gradients, Adam moments
+  int64_t mlp_D = 0;                    // embedding size the parameters are for
+  int adam_step = 0;
+  bool mlp_ready = false;
+  float mlp_b3 = 0.f;                   // host copy of the output bias
+  bool factors_ready = false;           // At / Bd / dbar for the current corpus and weights
+  DevBuf<uint16_t> w2b;
+  DevBuf<float> dbar, qbar, At, Bd, Cq, pred;
+  DevBuf<float> duals;
+  bool duals_valid = false;
+  DevBuf<uint32_t> fallbacks;
+  DevBuf<float> warm_vals;              // caller-supplied starting log v
+  bool warm_given = false;
 };
 
 namespace {
@@ -140,6 +159,7 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   a.Vt = (int)c->Vt;
   a.max_iter = o.max_iter;
   a.tol = (float)o.tol;
+  a.omega = (float)o.omega;
   a.inv_lamda = (float)(1.0 / lamda);
   a.lamda = (float)lamda;
   a.wmd = c->wmd.p;
@@ -368,6 +388,154 @@ int one_shot(const double* r, const double* vecs, double* WMD, double lamda, uin
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Learned warm start: parameter layout, training step, inference stage.
+namespace {
+
+struct MlpLayout {
+  int64_t F;  // input features 3D
+  int64_t W1, b1, W2, b2, w3, b3, total;
+  explicit MlpLayout(int64_t D) {
+    const int64_t H = wmd::kMlpHidden;
+    F = 3 * D;
+    W1 = 0;
+    b1 = H * F;
+    W2 = b1 + H;
+    b2 = W2 + H * H;
+    w3 = b2 + H;
+    b3 = w3 + H;
+    total = b3 + 1;
+  }
+};
+
+struct MlpScratch {
+  DevBuf<float> X, y, Z1, H1, Z2, H2, yhat, dy, G1, G2;
+  hipError_t ensure(int64_t nb, int64_t F) {
+    const int64_t H = wmd::kMlpHidden;
+    hipError_t e = X.ensure((size_t)(nb * F));
+    if (e == hipSuccess) e = y.ensure((size_t)nb);
+    if (e == hipSuccess) e = Z1.ensure((size_t)(nb * H));
+    if (e == hipSuccess) e = H1.ensure((size_t)(nb * H));
+    if (e == hipSuccess) e = Z2.ensure((size_t)(nb * H));
+    if (e == hipSuccess) e = H2.ensure((size_t)(nb * H));
+    if (e == hipSuccess) e = yhat.ensure((size_t)nb);
+    if (e == hipSuccess) e = dy.ensure((size_t)nb);
+    if (e == hipSuccess) e = G1.ensure((size_t)(nb * H));
+    if (e == hipSuccess) e = G2.ensure((size_t)(nb * H));
+    return e;
+  }
+  void release() {
+    X.release(); y.release(); Z1.release(); H1.release(); Z2.release(); H2.release();
+    yhat.release(); dy.release(); G1.release(); G2.release();
+  }
+};
+
+wmd::GemmArgs gemm(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,
+                   int64_t sbk, int64_t sbn, float* C, int64_t ldc, const int32_t* arow = nullptr) {
+  wmd::GemmArgs g{};
+  g.M = M; g.N = N; g.K = K;
+  g.A = A; g.sam = sam; g.sak = sak; g.arow = arow;
+  g.B = B; g.sbk = sbk; g.sbn = sbn;
+  g.C = C; g.ldc = ldc; g.beta = 0.f;
+  return g;
+}
+
+// Loss (added into *d_loss) and gradient of the centred squared error for nb
+// samples already in S.X / S.y, groups given by batch-relative offsets gptr.
+hipError_t mlp_loss_grad(hipStream_t st, int64_t nb, int64_t D, const int32_t* gptr, int groups,
+                         const float* p, float* g, double* d_loss, MlpScratch& S) {
+  const MlpLayout L(D);
+  const int H = wmd::kMlpHidden;
+  const int F = (int)L.F;
+  const int n = (int)nb;
+  hipError_t e;
+#define MLP_TRY(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
+  MLP_TRY(wmd::launch_gemm(gemm(n, H, F, S.X.p, F, 1, p + L.W1, 1, F, S.Z1.p, H), st));
+  MLP_TRY(wmd::launch_bias_gelu(S.Z1.p, S.H1.p, p + L.b1, nb, H, st));
+  MLP_TRY(wmd::launch_gemm(gemm(n, H, H, S.H1.p, H, 1, p + L.W2, 1, H, S.Z2.p, H), st));
+  MLP_TRY(wmd::launch_bias_gelu(S.Z2.p, S.H2.p, p + L.b2, nb, H, st));
+  MLP_TRY(wmd::launch_gemm(gemm(n, 1, H, S.H2.p, H, 1, p + L.w3, 1, 0, S.yhat.p, 1), st));
+  MLP_TRY(wmd::launch_centred_loss(S.yhat.p, p + L.b3, S.y.p, gptr, groups, nb, S.dy.p, d_loss, st));
+  MLP_TRY(wmd::launch_gemm(gemm(H, 1, n, S.H2.p, 1, H, S.dy.p, 1, 0, g + L.w3, 1), st));
+  MLP_TRY(wmd::launch_sum_to(S.dy.p, nb, g + L.b3, st));
+  MLP_TRY(wmd::launch_outer(S.G2.p, S.dy.p, p + L.w3, nb, H, st));
+  MLP_TRY(wmd::launch_gelu_back(S.G2.p, S.Z2.p, nb * H, st));
+  MLP_TRY(wmd::launch_gemm(gemm(H, H, n, S.G2.p, 1, H, S.H1.p, H, 1, g + L.W2, H), st));
+  MLP_TRY(wmd::launch_colsum(S.G2.p, g + L.b2, nb, H, st));
+  MLP_TRY(wmd::launch_gemm(gemm(n, H, H, S.G2.p, H, 1, p + L.W2, H, 1, S.G1.p, H), st));
+  MLP_TRY(wmd::launch_gelu_back(S.G1.p, S.Z1.p, nb * H, st));
+  MLP_TRY(wmd::launch_gemm(gemm(H, F, n, S.G1.p, 1, H, S.X.p, F, 1, g + L.W1, F), st));
+  MLP_TRY(wmd::launch_colsum(S.G1.p, g + L.b1, nb, H, st));
+#undef MLP_TRY
+  return hipSuccess;
+}
+
+// Mean embeddings of the corpus documents (weights c) into c->dbar.
+int mlp_doc_means(wmd_context* c) {
+  HIP_TRY(c->dbar.ensure((size_t)c->N * c->D));
+  HIP_TRY(wmd::launch_mean_rows(c->E.p, (int)c->D, c->doc_ptr.p, c->doc_t.p, c->touched.p,
+                                c->doc_val.p, (int)c->N, c->dbar.p, c->stream));
+  return WMD_OK;
+}
+
+int mlp_query_means(wmd_context* c) {
+  HIP_TRY(c->qbar.ensure((size_t)c->nq * c->D));
+  HIP_TRY(wmd::launch_mean_rows(c->E.p, (int)c->D, c->q_ptr.p, c->q_words.p, nullptr, c->q_r.p,
+                                c->nq, c->qbar.p, c->stream));
+  return WMD_OK;
+}
+
+// Inference stage: per-query factor, then every (query, nonzero) sample.
+// Corpus factors (per touched word, per document) are built once per
+// corpus and weight set.
+int mlp_predict(wmd_context* c, int* launches) {
+  const MlpLayout L(c->D);
+  const int H = wmd::kMlpHidden;
+  const int D = (int)c->D;
+  const int F = (int)L.F;
+  const float* p = c->mlp_p.p;
+  if (!c->factors_ready) {
+    int rc = mlp_doc_means(c);
+    if (rc != WMD_OK) return rc;
+    HIP_TRY(c->At.ensure((size_t)c->Vt * H));
+    HIP_TRY(c->Bd.ensure((size_t)c->N * H));
+    HIP_TRY(wmd::launch_gemm(gemm((int)c->Vt, H, D, c->E.p, D, 1, p + L.W1, 1, F, c->At.p, H,
+                                  c->touched.p), c->stream));
+    HIP_TRY(wmd::launch_gemm(gemm((int)c->N, H, D, c->dbar.p, D, 1, p + L.W1 + D, 1, F, c->Bd.p,
+                                  H), c->stream));
+    HIP_TRY(wmd::launch_add_rows(c->Bd.p, p + L.b1, c->N, H, c->stream));
+    HIP_TRY(c->w2b.ensure((size_t)H * H));
+    HIP_TRY(wmd::launch_to_bf16(p + L.W2, (int64_t)H * H, c->w2b.p, c->stream));
+    c->factors_ready = true;
+  }
+  int rc = mlp_query_means(c);
+  if (rc != WMD_OK) return rc;
+  HIP_TRY(c->Cq.ensure((size_t)c->nq * H));
+  HIP_TRY(wmd::launch_gemm(gemm(c->nq, H, D, c->qbar.p, D, 1, p + L.W1 + 2 * D, 1, F, c->Cq.p, H),
+                           c->stream));
+  HIP_TRY(c->pred.ensure((size_t)c->nq * (size_t)c->nnz));
+  const float b3 = c->mlp_b3;
+  wmd::MlpInferArgs ia{};
+  ia.At = c->At.p;
+  ia.Bd = c->Bd.p;
+  ia.Cq = c->Cq.p;
+  ia.W2b = c->w2b.p;
+  ia.b2 = p + L.b2;
+  ia.w3 = p + L.w3;
+  ia.b3 = b3;
+  ia.doc_t = c->doc_t.p;
+  ia.doc_of = c->doc_of.p;
+  ia.nnz = c->nnz;
+  ia.nq = c->nq;
+  ia.pred = c->pred.p;
+  HIP_TRY(wmd::launch_mlp_infer(ia, c->n_cu, c->stream));
+  *launches += 4;
+  return WMD_OK;
+}
+
+}  // namespace
+
+
 extern "C" {
 
 void wmd_options_default(wmd_options* opt) {
@@ -381,6 +549,9 @@ void wmd_options_default(wmd_options* opt) {
   opt->summarize = 1;
   opt->domain = WMD_DOMAIN_LINEAR;
   opt->cdist = WMD_CDIST_MFMA;
+  opt->warm_start = WMD_WARM_UNIFORM;
+  opt->export_duals = 0;
+  opt->omega = 1.0;
 }
 
 const char* wmd_last_error(void) { return g_err.c_str(); }
@@ -416,6 +587,7 @@ int wmd_context_create(int device, wmd_context** out) {
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 4 * sizeof(uint32_t), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
     set_err("context setup failed: %s", hipGetErrorString(e));
     wmd_context_destroy(c);
@@ -436,6 +608,10 @@ void wmd_context_destroy(wmd_context* c) {
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
   c->doc_perm.release();
   c->Es.release(); c->nt.release(); c->Qs.release(); c->nqw.release();
+  c->doc_of.release(); c->mlp_p.release(); c->mlp_g.release(); c->mlp_m.release();
+  c->mlp_v.release(); c->w2b.release(); c->dbar.release(); c->qbar.release();
+  c->At.release(); c->Bd.release(); c->Cq.release(); c->pred.release(); c->duals.release();
+  c->fallbacks.release(); c->warm_vals.release();
   c->status.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -515,8 +691,18 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   }
   class_off[wmd::kNumDocClasses] = (int64_t)perm.size();
   HIP_TRY(c->doc_perm.upload(perm.data(), perm.size(), c->stream));
+  std::vector<int32_t> doc_of(nnz);
+  for (uint64_t j = 0; j < N; ++j)
+    for (int32_t pos = dptr[j]; pos < dptr[j + 1]; ++pos) doc_of[pos] = (int32_t)j;
+  HIP_TRY(c->doc_of.upload(doc_of.data(), doc_of.size(), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->h_class_off = class_off;
+  c->h_doc_ptr = dptr;
+  c->h_doc_t = dt;
+  c->h_touched = touched;
+  c->factors_ready = false;
+  c->duals_valid = false;
+  if (c->mlp_D != (int64_t)D) c->mlp_ready = false;
   c->V = (int64_t)V;
   c->D = (int64_t)D;
   c->N = (int64_t)N;
@@ -605,6 +791,8 @@ int wmd_context_set_queries(wmd_context* c, uint64_t n, const int64_t* qp, const
   c->k_total = ktot;
   c->u_total = utot;
   c->has_queries = true;
+  c->duals_valid = false;
+  c->warm_given = false;
   c->ran = false;
   return WMD_OK;
 }
@@ -626,6 +814,10 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     set_err("bad max_iter/tol");
     return WMD_ERR_ARG;
   }
+  if (!(o.omega >= 1.0 && o.omega < 2.0)) {
+    set_err("omega must be in [1, 2)");
+    return WMD_ERR_ARG;
+  }
   int variant = o.variant;
   if (variant < WMD_VARIANT_RESIDENT || variant > WMD_VARIANT_NONFUSED) {
     set_err("unknown variant %d", variant);
@@ -645,6 +837,22 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     return WMD_ERR_ARG;
   }
   const bool mfma = o.cdist == WMD_CDIST_MFMA;
+  if (o.warm_start != WMD_WARM_UNIFORM && o.warm_start != WMD_WARM_LEARNED &&
+      o.warm_start != WMD_WARM_GIVEN) {
+    set_err("unknown warm_start %d", o.warm_start);
+    return WMD_ERR_ARG;
+  }
+  const bool learned = o.warm_start == WMD_WARM_LEARNED;
+  const bool given = o.warm_start == WMD_WARM_GIVEN;
+  if (given && !c->warm_given) {
+    set_err("warm_start = given needs wmd_context_set_warm on the current queries");
+    return WMD_ERR_STATE;
+  }
+  if (learned && !c->mlp_ready) {
+    set_err("learned warm start needs MLP parameters for D=%lld (wmd_context_mlp_set / train)",
+            (long long)c->D);
+    return WMD_ERR_STATE;
+  }
   const int Dp = ((int)c->D + 15) & ~15;
   if (mfma && !c->split_ready) {
     // Once per corpus, outside the timed stages: BF16 planes and FP64 norms
@@ -679,6 +887,10 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
       }
       variant = WMD_VARIANT_FUSED;  // documents or queries beyond the register tiles
     }
+  }
+  if ((learned || given || o.export_duals) && (variant != WMD_VARIANT_RESIDENT || log_domain)) {
+    set_err("learned warm start and dual export run on the resident variant in the linear domain");
+    return WMD_ERR_UNSUPPORTED;
   }
   if (log_domain && variant != WMD_VARIANT_RESIDENT) {
     set_err("the log domain runs on the resident variant only (documents up to %d words, "
@@ -751,6 +963,21 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   }
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   wmd::SolveArgs a = base_args(c, lamda, o);
+  if (learned) {
+    rc = mlp_predict(c, &launches);
+    if (rc != WMD_OK) return rc;
+    a.warm = c->pred.p;
+  } else if (given) {
+    a.warm = c->warm_vals.p;
+  }
+  HIP_TRY(c->fallbacks.ensure(1));
+  HIP_TRY(wmd::launch_zero_u32(c->fallbacks.p, 1, c->stream));
+  a.fallbacks = c->fallbacks.p;
+  if (o.export_duals) {
+    HIP_TRY(c->duals.ensure((size_t)c->nq * (size_t)c->nnz));
+    a.duals = c->duals.p;
+  }
+  HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   if (variant == WMD_VARIANT_RESIDENT)
     rc = run_resident(c, a, o.shape, log_domain, &launches);
   else
@@ -759,15 +986,21 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   HIP_TRY(hipEventSynchronize(c->ev[2]));
   c->ran = true;
+  c->duals_valid = o.export_duals != 0;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    float t_all = 0.f, t_cd = 0.f, t_sv = 0.f;
+    float t_all = 0.f, t_cd = 0.f, t_sv = 0.f, t_w = 0.f;
     HIP_TRY(hipEventElapsedTime(&t_all, c->ev[0], c->ev[2]));
     HIP_TRY(hipEventElapsedTime(&t_cd, c->ev[0], c->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&t_sv, c->ev[1], c->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&t_w, c->ev[1], c->ev[3]));
+    HIP_TRY(hipEventElapsedTime(&t_sv, c->ev[3], c->ev[2]));
     stats->ms_total = t_all;
     stats->ms_cdist = t_cd;
+    stats->ms_warm = t_w;
     stats->ms_solve = t_sv;
+    uint32_t fb = 0;
+    HIP_TRY(hipMemcpy(&fb, c->fallbacks.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    stats->fallbacks = (int32_t)fb;
     stats->pairs = (int64_t)c->nq * c->N;
     stats->launches = launches;
     stats->variant = variant;
@@ -846,6 +1079,268 @@ int wmd_sinkhorn_nonfused(const double* r, const double* vecs, double* WMD, doub
   o.max_iter = max_iter;
   o.variant = WMD_VARIANT_NONFUSED;
   return one_shot(r, vecs, WMD, -lamda, V, N, D, row_ptr, col_inds, vals, &o, nullptr);
+}
+
+
+uint64_t wmd_mlp_param_count(uint64_t D) { return (uint64_t)MlpLayout((int64_t)D).total; }
+
+void wmd_train_options_default(wmd_train_options* opt) {
+  if (!opt) return;
+  std::memset(opt, 0, sizeof(*opt));
+  opt->epochs = 4;
+  opt->pairs_per_batch = 32;
+  opt->lr = 1e-3;
+  opt->weight_decay = 0.0;
+  opt->seed = 0;
+}
+
+int wmd_context_mlp_set(wmd_context* c, const float* params, uint64_t n_params) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->has_corpus) {
+    set_err("set_corpus must come first (the input size is 3 D)");
+    return WMD_ERR_STATE;
+  }
+  const MlpLayout L(c->D);
+  if (!params || n_params != (uint64_t)L.total) {
+    set_err("expected %lld MLP parameters for D=%lld", (long long)L.total, (long long)c->D);
+    return WMD_ERR_ARG;
+  }
+  for (uint64_t k = 0; k < n_params; ++k)
+    if (!std::isfinite(params[k])) {
+      set_err("MLP parameter %llu is not finite", (unsigned long long)k);
+      return WMD_ERR_ARG;
+    }
+  HIP_TRY(c->mlp_p.upload(params, n_params, c->stream));
+  HIP_TRY(c->mlp_g.ensure(n_params));
+  HIP_TRY(c->mlp_m.ensure(n_params));
+  HIP_TRY(c->mlp_v.ensure(n_params));
+  HIP_TRY(hipMemsetAsync(c->mlp_m.p, 0, n_params * sizeof(float), c->stream));
+  HIP_TRY(hipMemsetAsync(c->mlp_v.p, 0, n_params * sizeof(float), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->mlp_D = c->D;
+  c->mlp_b3 = params[L.b3];
+  c->adam_step = 0;
+  c->mlp_ready = true;
+  c->factors_ready = false;
+  return WMD_OK;
+}
+
+int wmd_context_mlp_get(wmd_context* c, float* params, uint64_t n_params) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->mlp_ready) {
+    set_err("no MLP parameters in this context");
+    return WMD_ERR_STATE;
+  }
+  const MlpLayout L(c->mlp_D);
+  if (!params || n_params != (uint64_t)L.total) {
+    set_err("expected %lld MLP parameters", (long long)L.total);
+    return WMD_ERR_ARG;
+  }
+  HIP_TRY(hipMemcpy(params, c->mlp_p.p, n_params * sizeof(float), hipMemcpyDeviceToHost));
+  return WMD_OK;
+}
+
+int wmd_context_set_warm(wmd_context* c, const float* log_v) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->has_queries) {
+    set_err("set_queries must come first");
+    return WMD_ERR_STATE;
+  }
+  if (!log_v) {
+    set_err("null input");
+    return WMD_ERR_ARG;
+  }
+  const size_t n = (size_t)c->nq * (size_t)c->nnz;
+  for (size_t k = 0; k < n; ++k)
+    if (!std::isfinite(log_v[k])) {
+      set_err("warm value %zu is not finite", k);
+      return WMD_ERR_ARG;
+    }
+  HIP_TRY(c->warm_vals.upload(log_v, n, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->warm_given = true;
+  return WMD_OK;
+}
+
+int wmd_context_duals(wmd_context* c, float* out) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->duals_valid) {
+    set_err("no exported duals: run with export_duals = 1 on the current corpus and queries");
+    return WMD_ERR_STATE;
+  }
+  if (!out) {
+    set_err("null output");
+    return WMD_ERR_ARG;
+  }
+  HIP_TRY(hipMemcpy(out, c->duals.p, (size_t)c->nq * c->nnz * sizeof(float),
+                    hipMemcpyDeviceToHost));
+  return WMD_OK;
+}
+
+int wmd_context_mlp_train(wmd_context* c, const wmd_train_options* opt_in, double* epoch_loss) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->mlp_ready || c->mlp_D != c->D) {
+    set_err("set MLP parameters for this corpus first (wmd_context_mlp_set)");
+    return WMD_ERR_STATE;
+  }
+  if (!c->duals_valid) {
+    set_err("training targets missing: run with export_duals = 1 on the current batch first");
+    return WMD_ERR_STATE;
+  }
+  wmd_train_options o;
+  if (opt_in) o = *opt_in; else wmd_train_options_default(&o);
+  if (o.epochs < 0 || o.pairs_per_batch <= 0 || !(o.lr > 0.0) || !(o.weight_decay >= 0.0)) {
+    set_err("bad training options");
+    return WMD_ERR_ARG;
+  }
+  const MlpLayout L(c->D);
+  const int D = (int)c->D;
+  const int64_t n_pairs = (int64_t)c->nq * c->N;
+  rc = mlp_doc_means(c);
+  if (rc != WMD_OK) return rc;
+  rc = mlp_query_means(c);
+  if (rc != WMD_OK) return rc;
+  // Sample layout per epoch: pairs in shuffled order, each pair's words contiguous.
+  std::vector<int64_t> order(n_pairs);
+  for (int64_t k = 0; k < n_pairs; ++k) order[k] = k;
+  const int64_t total = (int64_t)c->nq * c->nnz;
+  std::vector<int32_t> s_word(total), s_doc(total), s_query(total);
+  std::vector<int64_t> s_tidx(total);
+  const int P = o.pairs_per_batch;
+  const int64_t n_batches = (n_pairs + P - 1) / P;
+  std::vector<int32_t> rel((size_t)n_batches * (P + 1));
+  std::vector<int64_t> b_first(n_batches + 1), b_groups(n_batches);
+  int64_t nb_max = 0;
+  DevBuf<int32_t> d_word, d_doc, d_query, d_rel;
+  DevBuf<int64_t> d_tidx;
+  DevBuf<double> d_loss;
+  MlpScratch S;
+  std::mt19937_64 rng(o.seed);
+  int status = WMD_OK;
+  for (int ep = 0; ep < o.epochs && status == WMD_OK; ++ep) {
+    std::shuffle(order.begin(), order.end(), rng);
+    int64_t at = 0;
+    for (int64_t b = 0; b < n_batches; ++b) {
+      b_first[b] = at;
+      const int64_t p0 = b * P, p1 = std::min<int64_t>(n_pairs, p0 + P);
+      b_groups[b] = p1 - p0;
+      int32_t* rb = rel.data() + (size_t)b * (P + 1);
+      rb[0] = 0;
+      for (int64_t k = p0; k < p1; ++k) {
+        const int64_t pr = order[k];
+        const int Q = (int)(pr / c->N), j = (int)(pr % c->N);
+        for (int32_t pos = c->h_doc_ptr[j]; pos < c->h_doc_ptr[j + 1]; ++pos) {
+          s_word[at] = c->h_touched[c->h_doc_t[pos]];
+          s_doc[at] = j;
+          s_query[at] = Q;
+          s_tidx[at] = (int64_t)Q * c->nnz + pos;
+          ++at;
+        }
+        rb[k - p0 + 1] = (int32_t)(at - b_first[b]);
+      }
+      nb_max = std::max(nb_max, at - b_first[b]);
+    }
+    b_first[n_batches] = at;
+    if (ep == 0) {
+      HIP_TRY(S.ensure(std::max<int64_t>(nb_max, 1), L.F));
+      HIP_TRY(d_loss.ensure((size_t)n_batches));
+    }
+    HIP_TRY(d_word.upload(s_word.data(), (size_t)at, c->stream));
+    HIP_TRY(d_doc.upload(s_doc.data(), (size_t)at, c->stream));
+    HIP_TRY(d_query.upload(s_query.data(), (size_t)at, c->stream));
+    HIP_TRY(d_tidx.upload(s_tidx.data(), (size_t)at, c->stream));
+    HIP_TRY(d_rel.upload(rel.data(), rel.size(), c->stream));
+    HIP_TRY(hipMemsetAsync(d_loss.p, 0, sizeof(double) * (size_t)n_batches, c->stream));
+    for (int64_t b = 0; b < n_batches; ++b) {
+      const int64_t s0 = b_first[b], nb = b_first[b + 1] - s0;
+      if (nb <= 0) continue;
+      HIP_TRY(wmd::launch_gather_x(c->E.p, D, d_word.p + s0, d_doc.p + s0, d_query.p + s0,
+                                   c->dbar.p, c->qbar.p, nb, S.X.p, c->stream));
+      HIP_TRY(wmd::launch_gather_f(c->duals.p, d_tidx.p + s0, nb, S.y.p, c->stream));
+      HIP_TRY(mlp_loss_grad(c->stream, nb, c->D, d_rel.p + (size_t)b * (P + 1), (int)b_groups[b],
+                            c->mlp_p.p, c->mlp_g.p, d_loss.p + b, S));
+      ++c->adam_step;
+      HIP_TRY(wmd::launch_adam(c->mlp_p.p, c->mlp_g.p, c->mlp_m.p, c->mlp_v.p, L.total,
+                               (float)o.lr, 0.9f, 0.999f, 1e-8f, c->adam_step,
+                               (float)o.weight_decay, c->stream));
+    }
+    std::vector<double> bl(n_batches);
+    HIP_TRY(hipMemcpyAsync(bl.data(), d_loss.p, sizeof(double) * (size_t)n_batches,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double num = 0.0;
+    for (int64_t b = 0; b < n_batches; ++b) num += bl[b] * (double)(b_first[b + 1] - b_first[b]);
+    if (epoch_loss) epoch_loss[ep] = at ? num / (double)at : 0.0;
+    if (!std::isfinite(num)) {
+      set_err("training diverged in epoch %d", ep);
+      status = WMD_ERR_NONFINITE;
+    }
+  }
+  HIP_TRY(hipMemcpy(&c->mlp_b3, c->mlp_p.p + L.b3, sizeof(float), hipMemcpyDeviceToHost));
+  d_word.release(); d_doc.release(); d_query.release(); d_rel.release(); d_tidx.release();
+  d_loss.release();
+  S.release();
+  c->factors_ready = false;
+  return status;
+}
+
+int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const float* y,
+                      uint64_t n_groups, const int32_t* group_ptr, const float* params,
+                      double* loss, float* grad) {
+  if (!X || !y || !group_ptr || !params || !loss || !grad || n == 0 || D == 0 || n_groups == 0 ||
+      n > (1u << 24) || D > 65536) {
+    set_err("bad arguments");
+    return WMD_ERR_ARG;
+  }
+  if (group_ptr[0] != 0 || (uint64_t)group_ptr[n_groups] != n) {
+    set_err("group_ptr must run from 0 to n");
+    return WMD_ERR_ARG;
+  }
+  for (uint64_t g = 0; g < n_groups; ++g)
+    if (group_ptr[g + 1] < group_ptr[g]) {
+      set_err("group_ptr decreases");
+      return WMD_ERR_ARG;
+    }
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) {
+    set_err("device %d not present", device);
+    return WMD_ERR_ARG;
+  }
+  HIP_TRY(hipSetDevice(device));
+  const MlpLayout L((int64_t)D);
+  MlpScratch S;
+  DevBuf<float> dp, dg;
+  DevBuf<int32_t> dgp;
+  DevBuf<double> dl;
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipError_t e = S.ensure((int64_t)n, L.F);
+  if (e == hipSuccess) e = S.X.upload(X, n * (size_t)L.F, st);
+  if (e == hipSuccess) e = S.y.upload(y, n, st);
+  if (e == hipSuccess) e = dp.upload(params, (size_t)L.total, st);
+  if (e == hipSuccess) e = dg.ensure((size_t)L.total);
+  if (e == hipSuccess) e = dgp.upload(group_ptr, n_groups + 1, st);
+  if (e == hipSuccess) e = dl.ensure(1);
+  if (e == hipSuccess) e = hipMemsetAsync(dl.p, 0, sizeof(double), st);
+  if (e == hipSuccess) e = mlp_loss_grad(st, (int64_t)n, (int64_t)D, dgp.p, (int)n_groups, dp.p,
+                                         dg.p, dl.p, S);
+  if (e == hipSuccess) e = hipMemcpyAsync(grad, dg.p, sizeof(float) * (size_t)L.total,
+                                          hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(loss, dl.p, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  S.release(); dp.release(); dg.release(); dgp.release(); dl.release();
+  (void)hipStreamDestroy(st);
+  if (e != hipSuccess) {
+    set_err("MLP loss/gradient failed: %s", hipGetErrorString(e));
+    return WMD_ERR_HIP;
+  }
+  return WMD_OK;
 }
 
 }  // extern "C"

@@ -81,6 +81,7 @@ struct SolveArgs {
   int Vt;
   int max_iter;
   float tol;               // 0: fixed max_iter iterations
+  float omega;             // u-update over-relaxation (1 = plain)
   float inv_lamda;         // 1 / lamda
   float lamda;
   double* wmd;             // nq x N
@@ -91,6 +92,10 @@ struct SolveArgs {
   const int64_t* u_off;    // nq
   uint8_t* done;           // nq x N
   float* ow;               // nq x nnz SDDMM output (nonfused)
+  // resident variant, learned warm start and training targets
+  const float* warm;       // nq x nnz predicted log v (row-rescaled convention), or null
+  float* duals;            // nq x nnz converged log v written by the final pass, or null
+  uint32_t* fallbacks;     // count of warm-started pairs restarted from the uniform start
   int lds_q;               // LDS floats reserved for u (>= max q)
   int lds_n;               // LDS floats reserved per document-length array (>= max n)
 };
@@ -110,6 +115,64 @@ constexpr int kNumDocClasses = 5;
 constexpr int kDocClassWidth = 32;
 extern const TileShape kShapes[kNumShapes];
 
+// ---- learned warm start (wmd_mlp.hip) ----
+constexpr int kMlpHidden = 256;
+
+// C[m, n] = beta C[m, n] + sum_k A(m, k) B(k, n) in FP32;
+// A(m, k) = A[r(m) sam + k sak] with r(m) = arow ? arow[m] : m, B(k, n) = B[k sbk + n sbn].
+struct GemmArgs {
+  int M, N, K;
+  const float* A;
+  int64_t sam, sak;
+  const int32_t* arow;
+  const float* B;
+  int64_t sbk, sbn;
+  float* C;
+  int64_t ldc;
+  float beta;
+};
+
+struct MlpInferArgs {
+  const float* At;         // Vt x H   W1e E[touched t]
+  const float* Bd;         // N x H    W1d dbar_j + b1
+  const float* Cq;         // nq x H   W1q qbar_Q
+  const uint16_t* W2b;     // H x H    BF16 bits, row = output unit
+  const float* b2;
+  const float* w3;
+  float b3;
+  const int32_t* doc_t;    // nnz
+  const int32_t* doc_of;   // nnz: document of each nonzero
+  int64_t nnz;
+  int nq;
+  float* pred;             // nq x nnz
+};
+
+hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
+hipError_t launch_bias_gelu(float* Z, float* Hout, const float* b, int64_t rows, int cols,
+                            hipStream_t s);
+hipError_t launch_gelu_back(float* dH, const float* Z, int64_t n, hipStream_t s);
+hipError_t launch_outer(float* dH, const float* dy, const float* w, int64_t rows, int cols,
+                        hipStream_t s);
+hipError_t launch_colsum(const float* X, float* out, int64_t rows, int cols, hipStream_t s);
+hipError_t launch_centred_loss(const float* yhat, const float* b3, const float* y,
+                               const int32_t* gptr, int groups, int64_t n_total, float* dy,
+                               double* loss, hipStream_t s);
+hipError_t launch_sum_to(const float* x, int64_t n, float* out, hipStream_t s);
+hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                       float b2, float eps, int step, float wd, hipStream_t s);
+hipError_t launch_mean_rows(const float* E, int D, const int32_t* ptr, const int32_t* ids,
+                            const int32_t* touched, const float* wts, int rows, float* out,
+                            hipStream_t s);
+hipError_t launch_gather_x(const float* E, int D, const int32_t* s_word, const int32_t* s_doc,
+                           const int32_t* s_query, const float* dbar, const float* qbar,
+                           int64_t n, float* X, hipStream_t s);
+hipError_t launch_gather_f(const float* src, const int64_t* idx, int64_t n, float* dst,
+                           hipStream_t s);
+hipError_t launch_add_rows(float* X, const float* b, int64_t rows, int cols, hipStream_t s);
+hipError_t launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t s);
+hipError_t launch_mlp_infer(const MlpInferArgs& a, int n_cu, hipStream_t s);
+size_t mlp_infer_lds_bytes();
+
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
 hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s);
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s);
@@ -120,5 +183,6 @@ hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s)
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 
 }  // namespace wmd

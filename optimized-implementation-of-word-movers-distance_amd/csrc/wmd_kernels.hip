@@ -322,18 +322,38 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     u[m] = (i < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
   }
 
+  // Learned warm start: the first pass takes v from the predicted log v
+  // (one half-step, counted as an iteration) instead of from u = q.  A
+  // warm-started pair whose residual stalls (no decrease over a 64-iteration
+  // window) restarts once from the uniform start.  Slow but steady progress
+  // is not a stall: such pairs are slow from any start.
   const bool use_tol = a.tol > 0.f;
+  const bool warm_on = a.warm != nullptr;
+  bool seeded = warm_on;
+  bool fell_back = !warm_on;
+  float res_win = INFINITY;
+  bool sor = a.omega > 1.f;
+  float res_prev = INFINITY;
   int it = 0;
   float res = 0.f;
   float o[NE];
   while (it < a.max_iter) {
+    if (seeded) {
 #pragma unroll
-    for (int k = 0; k < NE; ++k) {
-      float s = 0.f;
+      for (int k = 0; k < NE; ++k) {
+        const int e = ea + A * k;
+        const float lv = (c[k] > 0.f) ? a.warm[(int64_t)Q * a.nnz + db + e] : 0.f;
+        o[k] = (c[k] > 0.f) ? __expf(fminf(fmaxf(lv, -60.f), 60.f)) : 0.f;
+      }
+    } else {
 #pragma unroll
-      for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
-      s = group_sum<B>(s);
-      o[k] = (c[k] > 0.f) ? fdiv(c[k], s) : 0.f;
+      for (int k = 0; k < NE; ++k) {
+        float s = 0.f;
+#pragma unroll
+        for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+        s = group_sum<B>(s);
+        o[k] = (c[k] > 0.f) ? fdiv(c[k], s) : 0.f;
+      }
     }
     float xs[NI];
 #pragma unroll
@@ -360,15 +380,43 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       // Double buffering: the next write to this half happens two
       // iterations later, after every wave has passed the next barrier.
     }
-    float part = 0.f;
+    if (!seeded) {
+      float part = 0.f;
 #pragma unroll
-    for (int m = 0; m < NI; ++m)
-      if (r[m] > 0.f) part += fabsf(u[m] * xs[m] - r[m]);
-    res = group_sum<B>(part);
+      for (int m = 0; m < NI; ++m)
+        if (r[m] > 0.f) part += fabsf(u[m] * xs[m] - r[m]);
+      res = group_sum<B>(part);
+    }
+    if (sor && !seeded) {
+      // Over-relaxed step u <- u (r / (u x))^omega; a rise in the residual
+      // switches the pair to plain steps for good.
+      if (res > res_prev) sor = false;
+      res_prev = res;
+    }
+    if (sor && !seeded) {
 #pragma unroll
-    for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? fdiv(r[m], xs[m]) : 0.f;
+      for (int m = 0; m < NI; ++m)
+        u[m] = (r[m] > 0.f) ? u[m] * exp2f(a.omega * __log2f(fdiv(r[m], u[m] * xs[m]))) : 0.f;
+    } else {
+#pragma unroll
+      for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? fdiv(r[m], xs[m]) : 0.f;
+    }
     ++it;
+    if (seeded) {
+      seeded = false;
+      continue;
+    }
     if (use_tol && res < a.tol) break;
+    if (!fell_back && (it & 63) == 0) {
+      if (res >= res_win) {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? (float)q : 0.f;
+        fell_back = true;
+        if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
+      } else {
+        res_win = res;
+      }
+    }
   }
 
   // Final half-step: v = c / (K^T u), then sum u o ((K o M) v) in FP64.
@@ -382,6 +430,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     const int e = ea + A * k;
     if (e < n) {
       const float oe = fdiv(c[k], s);
+      if (a.duals) a.duals[(int64_t)Q * a.nnz + db + e] = __logf(oe);
       const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
 #pragma unroll
       for (int m = 0; m < NI; ++m) {
@@ -477,6 +526,8 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
   }
 
   const bool use_tol = a.tol > 0.f;
+  bool sor = a.omega > 1.f;
+  float res_prev = INFINITY;
   int it = 0;
   float res = 0.f;
   float g[NE];
@@ -544,8 +595,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
     for (int m = 0; m < NI; ++m)
       if (r[m] > 0.f) part += fabsf(exp2f(f[m] + lx[m]) - r[m]);
     res = group_sum<B>(part);
+    if (sor) {
+      if (res > res_prev) sor = false;
+      res_prev = res;
+    }
+    const float om = sor ? a.omega : 1.f;
 #pragma unroll
-    for (int m = 0; m < NI; ++m) f[m] = (r[m] > 0.f) ? lr[m] - lx[m] : kLogZero;
+    for (int m = 0; m < NI; ++m)
+      f[m] = (r[m] > 0.f) ? f[m] + om * (lr[m] - lx[m] - f[m]) : kLogZero;
     ++it;
     if (use_tol && res < a.tol) break;
   }
@@ -1020,6 +1077,16 @@ hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_spmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
                      final_pass ? 1 : 0);
+  return hipGetLastError();
+}
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u32, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
   return hipGetLastError();
 }
 

@@ -216,6 +216,18 @@ def add_near_duplicates(rng, corpus: Corpus, queries: Queries, frac: float = 0.1
     return Queries(np.array(out_ptr, dtype=np.int64), words, weights)
 
 
+def training_set(vecs, n_queries: int, n_docs: int, seed: int, len_lo: int = 40,
+                 len_hi: int = 150):
+    """Fresh documents and queries over existing embeddings (the learned warm
+    start's training pairs): n_queries queries of lognormal(3.5, 0.8) lengths
+    clipped to 5-300 against n_docs documents, all pairs used."""
+    rng = np.random.default_rng(seed)
+    V = vecs.shape[0]
+    dp, dw, dv = make_docs(rng, V, n_docs, len_lo, len_hi)
+    lens = np.clip(np.rint(rng.lognormal(3.5, 0.8, size=n_queries)), 5, 300).astype(np.int64)
+    return Corpus(vecs, dp, dw, dv), make_queries(rng, V, lens)
+
+
 # --------------------------------------------------------------------------
 # The five workloads (sizes as named; seeds fixed).
 

@@ -287,3 +287,25 @@ def test_batched_full_size_properties(P, dg, oracle):
         err = rel_err(w[k], wo)
         say(f"  query {k} (q={qs.lengths()[k]}): err {err:.2e}")
         assert err <= 1e-4
+
+
+@pytest.mark.parametrize("omega", [1.3, 1.5, 1.7])
+def test_over_relaxation_reaches_the_same_fixed_point(P, dg, oracle, omega):
+    say(f"[case] over-relaxed u-update omega={omega}, residual 1e-4")
+    corpus, _, _ = dg.workload("batched", seed=3, N=1500, n_queries=1)
+    qs = _batch_queries(dg, corpus.V, [6, 19, 38, 55, 90, 170, 290], seed=9)
+    vecs = corpus.vecs.astype(np.float64)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=20000, tol=1e-4)
+        wp, itp, _ = e.result()
+        e.run(-10.0, max_iter=20000, tol=1e-4, omega=omega)
+        wo, ito, reso = e.result()
+    truth, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, vecs, corpus.doc_ptr,
+                                      corpus.doc_words, corpus.doc_vals, -10.0, 100000, tol=1e-10)
+    ep, eo = rel_err(wp, truth), rel_err(wo, truth)
+    say(f"  iterations plain {itp.sum()} (max {itp.max()}), omega {ito.sum()} (max {ito.max()}), "
+        f"ratio {ito.sum() / itp.sum():.3f}; error vs converged FP64: plain {ep:.2e}, "
+        f"omega {eo:.2e}")
+    assert np.all(reso < 1e-4)
+    assert eo <= 2e-4
