@@ -124,6 +124,25 @@ def cpu_port_per_pair(corpus, qs, lamda, tol, max_iter, threads, n_docs):
                       f"{dt:.2f}s"}
 
 
+def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=4):
+    """Train the warm-start MLP on converged duals of n_queries x n_docs
+    seeded synthetic pairs over the same embeddings (not part of a step)."""
+    tc, tq = dg.training_set(vecs, n_queries=n_queries, n_docs=n_docs, seed=101)
+    t0 = time.perf_counter()
+    with P.WMDEngine(0) as e:
+        e.set_corpus(tc.vecs, tc.word_major())
+        e.set_queries(tq.q_ptr, tq.words, tq.weights)
+        e.run(lamda, max_iter=20000, tol=1e-6, export_duals=True)
+        e.mlp_set(P.mlp_init(tc.D, seed=0))
+        losses = e.mlp_train(epochs=epochs, pairs_per_batch=32, lr=1e-3, seed=0)
+        params = e.mlp_get()
+    dt = time.perf_counter() - t0
+    log(f"warm-start MLP trained on {tq.n * tc.N} pairs in {dt:.1f}s, epoch losses "
+        f"{np.array2string(losses, precision=3)}")
+    return {"params": params, "pairs": int(tq.n * tc.N), "epochs": epochs,
+            "epoch_loss": [float(x) for x in losses], "seconds": round(dt, 2)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +156,12 @@ def main() -> int:
     ap.add_argument("--plain-queries", type=int, default=8)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cdist", choices=("mfma", "fp32"), default="mfma")
+    ap.add_argument("--omega", type=float, default=1.5,
+                    help="over-relaxation of the u-update (1 = plain Sinkhorn)")
+    ap.add_argument("--warm", choices=("uniform", "learned"), default="uniform")
+    ap.add_argument("--train-queries", type=int, default=100)
+    ap.add_argument("--train-docs", type=int, default=200)
+    ap.add_argument("--no-variants", action="store_true")
     args = ap.parse_args()
     mfma = args.cdist == "mfma"
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -154,22 +179,33 @@ def main() -> int:
     log(f"workload: {qs.n} queries (sum q {int(qs.lengths().sum())}) x {corpus.N} docs, "
         f"nnz {corpus.nnz}, generated in {time.perf_counter() - t0:.1f}s")
 
+    learned = None
+    need_mlp = args.warm == "learned" or not args.no_variants
+    if need_mlp:
+        learned = train_warm_start(P, dg, corpus.vecs, lamda, args.train_queries, args.train_docs)
+
     eng = P.WMDEngine(0)
     eng.set_corpus(corpus.vecs, corpus.word_major())
     eng.set_queries(qs.q_ptr, qs.words, qs.weights)
+    if learned is not None:
+        eng.mlp_set(learned["params"])
     log("corpus and queries resident on the GPU")
+    kw = dict(max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=args.omega,
+              warm_start=args.warm == "learned")
 
     warm = None
     for w in range(max(1, args.warmup)):
-        warm = eng.run(lamda, max_iter=max_iter, tol=args.tol, summarize=True, mfma_cdist=mfma)
-        log(f"warmup {w}: {warm['ms_total']:.2f} ms (cdist {warm['ms_cdist']:.2f}, solve "
-            f"{warm['ms_solve']:.2f}), iters mean {warm['iters_mean']:.2f} max {warm['iters_max']}")
-    cd = sv = 0.0
+        warm = eng.run(lamda, summarize=True, **kw)
+        log(f"warmup {w}: {warm['ms_total']:.2f} ms (cdist {warm['ms_cdist']:.2f}, warm start "
+            f"{warm['ms_warm']:.2f}, solve {warm['ms_solve']:.2f}), iters mean "
+            f"{warm['iters_mean']:.2f} max {warm['iters_max']}")
+    cd = sv = wm = 0.0
     t_start = time.perf_counter()
     for s in range(args.steps):
-        st = eng.run(lamda, max_iter=max_iter, tol=args.tol, summarize=False, mfma_cdist=mfma)
+        st = eng.run(lamda, summarize=False, **kw)
         cd += st["ms_cdist"]
         sv += st["ms_solve"]
+        wm += st["ms_warm"]
     t_end = time.perf_counter()
     elapsed = t_end - t_start
     pairs = qs.n * corpus.N
@@ -200,6 +236,26 @@ def main() -> int:
                           "frac": cdist_issued / (cd_ms * 1e-3) / 1e12 / 2500.0,
                           "note": "6 BF16 products per FP32 dot (three-way split); issued "
                                   "FLOPs count 64-column query tiles and D padded to 16"})
+    variants = {}
+    if not args.no_variants:
+        # Each configuration: one untimed run, one timed run (device events).
+        for name, extra in (("batched, uniform start, omega 1", dict(omega=1.0, warm_start=False)),
+                            ("batched, uniform start, omega 1.5", dict(omega=1.5, warm_start=False)),
+                            ("batched, learned start, omega 1", dict(omega=1.0, warm_start=True)),
+                            ("batched, learned start, omega 1.5", dict(omega=1.5, warm_start=True))):
+            k2 = dict(kw, **extra)
+            eng.run(lamda, summarize=False, **k2)
+            t1 = time.perf_counter()
+            stv = eng.run(lamda, summarize=True, **k2)
+            dt = time.perf_counter() - t1
+            variants[name] = {"pairs_per_s": pairs / stv["ms_total"] * 1e3,
+                              "ms_total": stv["ms_total"], "ms_warm_start": stv["ms_warm"],
+                              "ms_solve": stv["ms_solve"], "iters_mean": stv["iters_mean"],
+                              "iters_max": stv["iters_max"], "fallbacks": stv["fallbacks"]}
+            log(f"variant {name}: {variants[name]['pairs_per_s']:.4g} pairs/s, iters mean "
+                f"{stv['iters_mean']:.2f} max {stv['iters_max']}, fallbacks {stv['fallbacks']}")
+        eng.run(lamda, summarize=False, **kw)  # leave the timed configuration's state
+
     plain = None
     if not args.no_plain and args.plain_queries > 0:
         nq = min(args.plain_queries, qs.n)
@@ -223,8 +279,17 @@ def main() -> int:
 
     cpu = cpu_port = None
     if not args.no_cpu_baseline:
+        # The reference iterates plain Sinkhorn: size its fixed iteration count
+        # from a uniform-start, omega = 1 solve of the same pairs.
+        eng2 = P.WMDEngine(0)
+        eng2.set_corpus(corpus.vecs, corpus.word_major())
+        wq, vq = qs.query(0)
+        eng2.set_queries(np.array([0, wq.shape[0]]), wq, vq)
+        eng2.run(lamda, max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=1.0)
+        iters_plain = eng2.result()[1]
+        eng2.close()
         threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-        cpu = cpu_baseline(corpus, qs, iters, lamda, args.cpu_seconds, threads)
+        cpu = cpu_baseline(corpus, qs, iters_plain, lamda, args.cpu_seconds, threads)
         log(f"cpu baseline ({cpu['kind']}, {cpu['cores']} threads): {cpu['value']:.4g} pairs/s")
         cpu_port = cpu_port_per_pair(corpus, qs, lamda, args.tol, max_iter, threads,
                                      min(corpus.N, 5000))
@@ -240,8 +305,8 @@ def main() -> int:
                          f"V={corpus.V}, D={corpus.D} FP32, lambda={-lamda:g}, per-pair stop at "
                          f"marginal residual {args.tol:g}"),
             "queries": qs.n, "docs": corpus.N, "nnz": corpus.nnz,
-            "sum_query_words": int(qs.lengths().sum()), "warm_start": "uniform",
-            "variant": "resident", "iters_mean": float(iters.mean()),
+            "sum_query_words": int(qs.lengths().sum()), "warm_start": args.warm,
+            "omega": args.omega, "variant": "resident", "iters_mean": float(iters.mean()),
             "iters_max": int(iters.max()),
         },
         "roofline": {
@@ -253,7 +318,10 @@ def main() -> int:
             "sinkhorn_alg_bytes_per_step": bytes_alg,
             "sinkhorn_alg_gbs": bytes_alg / (solve_ms * 1e-3) / 1e9,
         },
-        "stages_ms": {"cdist": cd / args.steps, "solve": solve_ms},
+        "stages_ms": {"cdist": cd / args.steps, "warm_start": wm / args.steps, "solve": solve_ms},
+        "variants": variants,
+        "learned_training": None if learned is None else
+        {k: v for k, v in learned.items() if k != "params"},
         "cdist_stage": cdist_obj,
         "cpu_baseline": cpu,
         "cpu_port_per_pair_stop": cpu_port,
