@@ -143,6 +143,25 @@ def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=4):
             "epoch_loss": [float(x) for x in losses], "seconds": round(dt, 2)}
 
 
+def hbm_copy(mib: int, reps: int = 5) -> int:
+    """Device-to-device copies of a known size: the scale for the FETCH_SIZE and
+    WRITE_SIZE counters (a copy of B bytes should read B and write B)."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    assert torch.equal(dst[-4:], src[-4:])
+    print(json.dumps({"copy_bytes": n * 4, "reps": reps, "ms": dt * 1e3,
+                      "gbs_read_plus_write": 2 * n * 4 / dt / 1e9}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,7 +181,11 @@ def main() -> int:
     ap.add_argument("--train-queries", type=int, default=100)
     ap.add_argument("--train-docs", type=int, default=200)
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--hbm-copy-mib", type=int, default=0,
+                    help="only time a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.hbm_copy_mib > 0:
+        return hbm_copy(args.hbm_copy_mib)
     mfma = args.cdist == "mfma"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

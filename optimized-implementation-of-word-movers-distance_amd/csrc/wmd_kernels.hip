@@ -118,6 +118,52 @@ __device__ __forceinline__ float group_red(float v) {
 template <int G>
 __device__ __forceinline__ float group_sum(float v) { return group_red<G, kSum>(v); }
 
+// Step-major forms for N independent values: every butterfly step is applied
+// to all values before the next step, so the cross-lane moves of different
+// values interleave and no value's DPP read waits on the add just before it.
+template <int G, int N>
+__device__ __forceinline__ void group_sum_n(float (&v)[N]) {
+  if constexpr (G >= 2) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<1>(v[n]);
+  }
+  if constexpr (G >= 4) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<2>(v[n]);
+  }
+  if constexpr (G >= 8) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] += dpp_mov<0x141>(v[n]);  // row_half_mirror
+  }
+  if constexpr (G >= 16) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<8>(v[n]);
+  }
+  if constexpr (G >= 32) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
+  }
+  if constexpr (G >= 64) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
+  }
+}
+
+template <int LO, int HI, int N>
+__device__ __forceinline__ void bfly_sum_n(float (&v)[N]) {
+  if constexpr (LO <= HI) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<LO>(v[n]);
+    bfly_sum_n<LO * 2, HI, N>(v);
+  }
+}
+
+// A value that every lane of the wave holds bitwise-equal, moved to a scalar
+// register so that branches on it are scalar.
+__device__ __forceinline__ float wave_uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
@@ -333,10 +379,21 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   bool fell_back = !warm_on;
   float res_win = INFINITY;
   bool sor = a.omega > 1.f;
+  // Over-relaxed step u <- u (r / (u x))^omega as exp2(omega (log2 r -
+  // log2 (u x))): no reciprocal; padded slots (r = 0, u = 0) give
+  // exp2(-inf) * 0 = 0.
+  float lr[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) lr[m] = __builtin_amdgcn_logf(r[m]);
   float res_prev = INFINITY;
   int it = 0;
   float res = 0.f;
   float o[NE];
+  // Every partial sum starts at kFltMin instead of 0: a padded slot (c = 0,
+  // r = 0, kernel entries 0) then ends with a sum of a few kFltMin, so its
+  // quotient is exactly 0 without a select, while real sums (bounded below by
+  // the largest entry of a rescaled row, 1, times a scaling) are unchanged
+  // except at exact rounding ties.
   while (it < a.max_iter) {
     if (seeded) {
 #pragma unroll
@@ -346,23 +403,27 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
         o[k] = (c[k] > 0.f) ? __expf(fminf(fmaxf(lv, -60.f), 60.f)) : 0.f;
       }
     } else {
+      float s[NE];
 #pragma unroll
       for (int k = 0; k < NE; ++k) {
-        float s = 0.f;
+        float acc = kFltMin;
 #pragma unroll
-        for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
-        s = group_sum<B>(s);
-        o[k] = (c[k] > 0.f) ? fdiv(c[k], s) : 0.f;
+        for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
+        s[k] = acc;
       }
+      group_sum_n<B>(s);
+#pragma unroll
+      for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
     }
     float xs[NI];
 #pragma unroll
     for (int m = 0; m < NI; ++m) {
-      float x = 0.f;
+      float x = kFltMin;
 #pragma unroll
       for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
-      xs[m] = bfly_sum<B, 32>(x);
+      xs[m] = x;
     }
+    bfly_sum_n<B, 32>(xs);
     if constexpr (W > 1) {
       float* buf = xsh + (it & 1) * (W * NI * B);
       if (lane < B) {
@@ -380,26 +441,30 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       // Double buffering: the next write to this half happens two
       // iterations later, after every wave has passed the next barrier.
     }
+    float ux[NI];
+#pragma unroll
+    for (int m = 0; m < NI; ++m) ux[m] = u[m] * xs[m];
     if (!seeded) {
+      // Row-marginal residual of the current plan (padded slots: 0 * x - 0).
+      // Every lane holds the same sum, so the stop test is a scalar branch.
       float part = 0.f;
 #pragma unroll
-      for (int m = 0; m < NI; ++m)
-        if (r[m] > 0.f) part += fabsf(u[m] * xs[m] - r[m]);
-      res = group_sum<B>(part);
+      for (int m = 0; m < NI; ++m) part += fabsf(ux[m] - r[m]);
+      res = wave_uniform(group_sum<B>(part));
     }
     if (sor && !seeded) {
-      // Over-relaxed step u <- u (r / (u x))^omega; a rise in the residual
-      // switches the pair to plain steps for good.
+      // A rise in the residual switches the pair to plain steps for good.
       if (res > res_prev) sor = false;
       res_prev = res;
     }
     if (sor && !seeded) {
 #pragma unroll
       for (int m = 0; m < NI; ++m)
-        u[m] = (r[m] > 0.f) ? u[m] * exp2f(a.omega * __log2f(fdiv(r[m], u[m] * xs[m]))) : 0.f;
+        u[m] *= __builtin_amdgcn_exp2f(
+            a.omega * (lr[m] - __builtin_amdgcn_logf(fmaxf(ux[m], kFltMin))));
     } else {
 #pragma unroll
-      for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? fdiv(r[m], xs[m]) : 0.f;
+      for (int m = 0; m < NI; ++m) u[m] = fdiv(r[m], xs[m]);
     }
     ++it;
     if (seeded) {
