@@ -164,6 +164,66 @@ __device__ __forceinline__ float wave_uniform(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
+// Reduce-scatter across the 16-lane rows of a wave.  v_permlane32_swap(X, Y)
+// exchanges the high half of X with the low half of Y, so the sum of its two
+// results holds X summed over both halves in the low half and Y summed in the
+// high half: two values reduced per swap (v_permlane16_swap does the same for
+// even/odd rows).  After the 32 then 16 levels, the lane in row rho holds in
+// out[k] the full sum of v[own_slot<2>(k, rho)] = v[4k + 2 (rho & 1) + (rho >> 1)];
+// after the 32 level alone, of v[own_slot<1>(k, rho)] = v[2k + (rho >> 1)].
+template <int XL>
+__device__ __forceinline__ int own_slot(int k, int rho) {
+  if constexpr (XL == 2) return 4 * k + 2 * (rho & 1) + (rho >> 1);
+  else if constexpr (XL == 1) return 2 * k + (rho >> 1);
+  else return k;
+}
+
+template <int N>
+__device__ __forceinline__ void halves_reduce_scatter(const float (&v)[N], float (&out)[N / 2]) {
+  static_assert(N % 2 == 0, "pairs of values");
+#pragma unroll
+  for (int j = 0; j < N / 2; ++j) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[2 * j]),
+                                                    __float_as_uint(v[2 * j + 1]), false, false);
+    out[j] = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void rows_reduce_scatter(const float (&v)[N], float (&out)[N / 4]) {
+  static_assert(N % 4 == 0, "quads of values");
+  float y[N / 2];
+  halves_reduce_scatter<N>(v, y);
+#pragma unroll
+  for (int k = 0; k < N / 4; ++k) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(y[2 * k]),
+                                                    __float_as_uint(y[2 * k + 1]), false, false);
+    out[k] = __uint_as_float(p[0]) + __uint_as_float(p[1]);
+  }
+}
+
+// Inverse distribution of rows_reduce_scatter: every lane receives all N
+// values, each from the row that owns it.  A swap of two copies of one value
+// returns the even-row (low-half) copy and the odd-row (high-half) copy.
+template <int N>
+__device__ __forceinline__ void rows_gather(const float (&own)[N / 4], float (&v)[N]) {
+  float y[N / 2];
+#pragma unroll
+  for (int k = 0; k < N / 4; ++k) {
+    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(own[k]),
+                                                    __float_as_uint(own[k]), false, false);
+    y[2 * k] = __uint_as_float(p[0]);
+    y[2 * k + 1] = __uint_as_float(p[1]);
+  }
+#pragma unroll
+  for (int j = 0; j < N / 2; ++j) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(y[j]), __float_as_uint(y[j]),
+                                                    false, false);
+    v[2 * j] = __uint_as_float(p[0]);
+    v[2 * j + 1] = __uint_as_float(p[1]);
+  }
+}
+
 // Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
@@ -319,10 +379,18 @@ template <int W, int B, int NI, int NE>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
   constexpr int AW = 64 / B;
   constexpr int A = W * AW;
-  __shared__ float xsh[(W > 1) ? 2 * W * NI * B : 1];
+  // In-wave x reduction over the AW lanes of a query word: its xor masks 32
+  // and 16 cross 16-lane rows and run as a reduce-scatter (XL swap levels),
+  // the masks below 16 as DPP all-reduce steps.  NIP pads NI to a multiple of
+  // 2^XL; after the scatter a lane owns NO of the NIP sums.
+  constexpr int XL = (B <= 16) ? 2 : (B == 32 ? 1 : 0);
+  constexpr int NIP = (NI + (1 << XL) - 1) >> XL << XL;
+  constexpr int NO = NIP >> XL;
+  __shared__ float xsh[(W > 1) ? 2 * W * NIP * B : 1];
   __shared__ double dsh[(W > 1) ? W : 1];
 
   const int lane = threadIdx.x & 63;
+  const int rho = lane >> 4;  // 16-lane row
   // The wave index is uniform; readfirstlane keeps everything derived from it
   // (pair, q, n, offsets) in scalar registers.
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -366,6 +434,19 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     const int i = b + B * m;
     r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
     u[m] = (i < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
+  }
+  // One wave per pair: the lane updates only the NO scalings it owns after
+  // the x reduce-scatter (slot k <-> query-word slot own_slot<XL>(k, rho)) and
+  // gathers the rest; uo, ro, lro are its owned u, r, log2 r.
+  float uo[NO], ro[NO], lro[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const int m = own_slot<XL>(k, rho);
+    const int i = b + B * m;
+    const bool live = m < NI && i < q;
+    ro[k] = live ? a.q_r[qb + i] : 0.f;
+    lro[k] = __builtin_amdgcn_logf(ro[k]);
+    uo[k] = live ? (float)q : 0.f;
   }
 
   // Learned warm start: the first pass takes v from the predicted log v
@@ -415,56 +496,99 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 #pragma unroll
       for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
     }
-    float xs[NI];
+    float xs[NIP];
 #pragma unroll
-    for (int m = 0; m < NI; ++m) {
+    for (int m = 0; m < NIP; ++m) {
       float x = kFltMin;
+      if (m < NI) {
 #pragma unroll
-      for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
+        for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
+      }
       xs[m] = x;
     }
-    bfly_sum_n<B, 32>(xs);
-    if constexpr (W > 1) {
-      float* buf = xsh + (it & 1) * (W * NI * B);
-      if (lane < B) {
+    if constexpr (W == 1) {
+      // x_i for the owned slots: scatter across rows, then the in-row masks.
+      float xo[NO];
+      rows_reduce_scatter<NIP>(xs, xo);
+      bfly_sum_n<B, 8>(xo);
+      if (!seeded) {
+        // Row-marginal residual of the current plan (padded slots: 0 * x -
+        // 0), summed over the owned slots, the B lanes and the four rows.
+        // Every lane holds the same sum, so the stop test is a scalar branch.
+        float part = 0.f;
 #pragma unroll
-        for (int m = 0; m < NI; ++m) buf[(wv * NI + m) * B + b] = xs[m];
+        for (int k = 0; k < NO; ++k) part += fabsf(uo[k] * xo[k] - ro[k]);
+        part = xor_step<32>(xor_step<16>(group_sum<B>(part)));
+        res = wave_uniform(part);
+      }
+      if (sor && !seeded) {
+        // A rise in the residual switches the pair to plain steps for good.
+        if (res > res_prev) sor = false;
+        res_prev = res;
+      }
+      if (sor && !seeded) {
+#pragma unroll
+        for (int k = 0; k < NO; ++k)
+          uo[k] *= __builtin_amdgcn_exp2f(
+              a.omega * (lro[k] - __builtin_amdgcn_logf(fmaxf(uo[k] * xo[k], kFltMin))));
+      } else {
+#pragma unroll
+        for (int k = 0; k < NO; ++k) uo[k] = fdiv(ro[k], xo[k]);
+      }
+      float ug[NIP];
+      rows_gather<NIP>(uo, ug);
+#pragma unroll
+      for (int m = 0; m < NI; ++m) u[m] = ug[m];
+    } else {
+      // Scatter inside the wave, then the W partial sums of every slot
+      // through LDS (each lane writes the slots it owns, reads all).
+      float* buf = xsh + (it & 1) * (W * NIP * B);
+      if constexpr (XL == 2) {
+        float xo[NIP / 4];
+        rows_reduce_scatter<NIP>(xs, xo);
+#pragma unroll
+        for (int k = 0; k < NIP / 4; ++k) buf[(wv * NIP + own_slot<2>(k, rho)) * B + b] = xo[k];
+      } else if constexpr (XL == 1) {
+        float xo[NIP / 2];
+        halves_reduce_scatter<NIP>(xs, xo);
+#pragma unroll
+        for (int k = 0; k < NIP / 2; ++k) buf[(wv * NIP + own_slot<1>(k, rho)) * B + b] = xo[k];
+      } else {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) buf[(wv * NIP + m) * B + b] = xs[m];
       }
       __syncthreads();
 #pragma unroll
       for (int m = 0; m < NI; ++m) {
         float x = 0.f;
 #pragma unroll
-        for (int w = 0; w < W; ++w) x += buf[(w * NI + m) * B + b];
+        for (int w = 0; w < W; ++w) x += buf[(w * NIP + m) * B + b];
         xs[m] = x;
       }
       // Double buffering: the next write to this half happens two
       // iterations later, after every wave has passed the next barrier.
-    }
-    float ux[NI];
+      float ux[NI];
 #pragma unroll
-    for (int m = 0; m < NI; ++m) ux[m] = u[m] * xs[m];
-    if (!seeded) {
-      // Row-marginal residual of the current plan (padded slots: 0 * x - 0).
-      // Every lane holds the same sum, so the stop test is a scalar branch.
-      float part = 0.f;
+      for (int m = 0; m < NI; ++m) ux[m] = u[m] * xs[m];
+      if (!seeded) {
+        float part = 0.f;
 #pragma unroll
-      for (int m = 0; m < NI; ++m) part += fabsf(ux[m] - r[m]);
-      res = wave_uniform(group_sum<B>(part));
-    }
-    if (sor && !seeded) {
-      // A rise in the residual switches the pair to plain steps for good.
-      if (res > res_prev) sor = false;
-      res_prev = res;
-    }
-    if (sor && !seeded) {
+        for (int m = 0; m < NI; ++m) part += fabsf(ux[m] - r[m]);
+        res = wave_uniform(group_sum<B>(part));
+      }
+      if (sor && !seeded) {
+        if (res > res_prev) sor = false;
+        res_prev = res;
+      }
+      if (sor && !seeded) {
 #pragma unroll
-      for (int m = 0; m < NI; ++m)
-        u[m] *= __builtin_amdgcn_exp2f(
-            a.omega * (lr[m] - __builtin_amdgcn_logf(fmaxf(ux[m], kFltMin))));
-    } else {
+        for (int m = 0; m < NI; ++m)
+          u[m] *= __builtin_amdgcn_exp2f(
+              a.omega * (lr[m] - __builtin_amdgcn_logf(fmaxf(ux[m], kFltMin))));
+      } else {
 #pragma unroll
-      for (int m = 0; m < NI; ++m) u[m] = fdiv(r[m], xs[m]);
+        for (int m = 0; m < NI; ++m) u[m] = fdiv(r[m], xs[m]);
+      }
     }
     ++it;
     if (seeded) {
@@ -476,6 +600,8 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       if (res >= res_win) {
 #pragma unroll
         for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? (float)q : 0.f;
+#pragma unroll
+        for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
         fell_back = true;
         if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
       } else {
