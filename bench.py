@@ -181,6 +181,8 @@ def main() -> int:
     ap.add_argument("--train-queries", type=int, default=100)
     ap.add_argument("--train-docs", type=int, default=200)
     ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="smallest-fitting tile shapes instead of autotuned ones")
     ap.add_argument("--hbm-copy-mib", type=int, default=0,
                     help="only time a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -213,6 +215,10 @@ def main() -> int:
     if learned is not None:
         eng.mlp_set(learned["params"])
     log("corpus and queries resident on the GPU")
+    tune_ms = None
+    if not args.no_autotune:
+        tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma)
+        log(f"tile shapes autotuned in {tune_ms:.0f} ms")
     kw = dict(max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=args.omega,
               warm_start=args.warm == "learned")
 
@@ -329,7 +335,8 @@ def main() -> int:
                          f"marginal residual {args.tol:g}"),
             "queries": qs.n, "docs": corpus.N, "nnz": corpus.nnz,
             "sum_query_words": int(qs.lengths().sum()), "warm_start": args.warm,
-            "omega": args.omega, "variant": "resident", "iters_mean": float(iters.mean()),
+            "omega": args.omega, "variant": "resident", "autotune_ms": tune_ms,
+            "iters_mean": float(iters.mean()),
             "iters_max": int(iters.max()),
         },
         "roofline": {

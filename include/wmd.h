@@ -235,6 +235,27 @@ int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const 
 int wmd_num_shapes(void);
 int wmd_shape_info(int shape, int32_t* waves, int32_t* cap_q, int32_t* cap_n);
 
+/* ---- Tile-shape autotuner (no reference counterpart).  The resident variant
+ * runs each (query, document length class) group with one register tile
+ * shape.  By default a query takes the smallest shape that holds its words;
+ * the autotuner instead times, per document class and per group of queries
+ * sharing that default, the default and up to three larger shapes (capacity
+ * at most twice the default's) on a sample of the context's pairs at `iters`
+ * fixed iterations, and keeps the fastest for later runs with shape = -1 in
+ * the options' domain.  It runs the distance stage first and leaves no valid
+ * result (run again before wmd_context_result).  ms_out (may be NULL)
+ * receives the tuning time.  Tables are [n_classes][max_q + 1] shape ids
+ * (-1: smallest that fits) per domain, document class c holding documents of
+ * at most 32 (c + 1) words. */
+int wmd_context_autotune(wmd_context* ctx, double lamda, const wmd_options* opt, int32_t iters,
+                         double* ms_out);
+int wmd_tuning_dims(int32_t* n_classes, int32_t* max_q);
+int wmd_context_get_tuning(wmd_context* ctx, int32_t domain, int32_t* table);
+/* Load a table (e.g. saved from an earlier autotune); NULL restores the
+ * default.  Entries must hold their q (and have a log-domain form for the
+ * log domain). */
+int wmd_context_set_tuning(wmd_context* ctx, int32_t domain, const int32_t* table);
+
 /* Human-readable description of the last error on this thread. */
 const char* wmd_last_error(void);
 

@@ -42,7 +42,8 @@ EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_
            "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_last_error",
            "wmd_mlp_param_count", "wmd_context_mlp_set", "wmd_context_mlp_get",
            "wmd_train_options_default", "wmd_context_mlp_train", "wmd_context_duals",
-           "wmd_mlp_loss_grad", "wmd_context_set_warm")
+           "wmd_mlp_loss_grad", "wmd_context_set_warm", "wmd_context_autotune",
+           "wmd_tuning_dims", "wmd_context_get_tuning", "wmd_context_set_tuning")
 
 
 class Options(C.Structure):
@@ -119,11 +120,18 @@ def load():
     L.wmd_context_duals.argtypes = [P, P]
     L.wmd_context_set_warm.argtypes = [P, P]
     L.wmd_mlp_loss_grad.argtypes = [C.c_int, U64, U64, P, P, U64, P, P, P, P]
+    L.wmd_context_autotune.argtypes = [P, C.c_double, C.POINTER(Options), C.c_int32,
+                                       C.POINTER(C.c_double)]
+    L.wmd_tuning_dims.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.wmd_context_get_tuning.argtypes = [P, C.c_int32, P]
+    L.wmd_context_set_tuning.argtypes = [P, C.c_int32, P]
     for name in ("wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
                  "wmd_context_create", "wmd_context_set_corpus", "wmd_context_set_queries",
                  "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info",
                  "wmd_context_mlp_set", "wmd_context_mlp_get", "wmd_context_mlp_train",
-                 "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm"):
+                 "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm",
+                 "wmd_context_autotune", "wmd_tuning_dims", "wmd_context_get_tuning",
+                 "wmd_context_set_tuning"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -139,6 +139,39 @@ class WMDEngine:
                   "wmd_context_run")
         return st.as_dict()
 
+    # ---- tile-shape autotuner ----
+    def autotune(self, lamda, iters=16, log_domain=False, mfma_cdist=True) -> float:
+        """Time candidate tile shapes on the current corpus and queries and keep
+        the fastest per (document class, query-length bucket); returns ms."""
+        o = nat.default_options(domain=nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR,
+                                cdist=nat.CDIST_MFMA if mfma_cdist else nat.CDIST_FP32)
+        ms = C.c_double()
+        nat.check(self._L.wmd_context_autotune(self._h, float(lamda), C.byref(o), int(iters),
+                                               C.byref(ms)), "wmd_context_autotune")
+        return ms.value
+
+    def get_tuning(self, log_domain=False):
+        """[n_classes, max_q + 1] shape ids used with shape=-1."""
+        ncls, maxq = tuning_dims()
+        t = np.empty((ncls, maxq + 1), dtype=np.int32)
+        nat.check(self._L.wmd_context_get_tuning(
+            self._h, nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR, t.ctypes.data),
+            "wmd_context_get_tuning")
+        return t
+
+    def set_tuning(self, table, log_domain=False):
+        """Load a table from get_tuning (None restores the default)."""
+        dom = nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR
+        if table is None:
+            nat.check(self._L.wmd_context_set_tuning(self._h, dom, None), "wmd_context_set_tuning")
+            return
+        ncls, maxq = tuning_dims()
+        t = np.ascontiguousarray(table, dtype=np.int32)
+        if t.shape != (ncls, maxq + 1):
+            raise ValueError(f"tuning table must be {ncls} x {maxq + 1}")
+        nat.check(self._L.wmd_context_set_tuning(self._h, dom, t.ctypes.data),
+                  "wmd_context_set_tuning")
+
     # ---- learned warm start ----
     def mlp_set(self, params):
         params = np.ascontiguousarray(params, dtype=np.float32)
@@ -228,6 +261,13 @@ def mlp_loss_grad(X, y, group_ptr, params, device=0):
                                            params.ctypes.data, C.byref(loss), grad.ctypes.data),
               "wmd_mlp_loss_grad")
     return loss.value, grad
+
+
+def tuning_dims():
+    """(document classes, largest query length) of a tuning table."""
+    n, q = C.c_int32(), C.c_int32()
+    nat.check(nat.load().wmd_tuning_dims(C.byref(n), C.byref(q)), "wmd_tuning_dims")
+    return n.value, q.value
 
 
 def shapes():

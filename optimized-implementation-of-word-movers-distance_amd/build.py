@@ -3,7 +3,10 @@
     python -m <package>.build        (or __graft_entry__.build())
 
 The library is the product: the C ABI of include/wmd.h over the kernels in
-csrc/.  Packed-FP32 instruction selection is disabled for all device code.
+csrc/.  Every translation unit compiles to an object in parallel (the
+resident-solver shapes are spread over four units), then one link makes the
+shared library.  Packed-FP32 instruction selection is disabled for all
+device code.
 """
 from __future__ import annotations
 
@@ -11,11 +14,15 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libwmd.so")
-SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in ("wmd_kernels.hip", "wmd_mlp.hip", "wmd_capi.hip")]
+UNITS = ("wmd_kernels.hip", "wmd_resident_0.hip", "wmd_resident_1.hip", "wmd_resident_2.hip",
+         "wmd_resident_3.hip", "wmd_mlp.hip", "wmd_capi.hip")
+SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in UNITS]
 ARCH = os.environ.get("WMD_OFFLOAD_ARCH", "gfx950")
 
 
@@ -26,21 +33,39 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def compile_command(out: str = LIB_PATH) -> list[str]:
-    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+def compile_command(src: str, obj: str) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-            "-I", os.path.join(REPO, "include"), *SOURCES, "-o", out]
+            "-I", os.path.join(REPO, "include"), "-c", src, "-o", obj]
 
 
-def build(verbose: bool = False) -> str:
-    tmp = LIB_PATH + ".tmp"
-    cmd = compile_command(tmp)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libwmd.so")
-    if verbose:
-        sys.stderr.write(res.stderr)
+def link_command(objs: list[str], out: str) -> list[str]:
+    return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out]
+
+
+def _run(cmd: list[str]) -> subprocess.CompletedProcess:
+    return subprocess.run(cmd, capture_output=True, text=True)
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1, 8)
+    with tempfile.TemporaryDirectory(prefix="wmd_build_") as tmpdir:
+        objs = [os.path.join(tmpdir, os.path.basename(s) + ".o") for s in SOURCES]
+        with ThreadPoolExecutor(max_workers=jobs) as pool:
+            results = list(pool.map(_run, [compile_command(s, o) for s, o in zip(SOURCES, objs)]))
+        failed = [(s, r) for s, r in zip(SOURCES, results) if r.returncode != 0]
+        for s, r in failed:
+            sys.stderr.write(f"--- {os.path.basename(s)}\n{r.stdout}{r.stderr}")
+        if failed:
+            raise RuntimeError("hipcc failed building libwmd.so")
+        if verbose:
+            for r in results:
+                sys.stderr.write(r.stderr)
+        tmp = LIB_PATH + ".tmp"
+        res = _run(link_command(objs, tmp))
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("hipcc failed linking libwmd.so")
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

@@ -7,6 +7,7 @@
 #include "wmd_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -71,11 +72,23 @@ int round_up4(int x) { return (x + 3) & ~3; }
 
 }  // namespace
 
+// Side streams of a context: the resident launches of a run fan out over
+// them so one launch's tail (its slowest pairs) overlaps the next launch.
+constexpr int kSideStreams = 4;
+constexpr int kTuneStride = wmd::kMaxQ + 1;
+
 struct wmd_context {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSideStreams] = {};
   uint32_t* h_status = nullptr;  // pinned
+  // Tile shape per (document class, query length) and domain (0 linear,
+  // 1 log): filled by wmd_context_autotune or wmd_context_set_tuning; -1 or
+  // an empty table means the smallest shape that fits.
+  std::vector<int32_t> tune[2];
 
   // corpus
   bool has_corpus = false;
@@ -170,48 +183,114 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   return a;
 }
 
-int pick_shape(int q, int nmax) {
+// Smallest-capacity tile shape that holds q query words and documents of
+// nmax words (log domain: among the shapes with a log-domain instantiation).
+int pick_shape(int q, int nmax, bool log_domain) {
   for (int s = 0; s < wmd::kNumShapes; ++s)
-    if (wmd::kShapes[s].cap_q() >= q && wmd::kShapes[s].cap_n() >= nmax) return s;
+    if (wmd::kShapes[s].cap_q() >= q && wmd::kShapes[s].cap_n() >= nmax &&
+        (!log_domain || wmd::kShapes[s].log))
+      return s;
   return -1;
+}
+
+bool shape_fits(int s, int q, bool log_domain) {
+  return s >= 0 && s < wmd::kNumShapes && wmd::kShapes[s].cap_q() >= q &&
+         (!log_domain || wmd::kShapes[s].log);
+}
+
+// Tile shape for a query of q words against documents of class cls: the
+// forced shape, else the tuned table's entry, else the smallest that fits.
+int shape_for(const wmd_context* c, int cls, int q, bool log_domain, int forced) {
+  if (forced >= 0) return forced;
+  const std::vector<int32_t>& t = c->tune[log_domain ? 1 : 0];
+  if (!t.empty()) {
+    const int s = t[(size_t)cls * kTuneStride + q];
+    if (shape_fits(s, q, log_domain)) return s;
+  }
+  return pick_shape(q, 0, log_domain);
+}
+
+// One resident launch: queries bucket[off, off + cnt) against the documents
+// doc_perm[d0, d1) of one length class, with one tile shape.
+struct ResidentLaunch {
+  int shape, cls;
+  size_t off;
+  int64_t cnt, d0, d1;
+  double work;  // padded tile entries, the list-scheduling weight
+};
+
+void add_launch(std::vector<ResidentLaunch>* plan, std::vector<int32_t>* flat,
+                const std::vector<int32_t>& queries, int s, int cls, int64_t d0, int64_t d1) {
+  if (queries.empty() || d1 == d0) return;
+  ResidentLaunch l{s, cls, flat->size(), (int64_t)queries.size(), d0, d1, 0.0};
+  l.work = (double)l.cnt * (double)(d1 - d0) * wmd::kShapes[s].cap_q() * (cls + 1);
+  flat->insert(flat->end(), queries.begin(), queries.end());
+  plan->push_back(l);
+}
+
+// Every (document class, tile shape) group of the current queries; the
+// bucket array receives the query ids of each group (query-major pairs, so
+// concurrently running waves share one query's kernel rows in cache).
+int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
+                  std::vector<ResidentLaunch>* plan) {
+  std::vector<int32_t> flat;
+  plan->clear();
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
+    const int64_t d0 = c->h_class_off[cls], d1 = c->h_class_off[cls + 1];
+    if (d1 == d0) continue;
+    std::vector<std::vector<int32_t>> groups(wmd::kNumShapes);
+    for (int Q = 0; Q < c->nq; ++Q) {
+      const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
+      groups[shape_for(c, cls, q, log_domain, forced_shape)].push_back(Q);
+    }
+    for (int s = 0; s < wmd::kNumShapes; ++s) add_launch(plan, &flat, groups[s], s, cls, d0, d1);
+  }
+  HIP_TRY(c->bucket.upload(flat.data(), flat.size(), c->stream));
+  return WMD_OK;
+}
+
+hipError_t launch_one(wmd_context* c, const wmd::SolveArgs& base, const ResidentLaunch& l,
+                      bool log_domain, hipStream_t s) {
+  wmd::SolveArgs a = base;
+  a.bucket_q = c->bucket.p + l.off;
+  a.doc_list = c->doc_perm.p + l.d0;
+  a.n_list = (int)(l.d1 - l.d0);
+  a.n_pairs = l.cnt * (l.d1 - l.d0);
+  return wmd::launch_resident(l.shape, l.cls, log_domain, a, s);
+}
+
+// Launches a plan on the side streams: largest first, each to the stream
+// with the least queued work (longest-processing-time list scheduling), so a
+// launch's tail of slow pairs overlaps other launches.  The context stream
+// forks before and joins after.
+int launch_plan(wmd_context* c, const wmd::SolveArgs& base,
+                const std::vector<ResidentLaunch>& plan, bool log_domain, int* launches) {
+  std::vector<size_t> order(plan.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return plan[x].work > plan[y].work; });
+  HIP_TRY(hipEventRecord(c->fork, c->stream));
+  for (int k = 0; k < kSideStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->side[k], c->fork, 0));
+  double load[kSideStreams] = {};
+  for (size_t i : order) {
+    const int k = (int)(std::min_element(load, load + kSideStreams) - load);
+    load[k] += plan[i].work;
+    HIP_TRY(launch_one(c, base, plan[i], log_domain, c->side[k]));
+    ++*launches;
+  }
+  for (int k = 0; k < kSideStreams; ++k) {
+    HIP_TRY(hipEventRecord(c->join[k], c->side[k]));
+    HIP_TRY(hipStreamWaitEvent(c->stream, c->join[k], 0));
+  }
+  return WMD_OK;
 }
 
 int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, bool log_domain,
                  int* launches) {
-  // Group queries by tile shape; one launch per (shape, document-length
-  // class) with both non-empty.  Pairs are query-major inside a launch so the
-  // concurrently running waves share one query's kernel rows in cache.
-  std::vector<std::vector<int32_t>> groups(wmd::kNumShapes);
-  for (int Q = 0; Q < c->nq; ++Q) {
-    const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
-    int s = forced_shape >= 0 ? forced_shape : pick_shape(q, c->nmax);
-    groups[s].push_back(Q);
-  }
-  std::vector<int32_t> flat;
-  std::vector<std::pair<int, size_t>> spans;  // shape, offset
-  for (int s = 0; s < wmd::kNumShapes; ++s) {
-    if (groups[s].empty()) continue;
-    spans.push_back({s, flat.size()});
-    flat.insert(flat.end(), groups[s].begin(), groups[s].end());
-  }
-  HIP_TRY(c->bucket.upload(flat.data(), flat.size(), c->stream));
-  for (size_t g = 0; g < spans.size(); ++g) {
-    const int s = spans[g].first;
-    const size_t off = spans[g].second;
-    const size_t cnt = groups[s].size();
-    for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
-      const int64_t d0 = c->h_class_off[cls], d1 = c->h_class_off[cls + 1];
-      if (d1 == d0) continue;
-      wmd::SolveArgs a = base;
-      a.bucket_q = c->bucket.p + off;
-      a.doc_list = c->doc_perm.p + d0;
-      a.n_list = (int)(d1 - d0);
-      a.n_pairs = (int64_t)cnt * (d1 - d0);
-      HIP_TRY(wmd::launch_resident(s, cls, log_domain, a, c->stream));
-      ++*launches;
-    }
-  }
-  return WMD_OK;
+  std::vector<ResidentLaunch> plan;
+  int rc = plan_resident(c, forced_shape, log_domain, &plan);
+  if (rc != WMD_OK) return rc;
+  return launch_plan(c, base, plan, log_domain, launches);
 }
 
 // Streaming variants: iterations launched from a captured HIP graph.
@@ -586,6 +665,11 @@ int wmd_context_create(int device, wmd_context** out) {
   c->device = device;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
+  for (int i = 0; i < kSideStreams && e == hipSuccess; ++i) {
+    e = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 4 * sizeof(uint32_t), hipHostMallocDefault);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device);
   if (e != hipSuccess) {
@@ -615,6 +699,12 @@ void wmd_context_destroy(wmd_context* c) {
   c->status.release();
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (int i = 0; i < kSideStreams; ++i) {
+    if (c->side[i]) (void)hipStreamSynchronize(c->side[i]);
+    if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+    if (c->join[i]) (void)hipEventDestroy(c->join[i]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -876,13 +966,14 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     for (int Q = 0; Q < c->nq && fits; ++Q) {
       const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
       if (o.shape >= 0)
-        fits = wmd::kShapes[o.shape].cap_q() >= q && wmd::kShapes[o.shape].cap_n() >= c->nmax;
+        fits = shape_fits(o.shape, q, log_domain) && wmd::kShapes[o.shape].cap_n() >= c->nmax;
       else
-        fits = pick_shape(q, c->nmax) >= 0;
+        fits = pick_shape(q, c->nmax, log_domain) >= 0;
     }
     if (!fits) {
       if (o.shape >= 0) {
-        set_err("tile shape %d cannot hold q or the longest document (%d words)", o.shape, c->nmax);
+        set_err("tile shape %d cannot hold q or the longest document (%d words), or has no "
+                "log-domain form", o.shape, c->nmax);
         return WMD_ERR_UNSUPPORTED;
       }
       variant = WMD_VARIANT_FUSED;  // documents or queries beyond the register tiles
@@ -1021,6 +1112,149 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     stats->iters_mean = np ? sum / (double)np : 0.0;
     stats->resid_max = rmax;
   }
+  return WMD_OK;
+}
+
+int wmd_tuning_dims(int32_t* n_classes, int32_t* max_q) {
+  if (n_classes) *n_classes = wmd::kNumDocClasses;
+  if (max_q) *max_q = wmd::kMaxQ;
+  return WMD_OK;
+}
+
+namespace {
+
+std::vector<int32_t> default_tuning(bool log_domain) {
+  std::vector<int32_t> t((size_t)wmd::kNumDocClasses * kTuneStride, -1);
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls)
+    for (int q = 1; q <= wmd::kMaxQ; ++q) t[(size_t)cls * kTuneStride + q] = pick_shape(q, 0, log_domain);
+  return t;
+}
+
+}  // namespace
+
+int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in, int32_t iters,
+                         double* ms_out) {
+  const auto t_start = std::chrono::steady_clock::now();
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  wmd_options o;
+  if (opt_in) o = *opt_in; else wmd_options_default(&o);
+  if (iters < 1 || iters > 4096) {
+    set_err("tuning iterations must be in [1, 4096]");
+    return WMD_ERR_ARG;
+  }
+  // The distance stage of a zero-iteration run leaves K and mrow for lamda in
+  // the options' domain (and validates corpus, queries and options).
+  o.variant = WMD_VARIANT_RESIDENT;
+  o.max_iter = 0;
+  o.tol = 0.0;
+  o.shape = -1;
+  o.warm_start = WMD_WARM_UNIFORM;
+  o.export_duals = 0;
+  o.summarize = 0;
+  if (c->has_corpus && c->has_queries &&
+      (c->nmax > wmd::kShapes[0].cap_n() || c->qmax > wmd::kMaxQ)) {
+    set_err("documents (%d words) or queries (%d words) beyond the resident tiles", c->nmax,
+            c->qmax);
+    return WMD_ERR_UNSUPPORTED;
+  }
+  rc = wmd_context_run(c, lamda, &o, nullptr);
+  if (rc != WMD_OK) return rc;
+  const bool lg = o.domain == WMD_DOMAIN_LOG;
+  o.max_iter = iters;
+  o.omega = 1.0;
+  wmd::SolveArgs base = base_args(c, lamda, o);
+  base.fallbacks = c->fallbacks.p;
+
+  std::vector<int32_t> table = default_tuning(lg);
+  std::vector<int32_t> ids;
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
+    const int64_t d0 = c->h_class_off[cls], d1 = c->h_class_off[cls + 1];
+    if (d1 == d0) continue;
+    // Buckets: queries sharing their smallest-fitting shape.
+    std::vector<std::vector<int32_t>> groups(wmd::kNumShapes);
+    for (int Q = 0; Q < c->nq; ++Q) {
+      const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
+      groups[pick_shape(q, 0, lg)].push_back(Q);
+    }
+    int prev_cap = 0;
+    for (int s0 = 0; s0 < wmd::kNumShapes; ++s0) {
+      if (lg && !wmd::kShapes[s0].log) continue;
+      const int cap0 = wmd::kShapes[s0].cap_q();
+      const std::vector<int32_t>& g = groups[s0];
+      if (!g.empty()) {
+        int qmax = 0;
+        for (int Q : g) qmax = std::max(qmax, c->h_q_ptr[Q + 1] - c->h_q_ptr[Q]);
+        // Sample of the class's documents: enough pairs to fill the device.
+        const int64_t n_docs = std::min<int64_t>(
+            d1 - d0, std::max<int64_t>(128, 65536 / (int64_t)g.size()));
+        HIP_TRY(c->bucket.upload(g.data(), g.size(), c->stream));
+        ResidentLaunch l{0, cls, 0, (int64_t)g.size(), d0, d0 + n_docs, 0.0};
+        float best_ms = INFINITY;
+        int best = s0, tried = 0;
+        for (int s = s0; s < wmd::kNumShapes && tried < 4; ++s) {
+          if (!shape_fits(s, qmax, lg) || wmd::kShapes[s].cap_q() > 2 * cap0) continue;
+          ++tried;
+          l.shape = s;
+          HIP_TRY(launch_one(c, base, l, lg, c->stream));  // code load, caches
+          HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+          HIP_TRY(launch_one(c, base, l, lg, c->stream));
+          HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+          HIP_TRY(hipEventSynchronize(c->ev[1]));
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+          if (ms < best_ms) {
+            best_ms = ms;
+            best = s;
+          }
+        }
+        for (int q = prev_cap + 1; q <= cap0; ++q) table[(size_t)cls * kTuneStride + q] = best;
+      }
+      prev_cap = cap0;
+    }
+  }
+  c->tune[lg ? 1 : 0] = table;
+  c->ran = false;  // the tuning launches overwrote the results
+  if (ms_out)
+    *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  return WMD_OK;
+}
+
+int wmd_context_get_tuning(wmd_context* c, int32_t domain, int32_t* table) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!table || (domain != WMD_DOMAIN_LINEAR && domain != WMD_DOMAIN_LOG)) {
+    set_err("null table or unknown domain");
+    return WMD_ERR_ARG;
+  }
+  const bool lg = domain == WMD_DOMAIN_LOG;
+  const std::vector<int32_t> t = c->tune[lg ? 1 : 0].empty() ? default_tuning(lg) : c->tune[lg ? 1 : 0];
+  std::memcpy(table, t.data(), t.size() * sizeof(int32_t));
+  return WMD_OK;
+}
+
+int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (domain != WMD_DOMAIN_LINEAR && domain != WMD_DOMAIN_LOG) {
+    set_err("unknown domain %d", domain);
+    return WMD_ERR_ARG;
+  }
+  const bool lg = domain == WMD_DOMAIN_LOG;
+  if (!table) {
+    c->tune[lg ? 1 : 0].clear();
+    return WMD_OK;
+  }
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls)
+    for (int q = 1; q <= wmd::kMaxQ; ++q) {
+      const int s = table[(size_t)cls * kTuneStride + q];
+      if (s != -1 && !shape_fits(s, q, lg)) {
+        set_err("tuning entry (class %d, q %d) = %d does not hold q words%s", cls, q, s,
+                lg ? " or has no log-domain form" : "");
+        return WMD_ERR_ARG;
+      }
+    }
+  c->tune[lg ? 1 : 0].assign(table, table + (size_t)wmd::kNumDocClasses * kTuneStride);
   return WMD_OK;
 }
 

@@ -101,19 +101,53 @@ struct SolveArgs {
 };
 
 // Register-resident tile shapes of the resident solver: W waves per pair,
-// a B-lane group per document word row, NI query-word slots and NE
-// document-word slots per lane.  Capacity: q <= B*NI, n <= (64W/B)*NE.
-// Each launch also fixes a document-length class (documents of at most
-// kDocClassWidth*(c+1) words), which sets the slot count it compiles for.
-struct TileShape {
-  int W, B, NI, NE;
-  int cap_q() const { return B * NI; }
-  int cap_n() const { return (64 * W / B) * NE; }
-};
-constexpr int kNumShapes = 7;
+// a B-lane group per document word row, NI query-word slots per lane, and
+// A = 64W/B document-word lanes.  Each launch also fixes a document-length
+// class c (documents of at most kDocClassWidth*(c+1) words), which sets the
+// lane's document-word slot count NE = kDocClassWidth*(c+1)/A.  Capacity:
+// q <= B*NI, n <= kNumDocClasses*kDocClassWidth.  Query capacities step by
+// one slot per lane (B words), so a query pads to at most B-1 extra words.
+// `log` marks the shapes that also have a log-domain instantiation.
 constexpr int kNumDocClasses = 5;
 constexpr int kDocClassWidth = 32;
-extern const TileShape kShapes[kNumShapes];
+struct TileShape {
+  int W, B, NI, log;
+  constexpr int cap_q() const { return B * NI; }
+  constexpr int cap_n() const { return kNumDocClasses * kDocClassWidth; }
+};
+constexpr int kNumShapes = 24;
+inline constexpr TileShape kShapes[kNumShapes] = {
+    // W  B  NI log     cap_q
+    {1, 2, 2, 0},    //   4
+    {1, 2, 3, 0},    //   6
+    {1, 2, 4, 1},    //   8
+    {1, 4, 3, 0},    //  12
+    {1, 4, 4, 0},    //  16
+    {1, 4, 5, 1},    //  20
+    {1, 8, 3, 0},    //  24
+    {1, 8, 4, 0},    //  32
+    {1, 8, 5, 1},    //  40
+    {1, 8, 6, 0},    //  48
+    {1, 8, 7, 0},    //  56
+    {1, 8, 8, 1},    //  64
+    {2, 16, 5, 0},   //  80
+    {2, 16, 6, 0},   //  96
+    {2, 16, 7, 0},   // 112
+    {2, 16, 8, 1},   // 128
+    {4, 32, 5, 0},   // 160
+    {4, 32, 6, 0},   // 192
+    {4, 32, 7, 0},   // 224
+    {4, 32, 8, 1},   // 256
+    {8, 64, 5, 0},   // 320
+    {8, 64, 6, 0},   // 384
+    {8, 64, 7, 0},   // 448
+    {8, 64, 8, 1},   // 512
+};
+constexpr int kMaxQ = kShapes[kNumShapes - 1].B * kShapes[kNumShapes - 1].NI;
+
+#define WMD_FOR_EACH_SHAPE(X)                                                                 \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)     \
+  X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23)
 
 // ---- learned warm start (wmd_mlp.hip) ----
 constexpr int kMlpHidden = 256;
@@ -178,6 +212,11 @@ hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s);
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s);
 hipError_t launch_resident(int shape, int doc_class, bool log_domain, const SolveArgs& a,
                            hipStream_t s);
+// Per-shape resident launchers (wmd_resident_*.hip); launch_resident dispatches.
+#define WMD_DECLARE_SHAPE(ID) \
+  hipError_t launch_resident_s##ID(int doc_class, bool log_domain, const SolveArgs& a, hipStream_t s);
+WMD_FOR_EACH_SHAPE(WMD_DECLARE_SHAPE)
+#undef WMD_DECLARE_SHAPE
 hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);

@@ -3,10 +3,12 @@
 // Stages (reference: sinkhorn_wmd_fused.cpp:112-412, sinkhorn_wmd_nonfused.cpp:114-318):
 //   k_cdist          distances M = cdist(query words, touched words) in FP32
 //                    direct differences, row-min rescale, exp epilogue -> Kt, mrow
+//   k_cdist_mfma     the same distances on the matrix cores (three-way BF16 split)
 //   k_resident<...>  one (query, doc) pair per wave/workgroup: the pair's q x n
 //                    tile of Kt is gathered once into registers and every
 //                    Sinkhorn iteration plus the final (K o M) pass runs on-chip
-//   k_fused_step     streaming variant, one iteration per launch (SDDMM + SpMM
+//                    (wmd_resident.h, instantiated in wmd_resident_*.hip)
+//   k_fused_step    streaming variant, one iteration per launch (SDDMM + SpMM
 //                    fused per pair, u kept in HBM between launches)
 //   k_sddmm/k_spmm   nonfused streaming variant (w = c / K^T u written to HBM)
 //
@@ -18,224 +20,16 @@
 
 #include <cfloat>
 
-namespace wmd {
+#include "wmd_lanes.h"
 
-const TileShape kShapes[kNumShapes] = {
-    // W  B  NI NE      cap_q cap_n
-    {1, 2, 4, 5},    //   8    160
-    {1, 4, 5, 10},   //  20    160
-    {1, 8, 5, 20},   //  40    160
-    {1, 8, 8, 20},   //  64    160
-    {2, 16, 8, 20},  // 128    160
-    {4, 32, 8, 20},  // 256    160
-    {8, 64, 8, 20},  // 512    160
-};
+namespace wmd {
 
 namespace {
 
-constexpr float kFltMin = 1.17549435e-38f;
-// Base-2 logit of an absent kernel entry: exp2 of it is 0, and sums with
-// finite logits stay finite (no inf - inf).
-constexpr float kLogZero = -1.0e30f;
 // Matrix-core distances: d^2 below this fraction of |a|^2 + |b|^2 is
 // recomputed directly in FP64 (the GEMM form's cancellation error is
 // ~1e-7 (|a|^2 + |b|^2)).
 constexpr double kNearTau = 0.02;
-
-// Cross-lane sums without LDS round trips.  Each step adds a lane's value to
-// its partner's; both lanes of a pair compute the same two operands, so every
-// lane of a group ends with the bitwise-same sum.
-//   xor 1, 2   DPP quad_perm            xor 8   DPP row_ror:8 (16-lane rows)
-//   xor 4      ds_swizzle xor mode      xor 16  v_permlane16_swap (gfx950)
-//   xor 32     v_permlane32_swap (gfx950)
-// A swap of two copies of v returns (row pair low half, row pair high half)
-// in its two results, so their sum is the pair sum in one fixed order.
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-
-// Reduction operators of the cross-lane helpers.
-enum RedOp { kSum = 0, kMax = 1, kMin = 2 };
-
-template <int OP>
-__device__ __forceinline__ float combine(float a, float b) {
-  if constexpr (OP == kMax) return fmaxf(a, b);
-  else if constexpr (OP == kMin) return fminf(a, b);
-  else return a + b;
-}
-
-template <int MASK, int OP = kSum>
-__device__ __forceinline__ float xor_step(float v) {
-  if constexpr (MASK == 1) {
-    return combine<OP>(v, dpp_mov<0xB1>(v));  // quad_perm [1,0,3,2]
-  } else if constexpr (MASK == 2) {
-    return combine<OP>(v, dpp_mov<0x4E>(v));  // quad_perm [2,3,0,1]
-  } else if constexpr (MASK == 4) {
-    return combine<OP>(v, __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x101F)));
-  } else if constexpr (MASK == 8) {
-    return combine<OP>(v, dpp_mov<0x128>(v));  // row_ror:8 == xor 8 inside a 16-lane row
-  } else if constexpr (MASK == 16) {
-    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v),
-                                                    false, false);
-    return combine<OP>(__uint_as_float(p[0]), __uint_as_float(p[1]));
-  } else {
-    static_assert(MASK == 32, "xor masks are powers of two below 64");
-    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v),
-                                                    false, false);
-    return combine<OP>(__uint_as_float(p[0]), __uint_as_float(p[1]));
-  }
-}
-
-// Butterfly all-reduce over xor masks LO, 2LO, ..., HI (no-op when LO > HI).
-template <int LO, int HI, int OP = kSum>
-__device__ __forceinline__ float bfly_red(float v) {
-  if constexpr (LO <= HI) {
-    return bfly_red<LO * 2, HI, OP>(xor_step<LO, OP>(v));
-  } else {
-    return v;
-  }
-}
-
-template <int LO, int HI>
-__device__ __forceinline__ float bfly_sum(float v) { return bfly_red<LO, HI, kSum>(v); }
-
-// All-reduce over aligned groups of G consecutive lanes.  After the xor-1 and
-// xor-2 steps every lane of a quad holds the same value, so the 8-lane step
-// can pair the two quads with DPP row_half_mirror (lane i <-> 7-i) instead of
-// an exact xor-4 swizzle through the LDS unit.
-template <int G, int OP = kSum>
-__device__ __forceinline__ float group_red(float v) {
-  if constexpr (G >= 2) v = xor_step<1, OP>(v);
-  if constexpr (G >= 4) v = xor_step<2, OP>(v);
-  if constexpr (G >= 8) v = combine<OP>(v, dpp_mov<0x141>(v));  // row_half_mirror
-  if constexpr (G >= 16) v = xor_step<8, OP>(v);
-  if constexpr (G >= 32) v = xor_step<16, OP>(v);
-  if constexpr (G >= 64) v = xor_step<32, OP>(v);
-  return v;
-}
-
-template <int G>
-__device__ __forceinline__ float group_sum(float v) { return group_red<G, kSum>(v); }
-
-// Step-major forms for N independent values: every butterfly step is applied
-// to all values before the next step, so the cross-lane moves of different
-// values interleave and no value's DPP read waits on the add just before it.
-template <int G, int N>
-__device__ __forceinline__ void group_sum_n(float (&v)[N]) {
-  if constexpr (G >= 2) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<1>(v[n]);
-  }
-  if constexpr (G >= 4) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<2>(v[n]);
-  }
-  if constexpr (G >= 8) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] += dpp_mov<0x141>(v[n]);  // row_half_mirror
-  }
-  if constexpr (G >= 16) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<8>(v[n]);
-  }
-  if constexpr (G >= 32) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
-  }
-  if constexpr (G >= 64) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
-  }
-}
-
-template <int LO, int HI, int N>
-__device__ __forceinline__ void bfly_sum_n(float (&v)[N]) {
-  if constexpr (LO <= HI) {
-#pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<LO>(v[n]);
-    bfly_sum_n<LO * 2, HI, N>(v);
-  }
-}
-
-// A value that every lane of the wave holds bitwise-equal, moved to a scalar
-// register so that branches on it are scalar.
-__device__ __forceinline__ float wave_uniform(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
-// Reduce-scatter across the 16-lane rows of a wave.  v_permlane32_swap(X, Y)
-// exchanges the high half of X with the low half of Y, so the sum of its two
-// results holds X summed over both halves in the low half and Y summed in the
-// high half: two values reduced per swap (v_permlane16_swap does the same for
-// even/odd rows).  After the 32 then 16 levels, the lane in row rho holds in
-// out[k] the full sum of v[own_slot<2>(k, rho)] = v[4k + 2 (rho & 1) + (rho >> 1)];
-// after the 32 level alone, of v[own_slot<1>(k, rho)] = v[2k + (rho >> 1)].
-template <int XL>
-__device__ __forceinline__ int own_slot(int k, int rho) {
-  if constexpr (XL == 2) return 4 * k + 2 * (rho & 1) + (rho >> 1);
-  else if constexpr (XL == 1) return 2 * k + (rho >> 1);
-  else return k;
-}
-
-template <int N>
-__device__ __forceinline__ void halves_reduce_scatter(const float (&v)[N], float (&out)[N / 2]) {
-  static_assert(N % 2 == 0, "pairs of values");
-#pragma unroll
-  for (int j = 0; j < N / 2; ++j) {
-    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[2 * j]),
-                                                    __float_as_uint(v[2 * j + 1]), false, false);
-    out[j] = __uint_as_float(p[0]) + __uint_as_float(p[1]);
-  }
-}
-
-template <int N>
-__device__ __forceinline__ void rows_reduce_scatter(const float (&v)[N], float (&out)[N / 4]) {
-  static_assert(N % 4 == 0, "quads of values");
-  float y[N / 2];
-  halves_reduce_scatter<N>(v, y);
-#pragma unroll
-  for (int k = 0; k < N / 4; ++k) {
-    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(y[2 * k]),
-                                                    __float_as_uint(y[2 * k + 1]), false, false);
-    out[k] = __uint_as_float(p[0]) + __uint_as_float(p[1]);
-  }
-}
-
-// Inverse distribution of rows_reduce_scatter: every lane receives all N
-// values, each from the row that owns it.  A swap of two copies of one value
-// returns the even-row (low-half) copy and the odd-row (high-half) copy.
-template <int N>
-__device__ __forceinline__ void rows_gather(const float (&own)[N / 4], float (&v)[N]) {
-  float y[N / 2];
-#pragma unroll
-  for (int k = 0; k < N / 4; ++k) {
-    const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(own[k]),
-                                                    __float_as_uint(own[k]), false, false);
-    y[2 * k] = __uint_as_float(p[0]);
-    y[2 * k + 1] = __uint_as_float(p[1]);
-  }
-#pragma unroll
-  for (int j = 0; j < N / 2; ++j) {
-    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(y[j]), __float_as_uint(y[j]),
-                                                    false, false);
-    v[2 * j] = __uint_as_float(p[0]);
-    v[2 * j + 1] = __uint_as_float(p[1]);
-  }
-}
-
-// Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
-__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-
-template <int LO, int HI>
-__device__ __forceinline__ double bfly_sum_d(double v) {
-  if constexpr (LO <= HI) {
-    v += __shfl_xor(v, LO, 64);
-    return bfly_sum_d<LO * 2, HI>(v);
-  } else {
-    return v;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // cdist + exp epilogue.  Block: 64 touched words x one query, query words in
@@ -359,487 +153,6 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
   }
 }
 
-// ---------------------------------------------------------------------------
-// Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
-// query-word lanes.  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
-// i = b + B*m (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
-//   s_e = sum_i K[e,i] u_i      (partial over own i, butterfly over the B lanes)
-//   o_e = c_e / s_e
-//   x_i = sum_e K[e,i] o_e      (partial over own e, butterfly over the A lanes,
-//                                LDS across waves when W > 1)
-//   res = sum_i |u_i x_i - r_i| (row-marginal residual of the current plan)
-//   u_i = r_i / x_i             (= 1 / x of the reference's K_over_r form)
-// then the final half-step and sum_{i,e} u_i K M o_e in FP64 (:339-393), with
-// M recovered from the rescaled kernel as mrow + log(Kt) / lamda.
-//
-// A launch covers one tile shape and one document-length class: NE is the
-// class's slot count, so the iteration loop has no slot guards.  Slots past
-// the document's or the query's end hold K = 0 and contribute nothing.
-template <int W, int B, int NI, int NE>
-__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
-  constexpr int AW = 64 / B;
-  constexpr int A = W * AW;
-  // In-wave x reduction over the AW lanes of a query word: its xor masks 32
-  // and 16 cross 16-lane rows and run as a reduce-scatter (XL swap levels),
-  // the masks below 16 as DPP all-reduce steps.  NIP pads NI to a multiple of
-  // 2^XL; after the scatter a lane owns NO of the NIP sums.
-  constexpr int XL = (B <= 16) ? 2 : (B == 32 ? 1 : 0);
-  constexpr int NIP = (NI + (1 << XL) - 1) >> XL << XL;
-  constexpr int NO = NIP >> XL;
-  __shared__ float xsh[(W > 1) ? 2 * W * NIP * B : 1];
-  __shared__ double dsh[(W > 1) ? W : 1];
-
-  const int lane = threadIdx.x & 63;
-  const int rho = lane >> 4;  // 16-lane row
-  // The wave index is uniform; readfirstlane keeps everything derived from it
-  // (pair, q, n, offsets) in scalar registers.
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
-  if (pair >= a.n_pairs) return;  // W == 1: per wave (no block barriers); W > 1: whole block
-  const int b = lane & (B - 1);
-  const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
-
-  const int Q = a.bucket_q[pair / a.n_list];
-  const int j = a.doc_list[pair % a.n_list];
-  const int qb = a.q_ptr[Q];
-  const int q = a.q_ptr[Q + 1] - qb;
-  const int ks = a.k_stride[Q];
-  const float* Kq = a.K + a.k_off[Q];
-  const int db = a.doc_ptr[j];
-  const int n = a.doc_ptr[j + 1] - db;
-
-  float Kt[NE][NI];
-  float c[NE];
-#pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = ea + A * k;
-    const bool ve = e < n;
-    int t = 0;
-    float cv = 0.f;
-    if (ve) {
-      t = a.doc_t[db + e];
-      cv = a.doc_val[db + e];
-    }
-    c[k] = cv;
-    const float* row = Kq + (int64_t)t * ks;
-#pragma unroll
-    for (int m = 0; m < NI; ++m) {
-      const int i = b + B * m;
-      Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
-    }
-  }
-  float u[NI], r[NI];
-#pragma unroll
-  for (int m = 0; m < NI; ++m) {
-    const int i = b + B * m;
-    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
-    u[m] = (i < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
-  }
-  // One wave per pair: the lane updates only the NO scalings it owns after
-  // the x reduce-scatter (slot k <-> query-word slot own_slot<XL>(k, rho)) and
-  // gathers the rest; uo, ro, lro are its owned u, r, log2 r.
-  float uo[NO], ro[NO], lro[NO];
-#pragma unroll
-  for (int k = 0; k < NO; ++k) {
-    const int m = own_slot<XL>(k, rho);
-    const int i = b + B * m;
-    const bool live = m < NI && i < q;
-    ro[k] = live ? a.q_r[qb + i] : 0.f;
-    lro[k] = __builtin_amdgcn_logf(ro[k]);
-    uo[k] = live ? (float)q : 0.f;
-  }
-
-  // Learned warm start: the first pass takes v from the predicted log v
-  // (one half-step, counted as an iteration) instead of from u = q.  A
-  // warm-started pair whose residual stalls (no decrease over a 64-iteration
-  // window) restarts once from the uniform start.  Slow but steady progress
-  // is not a stall: such pairs are slow from any start.
-  const bool use_tol = a.tol > 0.f;
-  const bool warm_on = a.warm != nullptr;
-  bool seeded = warm_on;
-  bool fell_back = !warm_on;
-  float res_win = INFINITY;
-  bool sor = a.omega > 1.f;
-  // Over-relaxed step u <- u (r / (u x))^omega as exp2(omega (log2 r -
-  // log2 (u x))): no reciprocal; padded slots (r = 0, u = 0) give
-  // exp2(-inf) * 0 = 0.
-  float lr[NI];
-#pragma unroll
-  for (int m = 0; m < NI; ++m) lr[m] = __builtin_amdgcn_logf(r[m]);
-  float res_prev = INFINITY;
-  int it = 0;
-  float res = 0.f;
-  float o[NE];
-  // Every partial sum starts at kFltMin instead of 0: a padded slot (c = 0,
-  // r = 0, kernel entries 0) then ends with a sum of a few kFltMin, so its
-  // quotient is exactly 0 without a select, while real sums (bounded below by
-  // the largest entry of a rescaled row, 1, times a scaling) are unchanged
-  // except at exact rounding ties.
-  while (it < a.max_iter) {
-    if (seeded) {
-#pragma unroll
-      for (int k = 0; k < NE; ++k) {
-        const int e = ea + A * k;
-        const float lv = (c[k] > 0.f) ? a.warm[(int64_t)Q * a.nnz + db + e] : 0.f;
-        o[k] = (c[k] > 0.f) ? __expf(fminf(fmaxf(lv, -60.f), 60.f)) : 0.f;
-      }
-    } else {
-      float s[NE];
-#pragma unroll
-      for (int k = 0; k < NE; ++k) {
-        float acc = kFltMin;
-#pragma unroll
-        for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
-        s[k] = acc;
-      }
-      group_sum_n<B>(s);
-#pragma unroll
-      for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
-    }
-    float xs[NIP];
-#pragma unroll
-    for (int m = 0; m < NIP; ++m) {
-      float x = kFltMin;
-      if (m < NI) {
-#pragma unroll
-        for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
-      }
-      xs[m] = x;
-    }
-    if constexpr (W == 1) {
-      // x_i for the owned slots: scatter across rows, then the in-row masks.
-      float xo[NO];
-      rows_reduce_scatter<NIP>(xs, xo);
-      bfly_sum_n<B, 8>(xo);
-      if (!seeded) {
-        // Row-marginal residual of the current plan (padded slots: 0 * x -
-        // 0), summed over the owned slots, the B lanes and the four rows.
-        // Every lane holds the same sum, so the stop test is a scalar branch.
-        float part = 0.f;
-#pragma unroll
-        for (int k = 0; k < NO; ++k) part += fabsf(uo[k] * xo[k] - ro[k]);
-        part = xor_step<32>(xor_step<16>(group_sum<B>(part)));
-        res = wave_uniform(part);
-      }
-      if (sor && !seeded) {
-        // A rise in the residual switches the pair to plain steps for good.
-        if (res > res_prev) sor = false;
-        res_prev = res;
-      }
-      if (sor && !seeded) {
-#pragma unroll
-        for (int k = 0; k < NO; ++k)
-          uo[k] *= __builtin_amdgcn_exp2f(
-              a.omega * (lro[k] - __builtin_amdgcn_logf(fmaxf(uo[k] * xo[k], kFltMin))));
-      } else {
-#pragma unroll
-        for (int k = 0; k < NO; ++k) uo[k] = fdiv(ro[k], xo[k]);
-      }
-      float ug[NIP];
-      rows_gather<NIP>(uo, ug);
-#pragma unroll
-      for (int m = 0; m < NI; ++m) u[m] = ug[m];
-    } else {
-      // Scatter inside the wave, then the W partial sums of every slot
-      // through LDS (each lane writes the slots it owns, reads all).
-      float* buf = xsh + (it & 1) * (W * NIP * B);
-      if constexpr (XL == 2) {
-        float xo[NIP / 4];
-        rows_reduce_scatter<NIP>(xs, xo);
-#pragma unroll
-        for (int k = 0; k < NIP / 4; ++k) buf[(wv * NIP + own_slot<2>(k, rho)) * B + b] = xo[k];
-      } else if constexpr (XL == 1) {
-        float xo[NIP / 2];
-        halves_reduce_scatter<NIP>(xs, xo);
-#pragma unroll
-        for (int k = 0; k < NIP / 2; ++k) buf[(wv * NIP + own_slot<1>(k, rho)) * B + b] = xo[k];
-      } else {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) buf[(wv * NIP + m) * B + b] = xs[m];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        float x = 0.f;
-#pragma unroll
-        for (int w = 0; w < W; ++w) x += buf[(w * NIP + m) * B + b];
-        xs[m] = x;
-      }
-      // Double buffering: the next write to this half happens two
-      // iterations later, after every wave has passed the next barrier.
-      float ux[NI];
-#pragma unroll
-      for (int m = 0; m < NI; ++m) ux[m] = u[m] * xs[m];
-      if (!seeded) {
-        float part = 0.f;
-#pragma unroll
-        for (int m = 0; m < NI; ++m) part += fabsf(ux[m] - r[m]);
-        res = wave_uniform(group_sum<B>(part));
-      }
-      if (sor && !seeded) {
-        if (res > res_prev) sor = false;
-        res_prev = res;
-      }
-      if (sor && !seeded) {
-#pragma unroll
-        for (int m = 0; m < NI; ++m)
-          u[m] *= __builtin_amdgcn_exp2f(
-              a.omega * (lr[m] - __builtin_amdgcn_logf(fmaxf(ux[m], kFltMin))));
-      } else {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) u[m] = fdiv(r[m], xs[m]);
-      }
-    }
-    ++it;
-    if (seeded) {
-      seeded = false;
-      continue;
-    }
-    if (use_tol && res < a.tol) break;
-    if (!fell_back && (it & 63) == 0) {
-      if (res >= res_win) {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? (float)q : 0.f;
-#pragma unroll
-        for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
-        fell_back = true;
-        if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
-      } else {
-        res_win = res;
-      }
-    }
-  }
-
-  // Final half-step: v = c / (K^T u), then sum u o ((K o M) v) in FP64.
-  double acc = 0.0;
-#pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    float s = 0.f;
-#pragma unroll
-    for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
-    s = group_sum<B>(s);
-    const int e = ea + A * k;
-    if (e < n) {
-      const float oe = fdiv(c[k], s);
-      if (a.duals) a.duals[(int64_t)Q * a.nnz + db + e] = __logf(oe);
-      const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        const float kv = Kt[k][m];
-        if (kv >= kFltMin) {
-          const float Mv = mr + __logf(kv) * a.inv_lamda;
-          acc += (double)u[m] * (double)kv * (double)Mv * (double)oe;
-        }
-      }
-    }
-  }
-  acc = bfly_sum_d<1, 32>(acc);
-  if constexpr (W > 1) {
-    if (lane == 0) dsh[wv] = acc;
-    __syncthreads();
-    acc = 0.0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) acc += dsh[w];
-  }
-  if (lane == 0 && (W == 1 || wv == 0)) {
-    const int64_t out = (int64_t)Q * a.N + j;
-    a.wmd[out] = (n > 0) ? acc : NAN;  // empty column: the reference's 0 * inf
-    a.iters[out] = it;
-    a.resid[out] = res;
-  }
-}
-// ---------------------------------------------------------------------------
-// Log-domain resident solver (large lambda, where exp(lamda M) underflows
-// FP32 even after the row rescale).  Same lane grid and launch structure as
-// k_resident; the tile holds base-2 logits L[e][i] = lamda (M - mrow) log2(e)
-// and the scalings are kept as base-2 potentials f_i = log2 u_i,
-// g_e = log2 v_e.  Per iteration (the log-sum-exp form of the same step,
-// main.py:75-77; the oracle's perdoc_log):
-//   g_e = log2 c_e - LSE2_i(L[e,i] + f_i)
-//   lx_i = LSE2_e(L[e,i] + g_e)
-//   res = sum_i |2^(f_i + lx_i) - r_i|,  f_i = log2 r_i - lx_i
-// where LSE2 is a two-pass (max, then sum of exp2) reduction over the lane
-// groups.  The final sum of P_ie M_ie is formed in FP64 with
-// M = mrow + L ln2 / lamda.
-template <int W, int B, int NI, int NE>
-__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArgs a) {
-  constexpr int AW = 64 / B;
-  constexpr int A = W * AW;
-  // Cross-wave maxima and sums.  Two barriers per iteration make single
-  // buffers safe: every reader of one buffer is done before the other
-  // buffer's barrier, which precedes the next write.
-  __shared__ float xsh[(W > 1) ? 2 * W * NI * B : 1];
-  __shared__ double dsh[(W > 1) ? W : 1];
-
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
-  if (pair >= a.n_pairs) return;
-  const int b = lane & (B - 1);
-  const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
-
-  const int Q = a.bucket_q[pair / a.n_list];
-  const int j = a.doc_list[pair % a.n_list];
-  const int qb = a.q_ptr[Q];
-  const int q = a.q_ptr[Q + 1] - qb;
-  const int ks = a.k_stride[Q];
-  const float* Lq = a.K + a.k_off[Q];
-  const int db = a.doc_ptr[j];
-  const int n = a.doc_ptr[j + 1] - db;
-
-  float L[NE][NI];
-  float lc[NE];
-#pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = ea + A * k;
-    const bool ve = e < n;
-    int t = 0;
-    float l = kLogZero;
-    if (ve) {
-      t = a.doc_t[db + e];
-      l = __log2f(a.doc_val[db + e]);
-    }
-    lc[k] = l;
-    const float* row = Lq + (int64_t)t * ks;
-#pragma unroll
-    for (int m = 0; m < NI; ++m) {
-      const int i = b + B * m;
-      L[k][m] = (ve && i < q) ? row[i] : kLogZero;
-    }
-  }
-  float f[NI], lr[NI], r[NI];
-#pragma unroll
-  for (int m = 0; m < NI; ++m) {
-    const int i = b + B * m;
-    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
-    lr[m] = (i < q) ? __log2f(r[m]) : kLogZero;
-    f[m] = (i < q) ? __log2f((float)q) : kLogZero;  // u = q
-  }
-
-  const bool use_tol = a.tol > 0.f;
-  bool sor = a.omega > 1.f;
-  float res_prev = INFINITY;
-  int it = 0;
-  float res = 0.f;
-  float g[NE];
-  while (it < a.max_iter) {
-#pragma unroll
-    for (int k = 0; k < NE; ++k) {
-      float mx = kLogZero;
-#pragma unroll
-      for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
-      mx = group_red<B, kMax>(mx);
-      float s = 0.f;
-#pragma unroll
-      for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
-      s = group_sum<B>(s);
-      g[k] = (lc[k] > 0.5f * kLogZero) ? lc[k] - mx - __log2f(s) : kLogZero;
-    }
-    float lx[NI];
-#pragma unroll
-    for (int m = 0; m < NI; ++m) {
-      float mx = kLogZero;
-#pragma unroll
-      for (int k = 0; k < NE; ++k) mx = fmaxf(mx, L[k][m] + g[k]);
-      lx[m] = bfly_red<B, 32, kMax>(mx);
-    }
-    float* bufm = xsh;
-    float* bufs = xsh + W * NI * B;
-    if constexpr (W > 1) {
-      if (lane < B) {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) bufm[(wv * NI + m) * B + b] = lx[m];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        float x = kLogZero;
-#pragma unroll
-        for (int w = 0; w < W; ++w) x = fmaxf(x, bufm[(w * NI + m) * B + b]);
-        lx[m] = x;
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < NI; ++m) {
-      float t = 0.f;
-#pragma unroll
-      for (int k = 0; k < NE; ++k) t += exp2f(L[k][m] + g[k] - lx[m]);
-      t = bfly_sum<B, 32>(t);
-      if constexpr (W > 1) {
-        if (lane < B) bufs[(wv * NI + m) * B + b] = t;
-      } else {
-        lx[m] += __log2f(t);
-      }
-    }
-    if constexpr (W > 1) {
-      __syncthreads();
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        float t = 0.f;
-#pragma unroll
-        for (int w = 0; w < W; ++w) t += bufs[(w * NI + m) * B + b];
-        lx[m] += __log2f(t);
-      }
-    }
-    float part = 0.f;
-#pragma unroll
-    for (int m = 0; m < NI; ++m)
-      if (r[m] > 0.f) part += fabsf(exp2f(f[m] + lx[m]) - r[m]);
-    res = group_sum<B>(part);
-    if (sor) {
-      if (res > res_prev) sor = false;
-      res_prev = res;
-    }
-    const float om = sor ? a.omega : 1.f;
-#pragma unroll
-    for (int m = 0; m < NI; ++m)
-      f[m] = (r[m] > 0.f) ? f[m] + om * (lr[m] - lx[m] - f[m]) : kLogZero;
-    ++it;
-    if (use_tol && res < a.tol) break;
-  }
-
-  // Final half-step and sum_{i,e} P_ie M_ie in FP64.
-  const double ln2 = 0.6931471805599453;
-  const double m_per_l = ln2 / (double)a.lamda;
-  double acc = 0.0;
-#pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    float mx = kLogZero;
-#pragma unroll
-    for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
-    mx = group_red<B, kMax>(mx);
-    float s = 0.f;
-#pragma unroll
-    for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
-    s = group_sum<B>(s);
-    const int e = ea + A * k;
-    if (e < n) {
-      const double ge = (double)lc[k] - (double)mx - (double)__log2f(s);
-      const double mr = (double)a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        if (r[m] > 0.f && L[k][m] > 0.5f * kLogZero) {
-          const double lp = (double)L[k][m] + (double)f[m] + ge;
-          acc += exp(lp * ln2) * (mr + (double)L[k][m] * m_per_l);
-        }
-      }
-    }
-  }
-  acc = bfly_sum_d<1, 32>(acc);
-  if constexpr (W > 1) {
-    if (lane == 0) dsh[wv] = acc;
-    __syncthreads();
-    acc = 0.0;
-#pragma unroll
-    for (int w = 0; w < W; ++w) acc += dsh[w];
-  }
-  if (lane == 0 && (W == 1 || wv == 0)) {
-    const int64_t out = (int64_t)Q * a.N + j;
-    a.wmd[out] = (n > 0) ? acc : NAN;
-    a.iters[out] = it;
-    a.resid[out] = res;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // MFMA distance stage.  d^2 = |a|^2 + |b|^2 - 2 a.b with a.b on the matrix
@@ -1196,48 +509,22 @@ hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s) {
   return hipGetLastError();
 }
 
-template <int W, int B, int NI, int NE>
-static hipError_t launch_res(bool log_domain, const SolveArgs& a, hipStream_t s) {
-  const int64_t blocks = (W == 1) ? (a.n_pairs + 3) / 4 : a.n_pairs;
-  if (log_domain)
-    hipLaunchKernelGGL((k_resident_log<W, B, NI, NE>), dim3((unsigned)blocks),
-                       dim3(W == 1 ? 256 : 64 * W), 0, s, a);
-  else
-    hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
-                       dim3(W == 1 ? 256 : 64 * W), 0, s, a);
-  return hipGetLastError();
-}
-
-// Document-length class c holds documents of at most kDocClassWidth*(c+1)
-// words: NE = that length over the shape's A document-word lanes.
-template <int W, int B, int NI>
-static hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
-  constexpr int A = 64 * W / B;
-  constexpr int C = kDocClassWidth;
-  static_assert(kNumDocClasses == 5, "one case per document-length class");
-  switch (cls) {
-    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A>(lg, a, s);
-    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A>(lg, a, s);
-    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A>(lg, a, s);
-    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A>(lg, a, s);
-    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A>(lg, a, s);
-    default: return hipErrorInvalidValue;
-  }
-}
+// Resident launchers, one per tile shape, defined in the wmd_resident_*.hip
+// translation units.
+namespace {
+using ResidentLaunch = hipError_t (*)(int, bool, const SolveArgs&, hipStream_t);
+#define WMD_SHAPE_FN(ID) launch_resident_s##ID,
+constexpr ResidentLaunch kResidentLaunch[kNumShapes] = {WMD_FOR_EACH_SHAPE(WMD_SHAPE_FN)};
+#undef WMD_SHAPE_FN
+}  // namespace
 
 hipError_t launch_resident(int shape, int doc_class, bool log_domain, const SolveArgs& a,
                            hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
-  switch (shape) {
-    case 0: return launch_shape<1, 2, 4>(doc_class, log_domain, a, s);
-    case 1: return launch_shape<1, 4, 5>(doc_class, log_domain, a, s);
-    case 2: return launch_shape<1, 8, 5>(doc_class, log_domain, a, s);
-    case 3: return launch_shape<1, 8, 8>(doc_class, log_domain, a, s);
-    case 4: return launch_shape<2, 16, 8>(doc_class, log_domain, a, s);
-    case 5: return launch_shape<4, 32, 8>(doc_class, log_domain, a, s);
-    case 6: return launch_shape<8, 64, 8>(doc_class, log_domain, a, s);
-    default: return hipErrorInvalidValue;
-  }
+  if (shape < 0 || shape >= kNumShapes || doc_class < 0 || doc_class >= kNumDocClasses)
+    return hipErrorInvalidValue;
+  if (log_domain && !kShapes[shape].log) return hipErrorInvalidValue;
+  return kResidentLaunch[shape](doc_class, log_domain, a, s);
 }
 
 static size_t stream_lds(const SolveArgs& a) {

@@ -1,0 +1,540 @@
+// Register-resident Sinkhorn solvers (linear and log domain) and their
+// launch templates.  Included by the wmd_resident_*.hip translation units,
+// each of which instantiates a share of the tile shapes so the shapes
+// compile in parallel.
+#pragma once
+
+#include "wmd_internal.h"
+#include "wmd_lanes.h"
+
+#include <cfloat>
+
+namespace wmd {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
+// query-word lanes.  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
+// i = b + B*m (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
+//   s_e = sum_i K[e,i] u_i      (partial over own i, butterfly over the B lanes)
+//   o_e = c_e / s_e
+//   x_i = sum_e K[e,i] o_e      (partial over own e, butterfly over the A lanes,
+//                                LDS across waves when W > 1)
+//   res = sum_i |u_i x_i - r_i| (row-marginal residual of the current plan)
+//   u_i = r_i / x_i             (= 1 / x of the reference's K_over_r form)
+// then the final half-step and sum_{i,e} u_i K M o_e in FP64 (:339-393), with
+// M recovered from the rescaled kernel as mrow + log(Kt) / lamda.
+//
+// A launch covers one tile shape and one document-length class: NE is the
+// class's slot count, so the iteration loop has no slot guards.  Slots past
+// the document's or the query's end hold K = 0 and contribute nothing.
+template <int W, int B, int NI, int NE>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
+  constexpr int AW = 64 / B;
+  constexpr int A = W * AW;
+  // In-wave x reduction over the AW lanes of a query word: its xor masks 32
+  // and 16 cross 16-lane rows and run as a reduce-scatter (XL swap levels),
+  // the masks below 16 as DPP all-reduce steps.  NIP pads NI to a multiple of
+  // 2^XL; after the scatter a lane owns NO of the NIP sums.
+  constexpr int XL = (B <= 16) ? 2 : (B == 32 ? 1 : 0);
+  constexpr int NIP = (NI + (1 << XL) - 1) >> XL << XL;
+  constexpr int NO = NIP >> XL;
+  __shared__ float xsh[(W > 1) ? 2 * W * NIP * B : 1];
+  __shared__ double dsh[(W > 1) ? W : 1];
+
+  const int lane = threadIdx.x & 63;
+  const int rho = lane >> 4;  // 16-lane row
+  // The wave index is uniform; readfirstlane keeps everything derived from it
+  // (pair, q, n, offsets) in scalar registers.
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
+  if (pair >= a.n_pairs) return;  // W == 1: per wave (no block barriers); W > 1: whole block
+  const int b = lane & (B - 1);
+  const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
+
+  const int Q = a.bucket_q[pair / a.n_list];
+  const int j = a.doc_list[pair % a.n_list];
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  const float* Kq = a.K + a.k_off[Q];
+  const int db = a.doc_ptr[j];
+  const int n = a.doc_ptr[j + 1] - db;
+
+  float Kt[NE][NI];
+  float c[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = ea + A * k;
+    const bool ve = e < n;
+    int t = 0;
+    float cv = 0.f;
+    if (ve) {
+      t = a.doc_t[db + e];
+      cv = a.doc_val[db + e];
+    }
+    c[k] = cv;
+    const float* row = Kq + (int64_t)t * ks;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      const int i = b + B * m;
+      Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
+    }
+  }
+  float u[NI], r[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) {
+    const int i = b + B * m;
+    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
+    u[m] = (i < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
+  }
+  // One wave per pair: the lane updates only the NO scalings it owns after
+  // the x reduce-scatter (slot k <-> query-word slot own_slot<XL>(k, rho)) and
+  // gathers the rest; uo, ro, lro are its owned u, r, log2 r.
+  float uo[NO], ro[NO], lro[NO];
+#pragma unroll
+  for (int k = 0; k < NO; ++k) {
+    const int m = own_slot<XL>(k, rho);
+    const int i = b + B * m;
+    const bool live = m < NI && i < q;
+    ro[k] = live ? a.q_r[qb + i] : 0.f;
+    lro[k] = __builtin_amdgcn_logf(ro[k]);
+    uo[k] = live ? (float)q : 0.f;
+  }
+
+  // Learned warm start: the first pass takes v from the predicted log v
+  // (one half-step, counted as an iteration) instead of from u = q.  A
+  // warm-started pair whose residual stalls (no decrease over a 64-iteration
+  // window) restarts once from the uniform start.  Slow but steady progress
+  // is not a stall: such pairs are slow from any start.
+  const bool use_tol = a.tol > 0.f;
+  const bool warm_on = a.warm != nullptr;
+  bool seeded = warm_on;
+  bool fell_back = !warm_on;
+  float res_win = INFINITY;
+  bool sor = a.omega > 1.f;
+  // Over-relaxed step u <- u (r / (u x))^omega as exp2(omega (log2 r -
+  // log2 (u x))): no reciprocal; padded slots (r = 0, u = 0) give
+  // exp2(-inf) * 0 = 0.
+  float lr[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) lr[m] = __builtin_amdgcn_logf(r[m]);
+  float res_prev = INFINITY;
+  int it = 0;
+  float res = 0.f;
+  float o[NE];
+  // Every partial sum starts at kFltMin instead of 0: a padded slot (c = 0,
+  // r = 0, kernel entries 0) then ends with a sum of a few kFltMin, so its
+  // quotient is exactly 0 without a select, while real sums (bounded below by
+  // the largest entry of a rescaled row, 1, times a scaling) are unchanged
+  // except at exact rounding ties.
+  while (it < a.max_iter) {
+    if (seeded) {
+#pragma unroll
+      for (int k = 0; k < NE; ++k) {
+        const int e = ea + A * k;
+        const float lv = (c[k] > 0.f) ? a.warm[(int64_t)Q * a.nnz + db + e] : 0.f;
+        o[k] = (c[k] > 0.f) ? __expf(fminf(fmaxf(lv, -60.f), 60.f)) : 0.f;
+      }
+    } else {
+      float s[NE];
+#pragma unroll
+      for (int k = 0; k < NE; ++k) {
+        float acc = kFltMin;
+#pragma unroll
+        for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
+        s[k] = acc;
+      }
+      group_sum_n<B>(s);
+#pragma unroll
+      for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
+    }
+    float xs[NIP];
+#pragma unroll
+    for (int m = 0; m < NIP; ++m) {
+      float x = kFltMin;
+      if (m < NI) {
+#pragma unroll
+        for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
+      }
+      xs[m] = x;
+    }
+    if constexpr (W == 1) {
+      // x_i for the owned slots: scatter across rows, then the in-row masks.
+      float xo[NO];
+      rows_reduce_scatter<NIP>(xs, xo);
+      bfly_sum_n<B, 8>(xo);
+      if (!seeded) {
+        // Row-marginal residual of the current plan (padded slots: 0 * x -
+        // 0), summed over the owned slots, the B lanes and the four rows.
+        // Every lane holds the same sum, so the stop test is a scalar branch.
+        float part = 0.f;
+#pragma unroll
+        for (int k = 0; k < NO; ++k) part += fabsf(uo[k] * xo[k] - ro[k]);
+        part = xor_step<32>(xor_step<16>(group_sum<B>(part)));
+        res = wave_uniform(part);
+      }
+      if (sor && !seeded) {
+        // A rise in the residual switches the pair to plain steps for good.
+        if (res > res_prev) sor = false;
+        res_prev = res;
+      }
+      if (sor && !seeded) {
+#pragma unroll
+        for (int k = 0; k < NO; ++k)
+          uo[k] *= __builtin_amdgcn_exp2f(
+              a.omega * (lro[k] - __builtin_amdgcn_logf(fmaxf(uo[k] * xo[k], kFltMin))));
+      } else {
+#pragma unroll
+        for (int k = 0; k < NO; ++k) uo[k] = fdiv(ro[k], xo[k]);
+      }
+      float ug[NIP];
+      rows_gather<NIP>(uo, ug);
+#pragma unroll
+      for (int m = 0; m < NI; ++m) u[m] = ug[m];
+    } else {
+      // Scatter inside the wave, then the W partial sums of every slot
+      // through LDS (each lane writes the slots it owns, reads all).
+      float* buf = xsh + (it & 1) * (W * NIP * B);
+      if constexpr (XL == 2) {
+        float xo[NIP / 4];
+        rows_reduce_scatter<NIP>(xs, xo);
+#pragma unroll
+        for (int k = 0; k < NIP / 4; ++k) buf[(wv * NIP + own_slot<2>(k, rho)) * B + b] = xo[k];
+      } else if constexpr (XL == 1) {
+        float xo[NIP / 2];
+        halves_reduce_scatter<NIP>(xs, xo);
+#pragma unroll
+        for (int k = 0; k < NIP / 2; ++k) buf[(wv * NIP + own_slot<1>(k, rho)) * B + b] = xo[k];
+      } else {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) buf[(wv * NIP + m) * B + b] = xs[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        float x = 0.f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) x += buf[(w * NIP + m) * B + b];
+        xs[m] = x;
+      }
+      // Double buffering: the next write to this half happens two
+      // iterations later, after every wave has passed the next barrier.
+      float ux[NI];
+#pragma unroll
+      for (int m = 0; m < NI; ++m) ux[m] = u[m] * xs[m];
+      if (!seeded) {
+        float part = 0.f;
+#pragma unroll
+        for (int m = 0; m < NI; ++m) part += fabsf(ux[m] - r[m]);
+        res = wave_uniform(group_sum<B>(part));
+      }
+      if (sor && !seeded) {
+        if (res > res_prev) sor = false;
+        res_prev = res;
+      }
+      if (sor && !seeded) {
+#pragma unroll
+        for (int m = 0; m < NI; ++m)
+          u[m] *= __builtin_amdgcn_exp2f(
+              a.omega * (lr[m] - __builtin_amdgcn_logf(fmaxf(ux[m], kFltMin))));
+      } else {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) u[m] = fdiv(r[m], xs[m]);
+      }
+    }
+    ++it;
+    if (seeded) {
+      seeded = false;
+      continue;
+    }
+    if (use_tol && res < a.tol) break;
+    if (!fell_back && (it & 63) == 0) {
+      if (res >= res_win) {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? (float)q : 0.f;
+#pragma unroll
+        for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
+        fell_back = true;
+        if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
+      } else {
+        res_win = res;
+      }
+    }
+  }
+
+  // Final half-step: v = c / (K^T u), then sum u o ((K o M) v) in FP64.
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+    s = group_sum<B>(s);
+    const int e = ea + A * k;
+    if (e < n) {
+      const float oe = fdiv(c[k], s);
+      if (a.duals) a.duals[(int64_t)Q * a.nnz + db + e] = __logf(oe);
+      const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        const float kv = Kt[k][m];
+        if (kv >= kFltMin) {
+          const float Mv = mr + __logf(kv) * a.inv_lamda;
+          acc += (double)u[m] * (double)kv * (double)Mv * (double)oe;
+        }
+      }
+    }
+  }
+  acc = bfly_sum_d<1, 32>(acc);
+  if constexpr (W > 1) {
+    if (lane == 0) dsh[wv] = acc;
+    __syncthreads();
+    acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc += dsh[w];
+  }
+  if (lane == 0 && (W == 1 || wv == 0)) {
+    const int64_t out = (int64_t)Q * a.N + j;
+    a.wmd[out] = (n > 0) ? acc : NAN;  // empty column: the reference's 0 * inf
+    a.iters[out] = it;
+    a.resid[out] = res;
+  }
+}
+// ---------------------------------------------------------------------------
+// Log-domain resident solver (large lambda, where exp(lamda M) underflows
+// FP32 even after the row rescale).  Same lane grid and launch structure as
+// k_resident; the tile holds base-2 logits L[e][i] = lamda (M - mrow) log2(e)
+// and the scalings are kept as base-2 potentials f_i = log2 u_i,
+// g_e = log2 v_e.  Per iteration (the log-sum-exp form of the same step,
+// main.py:75-77; the oracle's perdoc_log):
+//   g_e = log2 c_e - LSE2_i(L[e,i] + f_i)
+//   lx_i = LSE2_e(L[e,i] + g_e)
+//   res = sum_i |2^(f_i + lx_i) - r_i|,  f_i = log2 r_i - lx_i
+// where LSE2 is a two-pass (max, then sum of exp2) reduction over the lane
+// groups.  The final sum of P_ie M_ie is formed in FP64 with
+// M = mrow + L ln2 / lamda.
+template <int W, int B, int NI, int NE>
+__global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArgs a) {
+  constexpr int AW = 64 / B;
+  constexpr int A = W * AW;
+  // Cross-wave maxima and sums.  Two barriers per iteration make single
+  // buffers safe: every reader of one buffer is done before the other
+  // buffer's barrier, which precedes the next write.
+  __shared__ float xsh[(W > 1) ? 2 * W * NI * B : 1];
+  __shared__ double dsh[(W > 1) ? W : 1];
+
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t pair = (W == 1) ? (int64_t)blockIdx.x * 4 + wv : (int64_t)blockIdx.x;
+  if (pair >= a.n_pairs) return;
+  const int b = lane & (B - 1);
+  const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
+
+  const int Q = a.bucket_q[pair / a.n_list];
+  const int j = a.doc_list[pair % a.n_list];
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  const float* Lq = a.K + a.k_off[Q];
+  const int db = a.doc_ptr[j];
+  const int n = a.doc_ptr[j + 1] - db;
+
+  float L[NE][NI];
+  float lc[NE];
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    const int e = ea + A * k;
+    const bool ve = e < n;
+    int t = 0;
+    float l = kLogZero;
+    if (ve) {
+      t = a.doc_t[db + e];
+      l = __log2f(a.doc_val[db + e]);
+    }
+    lc[k] = l;
+    const float* row = Lq + (int64_t)t * ks;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      const int i = b + B * m;
+      L[k][m] = (ve && i < q) ? row[i] : kLogZero;
+    }
+  }
+  float f[NI], lr[NI], r[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) {
+    const int i = b + B * m;
+    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
+    lr[m] = (i < q) ? __log2f(r[m]) : kLogZero;
+    f[m] = (i < q) ? __log2f((float)q) : kLogZero;  // u = q
+  }
+
+  const bool use_tol = a.tol > 0.f;
+  bool sor = a.omega > 1.f;
+  float res_prev = INFINITY;
+  int it = 0;
+  float res = 0.f;
+  float g[NE];
+  while (it < a.max_iter) {
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      float mx = kLogZero;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
+      mx = group_red<B, kMax>(mx);
+      float s = 0.f;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
+      s = group_sum<B>(s);
+      g[k] = (lc[k] > 0.5f * kLogZero) ? lc[k] - mx - __log2f(s) : kLogZero;
+    }
+    float lx[NI];
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      float mx = kLogZero;
+#pragma unroll
+      for (int k = 0; k < NE; ++k) mx = fmaxf(mx, L[k][m] + g[k]);
+      lx[m] = bfly_red<B, 32, kMax>(mx);
+    }
+    float* bufm = xsh;
+    float* bufs = xsh + W * NI * B;
+    if constexpr (W > 1) {
+      if (lane < B) {
+#pragma unroll
+        for (int m = 0; m < NI; ++m) bufm[(wv * NI + m) * B + b] = lx[m];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        float x = kLogZero;
+#pragma unroll
+        for (int w = 0; w < W; ++w) x = fmaxf(x, bufm[(w * NI + m) * B + b]);
+        lx[m] = x;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < NI; ++m) {
+      float t = 0.f;
+#pragma unroll
+      for (int k = 0; k < NE; ++k) t += exp2f(L[k][m] + g[k] - lx[m]);
+      t = bfly_sum<B, 32>(t);
+      if constexpr (W > 1) {
+        if (lane < B) bufs[(wv * NI + m) * B + b] = t;
+      } else {
+        lx[m] += __log2f(t);
+      }
+    }
+    if constexpr (W > 1) {
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < W; ++w) t += bufs[(w * NI + m) * B + b];
+        lx[m] += __log2f(t);
+      }
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m)
+      if (r[m] > 0.f) part += fabsf(exp2f(f[m] + lx[m]) - r[m]);
+    res = group_sum<B>(part);
+    if (sor) {
+      if (res > res_prev) sor = false;
+      res_prev = res;
+    }
+    const float om = sor ? a.omega : 1.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m)
+      f[m] = (r[m] > 0.f) ? f[m] + om * (lr[m] - lx[m] - f[m]) : kLogZero;
+    ++it;
+    if (use_tol && res < a.tol) break;
+  }
+
+  // Final half-step and sum_{i,e} P_ie M_ie in FP64.
+  const double ln2 = 0.6931471805599453;
+  const double m_per_l = ln2 / (double)a.lamda;
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < NE; ++k) {
+    float mx = kLogZero;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) mx = fmaxf(mx, L[k][m] + f[m]);
+    mx = group_red<B, kMax>(mx);
+    float s = 0.f;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
+    s = group_sum<B>(s);
+    const int e = ea + A * k;
+    if (e < n) {
+      const double ge = (double)lc[k] - (double)mx - (double)__log2f(s);
+      const double mr = (double)a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        if (r[m] > 0.f && L[k][m] > 0.5f * kLogZero) {
+          const double lp = (double)L[k][m] + (double)f[m] + ge;
+          acc += exp(lp * ln2) * (mr + (double)L[k][m] * m_per_l);
+        }
+      }
+    }
+  }
+  acc = bfly_sum_d<1, 32>(acc);
+  if constexpr (W > 1) {
+    if (lane == 0) dsh[wv] = acc;
+    __syncthreads();
+    acc = 0.0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc += dsh[w];
+  }
+  if (lane == 0 && (W == 1 || wv == 0)) {
+    const int64_t out = (int64_t)Q * a.N + j;
+    a.wmd[out] = (n > 0) ? acc : NAN;
+    a.iters[out] = it;
+    a.resid[out] = res;
+  }
+}
+
+template <int W, int B, int NI, int NE, bool LOG>
+hipError_t launch_res(bool log_domain, const SolveArgs& a, hipStream_t s) {
+  const int64_t blocks = (W == 1) ? (a.n_pairs + 3) / 4 : a.n_pairs;
+  if constexpr (LOG) {
+    if (log_domain) {
+      hipLaunchKernelGGL((k_resident_log<W, B, NI, NE>), dim3((unsigned)blocks),
+                         dim3(W == 1 ? 256 : 64 * W), 0, s, a);
+      return hipGetLastError();
+    }
+  } else {
+    if (log_domain) return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
+                     dim3(W == 1 ? 256 : 64 * W), 0, s, a);
+  return hipGetLastError();
+}
+
+// Document-length class c holds documents of at most kDocClassWidth*(c+1)
+// words: NE = that length over the shape's A document-word lanes.
+template <int W, int B, int NI, bool LOG>
+hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
+  constexpr int A = 64 * W / B;
+  constexpr int C = kDocClassWidth;
+  static_assert(kNumDocClasses == 5, "one case per document-length class");
+  switch (cls) {
+    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A, LOG>(lg, a, s);
+    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A, LOG>(lg, a, s);
+    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A, LOG>(lg, a, s);
+    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A, LOG>(lg, a, s);
+    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A, LOG>(lg, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+// Defines launch_resident_s<ID> for entry ID of kShapes.
+#define WMD_RESIDENT_SHAPE(ID)                                                             \
+  hipError_t launch_resident_s##ID(int cls, bool lg, const SolveArgs& a, hipStream_t s) { \
+    return launch_shape<kShapes[ID].W, kShapes[ID].B, kShapes[ID].NI,                      \
+                        kShapes[ID].log != 0>(cls, lg, a, s);                               \
+  }
+
+}  // namespace wmd

@@ -56,6 +56,14 @@ def test_host_only_entry_points():
     assert caps == sorted(caps) and caps[-1] >= 300  # every batched query length fits
     assert L.wmd_shape_info(99, None, None, None) == nat.WMD_ERR_ARG
     assert b"out of range" in L.wmd_last_error()
+    # query capacities step by one slot per lane group: at most B - 1 padded words
+    for (w, cap, _), (_, prev, _) in zip(shapes[1:], shapes[:-1]):
+        assert cap - prev <= 64 * w // 8 or cap - prev <= 16
+    n_cls, max_q = pkg().api.tuning_dims()
+    assert n_cls == 5 and max_q == caps[-1]
+    assert L.wmd_context_autotune(None, -1.0, None, 8, None) == nat.WMD_ERR_ARG
+    assert L.wmd_context_get_tuning(None, 0, None) == nat.WMD_ERR_ARG
+    assert L.wmd_context_set_tuning(None, 0, None) == nat.WMD_ERR_ARG
 
 
 def test_null_arguments_rejected_before_device_use():

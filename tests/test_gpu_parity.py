@@ -116,9 +116,11 @@ def _batch_queries(dg, V, lengths, seed=11):
 
 
 def test_batched_every_tile_shape(P, dg, oracle):
-    # one query per shape boundary (1..300 words) against a 1500-doc corpus
-    corpus, _, _ = dg.workload("batched", seed=1, N=1500, n_queries=1)
-    lengths = [1, 5, 8, 9, 20, 21, 40, 41, 64, 65, 128, 129, 256, 257, 300]
+    # one query at and one just past every shape's capacity (1..512 words)
+    # against a 600-doc corpus
+    corpus, _, _ = dg.workload("batched", seed=1, N=600, n_queries=1)
+    caps = [s[1] for s in P.shapes()]
+    lengths = sorted({1} | set(caps) | {c + 1 for c in caps[:-1]})
     qs = _batch_queries(dg, corpus.V, lengths)
     vecs = corpus.vecs.astype(np.float64)
     for tol, max_iter in ((0.0, 15), (1e-4, 10000)):
@@ -146,12 +148,47 @@ def test_forced_shapes_agree(P, dg):
     with engine(P, corpus) as e:
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
         ref = None
-        for s in range(len(P.shapes())):
+        for s, (_, cap_q, _) in enumerate(P.shapes()):
+            if cap_q < 7:
+                continue
             e.run(-10.0, max_iter=15, shape=s)
             w = e.result()[0]
             if ref is None:
                 ref = w.copy()
             assert rel_err(w, ref) < 2e-6, s
+
+
+def test_autotuned_shapes_agree(P, dg):
+    say("[case] autotuner: tuned table, same results, table round trip")
+    corpus, _, _ = dg.workload("batched", seed=2, N=2000, n_queries=1)
+    qs = _batch_queries(dg, corpus.V, [3, 11, 30, 45, 70, 150, 260, 400], seed=5)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=40)
+        ref = e.result()[0]
+        default = e.get_tuning()
+        ms = e.autotune(-10.0, iters=8)
+        tuned = e.get_tuning()
+        say(f"  tuning {ms:.1f} ms; entries changed: {int((tuned != default).sum())}")
+        caps = np.array([s[1] for s in P.shapes()])
+        q = np.arange(tuned.shape[1])
+        assert np.all(caps[tuned[:, 1:]] >= q[None, 1:])  # every entry holds its q
+        e.run(-10.0, max_iter=40)
+        assert rel_err(e.result()[0], ref) < 2e-6
+        # a table choosing the largest single-wave shape for every q it holds
+        forced = default.copy()
+        one_wave = max(i for i, s in enumerate(P.shapes()) if s[0] == 1)
+        forced[:, 1:caps[one_wave] + 1] = one_wave
+        e.set_tuning(forced)
+        assert np.array_equal(e.get_tuning(), forced)
+        e.run(-10.0, max_iter=40)
+        assert rel_err(e.result()[0], ref) < 2e-6
+        bad = default.copy()
+        bad[0, caps[0] + 1] = 0  # shape 0 cannot hold cap + 1 words
+        with pytest.raises(P.WMDError):
+            e.set_tuning(bad)
+        e.set_tuning(None)
+        assert np.array_equal(e.get_tuning(), default)
 
 
 def test_closed_forms_on_gpu(P, dg):
@@ -263,9 +300,15 @@ def test_invalid_arguments(P, dg):
         e.set_queries([0, 1], [2], [1.0])
         assert status(lambda: e.run(-1.0, variant=7)) == nat.WMD_ERR_ARG
         e.run(-1.0, shape=0, max_iter=3)  # q = 1 fits the smallest tile
-        big = np.arange(0, 2 * 9, 2)      # 9 words: beyond shape 0's 8-word capacity
-        e.set_queries([0, 9], big, np.full(9, 1.0 / 9))
+        cap0 = P.shapes()[0][1]
+        big = np.arange(0, 2 * (cap0 + 1), 2)  # one word beyond shape 0's capacity
+        e.set_queries([0, cap0 + 1], big, np.full(cap0 + 1, 1.0 / (cap0 + 1)))
         assert status(lambda: e.run(-1.0, shape=0, max_iter=3)) == nat.WMD_ERR_UNSUPPORTED
+        # a shape without a log-domain form is refused in the log domain
+        no_log = 1
+        e.set_queries([0, 1], [2], [1.0])
+        assert status(lambda: e.run(-1.0, shape=no_log, max_iter=3, log_domain=True)) == \
+            nat.WMD_ERR_UNSUPPORTED
 
 
 def test_batched_full_size_properties(P, dg, oracle):
