@@ -162,6 +162,34 @@ def hbm_copy(mib: int, reps: int = 5) -> int:
     return 0
 
 
+def trace_span(path: str, kernel: str = "k_resident", gap_ms: float = 20.0) -> int:
+    """Per-step wall span of one kernel family in a rocprofv3 kernel_trace.csv.
+
+    The solver stage's launches run concurrently on several streams, so the
+    sum of their durations exceeds the stage's time; the union of their
+    [start, end] intervals, split into steps at idle gaps longer than gap_ms,
+    is what the HIP events around the stage measure."""
+    import csv
+    iv = []
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if kernel in row["Kernel_Name"]:
+                iv.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"])))
+    iv.sort()
+    steps, busy = [], []
+    for s, e in iv:
+        if steps and s - steps[-1][1] <= gap_ms * 1e6:
+            steps[-1][1] = max(steps[-1][1], e)
+            busy[-1] += e - s
+        else:
+            steps.append([s, e])
+            busy.append(e - s)
+    for k, ((s, e), b) in enumerate(zip(steps, busy)):
+        print(json.dumps({"step": k, "span_ms": (e - s) / 1e6, "sum_kernel_ms": b / 1e6,
+                          "overlap": b / max(e - s, 1)}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -178,6 +206,8 @@ def main() -> int:
     ap.add_argument("--omega", type=float, default=1.5,
                     help="over-relaxation of the u-update (1 = plain Sinkhorn)")
     ap.add_argument("--warm", choices=("uniform", "learned"), default="uniform")
+    ap.add_argument("--accel", choices=("none", "anderson"), default="anderson",
+                    help="u-update acceleration (anderson overrides --omega)")
     ap.add_argument("--train-queries", type=int, default=100)
     ap.add_argument("--train-docs", type=int, default=200)
     ap.add_argument("--no-variants", action="store_true")
@@ -185,9 +215,13 @@ def main() -> int:
                     help="smallest-fitting tile shapes instead of autotuned ones")
     ap.add_argument("--hbm-copy-mib", type=int, default=0,
                     help="only time a device copy of this many MiB (counter calibration)")
+    ap.add_argument("--trace-span", metavar="KERNEL_TRACE_CSV",
+                    help="only print the per-step span of the solver kernels in a trace")
     args = ap.parse_args()
     if args.hbm_copy_mib > 0:
         return hbm_copy(args.hbm_copy_mib)
+    if args.trace_span:
+        return trace_span(args.trace_span)
     mfma = args.cdist == "mfma"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -215,12 +249,15 @@ def main() -> int:
     if learned is not None:
         eng.mlp_set(learned["params"])
     log("corpus and queries resident on the GPU")
-    tune_ms = None
+    tune_ms = tune_changed = None
     if not args.no_autotune:
+        default_table = eng.get_tuning()
         tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma)
-        log(f"tile shapes autotuned in {tune_ms:.0f} ms")
+        tune_changed = int((eng.get_tuning() != default_table).sum())
+        log(f"tile shapes autotuned in {tune_ms:.0f} ms ({tune_changed} table entries differ "
+            f"from the smallest-fitting shape)")
     kw = dict(max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=args.omega,
-              warm_start=args.warm == "learned")
+              warm_start=args.warm == "learned", accel=args.accel == "anderson")
 
     warm = None
     for w in range(max(1, args.warmup)):
@@ -268,10 +305,13 @@ def main() -> int:
     variants = {}
     if not args.no_variants:
         # Each configuration: one untimed run, one timed run (device events).
-        for name, extra in (("batched, uniform start, omega 1", dict(omega=1.0, warm_start=False)),
-                            ("batched, uniform start, omega 1.5", dict(omega=1.5, warm_start=False)),
-                            ("batched, learned start, omega 1", dict(omega=1.0, warm_start=True)),
-                            ("batched, learned start, omega 1.5", dict(omega=1.5, warm_start=True))):
+        for name, extra in (
+                ("batched, uniform start, omega 1", dict(omega=1.0, warm_start=False, accel=False)),
+                ("batched, uniform start, omega 1.5", dict(omega=1.5, warm_start=False, accel=False)),
+                ("batched, uniform start, Anderson", dict(warm_start=False, accel=True)),
+                ("batched, learned start, omega 1", dict(omega=1.0, warm_start=True, accel=False)),
+                ("batched, learned start, omega 1.5", dict(omega=1.5, warm_start=True, accel=False)),
+                ("batched, learned start, Anderson", dict(warm_start=True, accel=True))):
             k2 = dict(kw, **extra)
             eng.run(lamda, summarize=False, **k2)
             t1 = time.perf_counter()
@@ -335,7 +375,9 @@ def main() -> int:
                          f"marginal residual {args.tol:g}"),
             "queries": qs.n, "docs": corpus.N, "nnz": corpus.nnz,
             "sum_query_words": int(qs.lengths().sum()), "warm_start": args.warm,
-            "omega": args.omega, "variant": "resident", "autotune_ms": tune_ms,
+            "omega": args.omega, "accel": args.accel, "variant": "resident",
+            "autotune_ms": tune_ms,
+            "autotune_entries_changed": tune_changed,
             "iters_mean": float(iters.mean()),
             "iters_max": int(iters.max()),
         },

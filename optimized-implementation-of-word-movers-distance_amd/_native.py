@@ -33,6 +33,8 @@ CDIST_MFMA = 1
 WARM_UNIFORM = 0
 WARM_LEARNED = 1
 WARM_GIVEN = 2
+ACCEL_NONE = 0
+ACCEL_ANDERSON = 1
 MLP_HIDDEN = 256
 
 # Every symbol include/wmd.h declares.
@@ -50,7 +52,7 @@ class Options(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("variant", C.c_int32), ("tol", C.c_double),
                 ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
                 ("domain", C.c_int32), ("cdist", C.c_int32), ("warm_start", C.c_int32),
-                ("export_duals", C.c_int32), ("reserved0", C.c_int32), ("omega", C.c_double),
+                ("export_duals", C.c_int32), ("accel", C.c_int32), ("omega", C.c_double),
                 ("reserved", C.c_int32 * 4)]
 
 

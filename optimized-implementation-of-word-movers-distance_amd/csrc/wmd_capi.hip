@@ -173,6 +173,7 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   a.max_iter = o.max_iter;
   a.tol = (float)o.tol;
   a.omega = (float)o.omega;
+  a.accel = o.accel;
   a.inv_lamda = (float)(1.0 / lamda);
   a.lamda = (float)lamda;
   a.wmd = c->wmd.p;
@@ -630,6 +631,7 @@ void wmd_options_default(wmd_options* opt) {
   opt->cdist = WMD_CDIST_MFMA;
   opt->warm_start = WMD_WARM_UNIFORM;
   opt->export_duals = 0;
+  opt->accel = WMD_ACCEL_NONE;
   opt->omega = 1.0;
 }
 
@@ -908,6 +910,10 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     set_err("omega must be in [1, 2)");
     return WMD_ERR_ARG;
   }
+  if (o.accel != WMD_ACCEL_NONE && o.accel != WMD_ACCEL_ANDERSON) {
+    set_err("unknown accel %d", o.accel);
+    return WMD_ERR_ARG;
+  }
   int variant = o.variant;
   if (variant < WMD_VARIANT_RESIDENT || variant > WMD_VARIANT_NONFUSED) {
     set_err("unknown variant %d", variant);
@@ -981,6 +987,12 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   }
   if ((learned || given || o.export_duals) && (variant != WMD_VARIANT_RESIDENT || log_domain)) {
     set_err("learned warm start and dual export run on the resident variant in the linear domain");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (o.accel != WMD_ACCEL_NONE &&
+      (variant != WMD_VARIANT_RESIDENT || log_domain || !(o.tol > 0.0))) {
+    set_err("Anderson acceleration runs on the resident variant in the linear domain with a "
+            "tolerance (tol > 0)");
     return WMD_ERR_UNSUPPORTED;
   }
   if (log_domain && variant != WMD_VARIANT_RESIDENT) {

@@ -82,6 +82,7 @@ struct SolveArgs {
   int max_iter;
   float tol;               // 0: fixed max_iter iterations
   float omega;             // u-update over-relaxation (1 = plain)
+  int accel;               // 1: Anderson acceleration of the u-update (resident, linear)
   float inv_lamda;         // 1 / lamda
   float lamda;
   double* wmd;             // nq x N

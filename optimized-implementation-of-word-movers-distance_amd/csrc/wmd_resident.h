@@ -13,6 +13,141 @@ namespace wmd {
 namespace {
 
 // ---------------------------------------------------------------------------
+// Sum over every query slot of a pair of per-lane partials (step-major over
+// N values).  W == 1: after the x reduce-scatter the lanes of a row own
+// disjoint slots, so the sum runs over the B lanes and the four rows; W > 1:
+// every lane holds all slots of its query-word lane b, so over the B lanes.
+template <int W, int B, int N>
+__device__ __forceinline__ void slot_sum_n(float (&v)[N]) {
+  group_sum_n<B>(v);
+  if constexpr (W == 1) {
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
+  }
+}
+
+// State of the u-update rule for the N query slots a lane updates.
+template <int N>
+struct UpdateState {
+  bool sor;          // over-relaxed steps still on
+  float res_prev;    // residual of the previous (accepted) step
+  // Anderson acceleration, history 1: centred base-2 potentials l (u = 2^l
+  // up to the scaling gauge) and the previous accepted point and its f.
+  float l[N], pl[N], pf[N];
+  bool prev;         // pl, pf hold a point
+};
+
+template <int N>
+__device__ __forceinline__ void update_state_reset(UpdateState<N>& st, float omega) {
+  st.sor = omega > 1.f;
+  st.res_prev = INFINITY;
+  st.prev = false;
+#pragma unroll
+  for (int n = 0; n < N; ++n) st.l[n] = st.pl[n] = st.pf[n] = 0.f;
+}
+
+// One u-update of the N slots this lane updates, from their x = K o, r and
+// log2 r; returns the row-marginal residual sum_i |u_i x_i - r_i| of the
+// current plan (every lane the same value).  stop is set when the pair meets
+// the tolerance: the plain and over-relaxed rules still take their step
+// (the iterate the reference would hold next), the accelerated rule keeps the
+// verified point.
+//   plain      u = r / x
+//   omega > 1  u <- u (r / (u x))^omega = u exp2(omega (log2 r - log2 (u x)));
+//              a rise in the residual switches the pair to plain steps
+//   Anderson   g = centre(log2 r - log2 x), f = g - l,
+//              l <- l + f - gamma (dl + df), gamma = (df . f) / (df . df),
+//              dl, df the changes since the previous accepted step; a step
+//              whose residual rises (or is not finite) is replaced by the
+//              previous point's plain update, and the history restarts.
+//              Centring pins the gauge (the plan ignores a constant added to
+//              every log2 u_i), so the extrapolation cannot drift along it.
+template <int W, int B, int N>
+__device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState<N>& st,
+                                                 const float (&x)[N], const float (&r)[N],
+                                                 const float (&lr)[N], float (&u)[N], bool seeded,
+                                                 int q, bool& stop) {
+  const bool use_tol = a.tol > 0.f;
+  if (a.accel) {
+    float g[N];
+    float red[2] = {0.f, 0.f};
+#pragma unroll
+    for (int n = 0; n < N; ++n) {
+      red[0] += fabsf(u[n] * x[n] - r[n]);
+      g[n] = (r[n] > 0.f) ? lr[n] - __builtin_amdgcn_logf(x[n]) : 0.f;
+      red[1] += g[n];
+    }
+    slot_sum_n<W, B>(red);
+    const float res = wave_uniform(red[0]);
+    if (!seeded && use_tol && res < a.tol) {
+      stop = true;
+      return res;
+    }
+    // NaN compares false: a non-finite residual is rejected as well.
+    const bool ok = seeded || !st.prev || (res <= st.res_prev && res < INFINITY);
+    if (!ok) {
+#pragma unroll
+      for (int n = 0; n < N; ++n) st.l[n] = st.pl[n] + st.pf[n];
+      st.prev = false;
+    } else {
+      const float mean = wave_uniform(red[1]) / (float)q;
+      float f[N], dl[N], df[N];
+      float d[2] = {0.f, 0.f};
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        f[n] = (r[n] > 0.f) ? g[n] - mean - st.l[n] : 0.f;
+        dl[n] = st.l[n] - st.pl[n];
+        df[n] = f[n] - st.pf[n];
+        d[0] = fmaf(df[n], df[n], d[0]);
+        d[1] = fmaf(df[n], f[n], d[1]);
+      }
+      float gam = 0.f;
+      if (st.prev) {
+        slot_sum_n<W, B>(d);
+        gam = wave_uniform(d[1]) / fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f);
+      }
+#pragma unroll
+      for (int n = 0; n < N; ++n) {
+        st.pl[n] = st.l[n];
+        st.pf[n] = f[n];
+        st.l[n] += f[n] - gam * (dl[n] + df[n]);
+      }
+      st.prev = !seeded;
+      st.res_prev = res;
+    }
+#pragma unroll
+    for (int n = 0; n < N; ++n) u[n] = (r[n] > 0.f) ? __builtin_amdgcn_exp2f(st.l[n]) : 0.f;
+    return res;
+  }
+  float ux[N];
+  float part = 0.f;
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    ux[n] = u[n] * x[n];
+    part += fabsf(ux[n] - r[n]);  // padded slots: 0 * x - 0
+  }
+  float red[1] = {part};
+  slot_sum_n<W, B>(red);
+  const float res = wave_uniform(red[0]);
+  if (st.sor && !seeded) {
+    if (res > st.res_prev) st.sor = false;
+    st.res_prev = res;
+  }
+  if (st.sor && !seeded) {
+#pragma unroll
+    for (int n = 0; n < N; ++n)
+      u[n] *= __builtin_amdgcn_exp2f(a.omega * (lr[n] - __builtin_amdgcn_logf(fmaxf(ux[n], kFltMin))));
+  } else {
+#pragma unroll
+    for (int n = 0; n < N; ++n) u[n] = fdiv(r[n], x[n]);
+  }
+  stop = use_tol && !seeded && res < a.tol;
+  return res;
+}
+
+// ---------------------------------------------------------------------------
 // Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
 // query-word lanes.  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
 // i = b + B*m (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
@@ -107,19 +242,16 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   // warm-started pair whose residual stalls (no decrease over a 64-iteration
   // window) restarts once from the uniform start.  Slow but steady progress
   // is not a stall: such pairs are slow from any start.
-  const bool use_tol = a.tol > 0.f;
   const bool warm_on = a.warm != nullptr;
   bool seeded = warm_on;
   bool fell_back = !warm_on;
   float res_win = INFINITY;
-  bool sor = a.omega > 1.f;
-  // Over-relaxed step u <- u (r / (u x))^omega as exp2(omega (log2 r -
-  // log2 (u x))): no reciprocal; padded slots (r = 0, u = 0) give
-  // exp2(-inf) * 0 = 0.
+  // log2 r (padded slots: -inf, which the update rules map to u = 0).
   float lr[NI];
 #pragma unroll
   for (int m = 0; m < NI; ++m) lr[m] = __builtin_amdgcn_logf(r[m]);
-  float res_prev = INFINITY;
+  UpdateState<(W == 1) ? NO : NI> st;
+  update_state_reset(st, a.omega);
   int it = 0;
   float res = 0.f;
   float o[NE];
@@ -159,35 +291,16 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       }
       xs[m] = x;
     }
+    bool stop = false;
     if constexpr (W == 1) {
       // x_i for the owned slots: scatter across rows, then the in-row masks.
       float xo[NO];
       rows_reduce_scatter<NIP>(xs, xo);
       bfly_sum_n<B, 8>(xo);
-      if (!seeded) {
-        // Row-marginal residual of the current plan (padded slots: 0 * x -
-        // 0), summed over the owned slots, the B lanes and the four rows.
-        // Every lane holds the same sum, so the stop test is a scalar branch.
-        float part = 0.f;
-#pragma unroll
-        for (int k = 0; k < NO; ++k) part += fabsf(uo[k] * xo[k] - ro[k]);
-        part = xor_step<32>(xor_step<16>(group_sum<B>(part)));
-        res = wave_uniform(part);
-      }
-      if (sor && !seeded) {
-        // A rise in the residual switches the pair to plain steps for good.
-        if (res > res_prev) sor = false;
-        res_prev = res;
-      }
-      if (sor && !seeded) {
-#pragma unroll
-        for (int k = 0; k < NO; ++k)
-          uo[k] *= __builtin_amdgcn_exp2f(
-              a.omega * (lro[k] - __builtin_amdgcn_logf(fmaxf(uo[k] * xo[k], kFltMin))));
-      } else {
-#pragma unroll
-        for (int k = 0; k < NO; ++k) uo[k] = fdiv(ro[k], xo[k]);
-      }
+      // The residual is the same on every lane, so the stop test is a
+      // scalar branch.
+      const float rv = update_scalings<W, B>(a, st, xo, ro, lro, uo, seeded, q, stop);
+      if (!seeded) res = rv;
       float ug[NIP];
       rows_gather<NIP>(uo, ug);
 #pragma unroll
@@ -220,41 +333,25 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       }
       // Double buffering: the next write to this half happens two
       // iterations later, after every wave has passed the next barrier.
-      float ux[NI];
+      float xv[NI];
 #pragma unroll
-      for (int m = 0; m < NI; ++m) ux[m] = u[m] * xs[m];
-      if (!seeded) {
-        float part = 0.f;
-#pragma unroll
-        for (int m = 0; m < NI; ++m) part += fabsf(ux[m] - r[m]);
-        res = wave_uniform(group_sum<B>(part));
-      }
-      if (sor && !seeded) {
-        if (res > res_prev) sor = false;
-        res_prev = res;
-      }
-      if (sor && !seeded) {
-#pragma unroll
-        for (int m = 0; m < NI; ++m)
-          u[m] *= __builtin_amdgcn_exp2f(
-              a.omega * (lr[m] - __builtin_amdgcn_logf(fmaxf(ux[m], kFltMin))));
-      } else {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) u[m] = fdiv(r[m], xs[m]);
-      }
+      for (int m = 0; m < NI; ++m) xv[m] = xs[m];
+      const float rv = update_scalings<W, B>(a, st, xv, r, lr, u, seeded, q, stop);
+      if (!seeded) res = rv;
     }
     ++it;
     if (seeded) {
       seeded = false;
       continue;
     }
-    if (use_tol && res < a.tol) break;
+    if (stop) break;
     if (!fell_back && (it & 63) == 0) {
       if (res >= res_win) {
 #pragma unroll
         for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? (float)q : 0.f;
 #pragma unroll
         for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
+        update_state_reset(st, a.omega);
         fell_back = true;
         if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
       } else {

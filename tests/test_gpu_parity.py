@@ -352,3 +352,29 @@ def test_over_relaxation_reaches_the_same_fixed_point(P, dg, oracle, omega):
         f"omega {eo:.2e}")
     assert np.all(reso < 1e-4)
     assert eo <= 2e-4
+
+
+def test_anderson_acceleration_reaches_the_same_fixed_point(P, dg, oracle):
+    say("[case] Anderson-accelerated u-update, residual 1e-4, against converged FP64")
+    corpus, _, _ = dg.workload("batched", seed=3, N=1500, n_queries=1)
+    qs = _batch_queries(dg, corpus.V, [1, 6, 19, 38, 55, 64, 90, 170, 290, 450], seed=9)
+    vecs = corpus.vecs.astype(np.float64)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=20000, tol=1e-4, omega=1.5)
+        ws, its, _ = e.result()
+        e.run(-10.0, max_iter=20000, tol=1e-4, accel=True)
+        wa, ita, resa = e.result()
+        e.run(-10.0, max_iter=20000, tol=1e-4, accel=True)
+        assert np.array_equal(e.result()[0], wa)  # deterministic
+        with pytest.raises(P.WMDError):  # fixed iterations reproduce plain Sinkhorn only
+            e.run(-10.0, max_iter=15, accel=True)
+    truth, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, vecs, corpus.doc_ptr,
+                                      corpus.doc_words, corpus.doc_vals, -10.0, 100000, tol=1e-10)
+    es, ea = rel_err(ws, truth), rel_err(wa, truth)
+    say(f"  iterations omega 1.5 {its.sum()} (max {its.max()}), Anderson {ita.sum()} "
+        f"(max {ita.max()}), ratio {ita.sum() / its.sum():.3f}; error vs converged FP64: "
+        f"omega {es:.2e}, Anderson {ea:.2e}")
+    assert np.all(resa < 1e-4)
+    assert ea <= 2e-4
+    assert ita.sum() < its.sum()
