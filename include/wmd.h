@@ -253,7 +253,7 @@ int wmd_shape_info(int shape, int32_t* waves, int32_t* cap_q, int32_t* cap_n);
  * runs each (query, document length class) group with one register tile
  * shape.  By default a query takes the smallest shape that holds its words;
  * the autotuner instead times, per document class and per group of queries
- * sharing that default, the default and up to three larger shapes (capacity
+ * sharing that default, the default and up to five further shapes (capacity
  * at most twice the default's) on a sample of the context's pairs at `iters`
  * fixed iterations, and keeps the fastest for later runs with shape = -1 in
  * the options' domain.  It runs the distance stage first and leaves no valid

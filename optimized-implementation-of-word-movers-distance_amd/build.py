@@ -4,7 +4,7 @@
 
 The library is the product: the C ABI of include/wmd.h over the kernels in
 csrc/.  Every translation unit compiles to an object in parallel (the
-resident-solver shapes are spread over four units), then one link makes the
+resident-solver shapes are spread over six units), then one link makes the
 shared library.  Packed-FP32 instruction selection is disabled for all
 device code.
 """
@@ -20,8 +20,8 @@ from concurrent.futures import ThreadPoolExecutor
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libwmd.so")
-UNITS = ("wmd_kernels.hip", "wmd_resident_0.hip", "wmd_resident_1.hip", "wmd_resident_2.hip",
-         "wmd_resident_3.hip", "wmd_mlp.hip", "wmd_capi.hip")
+UNITS = ("wmd_kernels.hip", *(f"wmd_resident_{p}.hip" for p in range(6)), "wmd_mlp.hip",
+         "wmd_capi.hip")
 SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in UNITS]
 ARCH = os.environ.get("WMD_OFFLOAD_ARCH", "gfx950")
 

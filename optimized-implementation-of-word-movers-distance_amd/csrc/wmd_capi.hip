@@ -76,6 +76,9 @@ int round_up4(int x) { return (x + 3) & ~3; }
 // them so one launch's tail (its slowest pairs) overlaps the next launch.
 constexpr int kSideStreams = 4;
 constexpr int kTuneStride = wmd::kMaxQ + 1;
+// Shapes the autotuner times per (document class, query bucket): the default
+// and the next ones in capacity order, up to twice the default's capacity.
+constexpr int kTuneCandidates = 6;
 
 struct wmd_context {
   int device = 0;
@@ -1204,7 +1207,7 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
         ResidentLaunch l{0, cls, 0, (int64_t)g.size(), d0, d0 + n_docs, 0.0};
         float best_ms = INFINITY;
         int best = s0, tried = 0;
-        for (int s = s0; s < wmd::kNumShapes && tried < 4; ++s) {
+        for (int s = s0; s < wmd::kNumShapes && tried < kTuneCandidates; ++s) {
           if (!shape_fits(s, qmax, lg) || wmd::kShapes[s].cap_q() > 2 * cap0) continue;
           ++tried;
           l.shape = s;

@@ -116,7 +116,10 @@ struct TileShape {
   constexpr int cap_q() const { return B * NI; }
   constexpr int cap_n() const { return kNumDocClasses * kDocClassWidth; }
 };
-constexpr int kNumShapes = 24;
+// Sorted by capacity; equal capacities list the narrower query-word group
+// first (more document-word lanes: fewer slots per lane to reduce and divide
+// for the same tile), which is the default the autotuner starts from.
+constexpr int kNumShapes = 31;
 inline constexpr TileShape kShapes[kNumShapes] = {
     // W  B  NI log     cap_q
     {1, 2, 2, 0},    //   4
@@ -125,9 +128,16 @@ inline constexpr TileShape kShapes[kNumShapes] = {
     {1, 4, 3, 0},    //  12
     {1, 4, 4, 0},    //  16
     {1, 4, 5, 1},    //  20
+    {1, 4, 6, 0},    //  24
     {1, 8, 3, 0},    //  24
+    {1, 4, 7, 0},    //  28
+    {1, 4, 8, 0},    //  32
     {1, 8, 4, 0},    //  32
+    {1, 4, 9, 0},    //  36
+    {1, 4, 10, 0},   //  40
     {1, 8, 5, 1},    //  40
+    {1, 4, 11, 0},   //  44
+    {1, 4, 12, 0},   //  48
     {1, 8, 6, 0},    //  48
     {1, 8, 7, 0},    //  56
     {1, 8, 8, 1},    //  64
@@ -148,7 +158,7 @@ constexpr int kMaxQ = kShapes[kNumShapes - 1].B * kShapes[kNumShapes - 1].NI;
 
 #define WMD_FOR_EACH_SHAPE(X)                                                                 \
   X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)     \
-  X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23)
+  X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)
 
 // ---- learned warm start (wmd_mlp.hip) ----
 constexpr int kMlpHidden = 256;
