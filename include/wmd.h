@@ -248,6 +248,8 @@ int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const 
 /* Number of resident tile shapes and their limits (for tuners/tests). */
 int wmd_num_shapes(void);
 int wmd_shape_info(int shape, int32_t* waves, int32_t* cap_q, int32_t* cap_n);
+/* Domains a shape runs in: bit 0 linear, bit 1 log (negative: bad shape). */
+int wmd_shape_domains(int shape);
 
 /* ---- Tile-shape autotuner (no reference counterpart).  The resident variant
  * runs each (query, document length class) group with one register tile
@@ -269,6 +271,30 @@ int wmd_context_get_tuning(wmd_context* ctx, int32_t domain, int32_t* table);
  * default.  Entries must hold their q (and have a log-domain form for the
  * log domain). */
 int wmd_context_set_tuning(wmd_context* ctx, int32_t domain, const int32_t* table);
+
+/* ---- The reference's file formats (host only, no device), as read and
+ * written by its programs' main (sinkhorn_wmd_fused.cpp:415-624):
+ *   mat.mtx   "rows cols nnz", then nnz lines "word doc weight", 1-based ids,
+ *             weight > 0 (:427-480) -> the V x N word-major CSR, each row in
+ *             file order like the reference's counting sort (:494-506)
+ *   vecs.out  one line per word of D space-separated values (:510-545)
+ *   r.out     V values >= 0, one per line (:553-586)
+ *   scores    one line per document, 9 significant digits (:614-622)
+ * Invalid content returns WMD_ERR_ARG (the reference prints and exits). */
+int wmd_mtx_header(const char* path, uint64_t* V, uint64_t* N, uint64_t* nnz);
+int wmd_read_mtx(const char* path, uint64_t V, uint64_t N, uint64_t nnz, uint64_t* row_ptr,
+                 uint64_t* col_inds, double* vals);
+int wmd_vecs_dims(const char* path, uint64_t* V, uint64_t* D);
+int wmd_read_vecs(const char* path, uint64_t V, uint64_t D, double* vecs);
+int wmd_read_r(const char* path, uint64_t V, double* r);
+int wmd_write_scores(const char* path, const double* wmd, uint64_t N);
+/* Writers of the three input formats (no reference counterpart: they make
+ * inputs for the reference's programs and for wmd_b), values with 17
+ * significant digits so they read back exactly. */
+int wmd_write_mtx(const char* path, uint64_t V, uint64_t N, const uint64_t* row_ptr,
+                  const uint64_t* col_inds, const double* vals);
+int wmd_write_vecs(const char* path, uint64_t V, uint64_t D, const double* vecs);
+int wmd_write_r(const char* path, uint64_t V, const double* r);
 
 /* Human-readable description of the last error on this thread. */
 const char* wmd_last_error(void);

@@ -41,11 +41,14 @@ MLP_HIDDEN = 256
 EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
            "wmd_context_create", "wmd_context_destroy", "wmd_context_set_corpus",
            "wmd_context_set_queries", "wmd_context_run", "wmd_context_result",
-           "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_last_error",
+           "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_shape_domains",
+           "wmd_last_error",
            "wmd_mlp_param_count", "wmd_context_mlp_set", "wmd_context_mlp_get",
            "wmd_train_options_default", "wmd_context_mlp_train", "wmd_context_duals",
            "wmd_mlp_loss_grad", "wmd_context_set_warm", "wmd_context_autotune",
-           "wmd_tuning_dims", "wmd_context_get_tuning", "wmd_context_set_tuning")
+           "wmd_tuning_dims", "wmd_context_get_tuning", "wmd_context_set_tuning",
+           "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims", "wmd_read_vecs", "wmd_read_r",
+           "wmd_write_scores", "wmd_write_mtx", "wmd_write_vecs", "wmd_write_r")
 
 
 class Options(C.Structure):
@@ -108,6 +111,7 @@ def load():
     L.wmd_context_device_result.argtypes = [P]
     L.wmd_context_device_result.restype = C.c_void_p
     L.wmd_num_shapes.argtypes = []
+    L.wmd_shape_domains.argtypes = [C.c_int]
     L.wmd_shape_info.argtypes = [C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int32)]
     L.wmd_last_error.argtypes = []
@@ -127,13 +131,26 @@ def load():
     L.wmd_tuning_dims.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.wmd_context_get_tuning.argtypes = [P, C.c_int32, P]
     L.wmd_context_set_tuning.argtypes = [P, C.c_int32, P]
+    S = C.c_char_p
+    L.wmd_mtx_header.argtypes = [S, P, P, P]
+    L.wmd_read_mtx.argtypes = [S, U64, U64, U64, P, P, P]
+    L.wmd_vecs_dims.argtypes = [S, P, P]
+    L.wmd_read_vecs.argtypes = [S, U64, U64, P]
+    L.wmd_read_r.argtypes = [S, U64, P]
+    L.wmd_write_scores.argtypes = [S, P, U64]
+    L.wmd_write_mtx.argtypes = [S, U64, U64, P, P, P]
+    L.wmd_write_vecs.argtypes = [S, U64, U64, P]
+    L.wmd_write_r.argtypes = [S, U64, P]
     for name in ("wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
                  "wmd_context_create", "wmd_context_set_corpus", "wmd_context_set_queries",
                  "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info",
+                 "wmd_shape_domains",
                  "wmd_context_mlp_set", "wmd_context_mlp_get", "wmd_context_mlp_train",
                  "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm",
                  "wmd_context_autotune", "wmd_tuning_dims", "wmd_context_get_tuning",
-                 "wmd_context_set_tuning"):
+                 "wmd_context_set_tuning", "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims",
+                 "wmd_read_vecs", "wmd_read_r", "wmd_write_scores", "wmd_write_mtx",
+                 "wmd_write_vecs", "wmd_write_r"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

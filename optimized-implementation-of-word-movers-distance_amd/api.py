@@ -271,6 +271,18 @@ def tuning_dims():
     return n.value, q.value
 
 
+def shape_domains():
+    """[(linear, log)] availability of each resident tile shape per domain."""
+    L = nat.load()
+    out = []
+    for s in range(L.wmd_num_shapes()):
+        d = L.wmd_shape_domains(s)
+        if d < 0:
+            nat.check(d, "wmd_shape_domains")
+        out.append((bool(d & 1), bool(d & 2)))
+    return out
+
+
 def shapes():
     """[(waves, cap_q, cap_n)] of the resident tile shapes."""
     L = nat.load()

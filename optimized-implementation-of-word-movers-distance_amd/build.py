@@ -4,7 +4,7 @@
 
 The library is the product: the C ABI of include/wmd.h over the kernels in
 csrc/.  Every translation unit compiles to an object in parallel (the
-resident-solver shapes are spread over six units), then one link makes the
+resident-solver shapes are spread over eight units), then one link makes the
 shared library.  Packed-FP32 instruction selection is disabled for all
 device code.
 """
@@ -20,9 +20,12 @@ from concurrent.futures import ThreadPoolExecutor
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libwmd.so")
-UNITS = ("wmd_kernels.hip", *(f"wmd_resident_{p}.hip" for p in range(6)), "wmd_mlp.hip",
-         "wmd_capi.hip")
+UNITS = ("wmd_kernels.hip", *(f"wmd_resident_{p}.hip" for p in range(8)), "wmd_mlp.hip",
+         "wmd_capi.hip", "wmd_io.cpp")
 SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in UNITS]
+# The reference program's command line (reads its data files, writes scores).
+CLI_SOURCE = os.path.join(PKG_DIR, "csrc", "wmd_b.cpp")
+CLI_PATH = os.path.join(PKG_DIR, "wmd_b")
 ARCH = os.environ.get("WMD_OFFLOAD_ARCH", "gfx950")
 
 
@@ -41,6 +44,13 @@ def compile_command(src: str, obj: str) -> list[str]:
 
 def link_command(objs: list[str], out: str) -> list[str]:
     return [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out]
+
+
+def cli_command(out: str) -> list[str]:
+    """Host-only program over the C ABI: g++ against libwmd.so, found next to
+    the executable at run time."""
+    return [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-I", os.path.join(REPO, "include"),
+            CLI_SOURCE, "-L", PKG_DIR, "-lwmd", "-Wl,-rpath,$ORIGIN", "-o", out]
 
 
 def _run(cmd: list[str]) -> subprocess.CompletedProcess:
@@ -67,6 +77,11 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
             sys.stderr.write(res.stdout + res.stderr)
             raise RuntimeError("hipcc failed linking libwmd.so")
     os.replace(tmp, LIB_PATH)
+    res = _run(cli_command(CLI_PATH + ".tmp"))
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building wmd_b")
+    os.replace(CLI_PATH + ".tmp", CLI_PATH)
     return LIB_PATH
 
 

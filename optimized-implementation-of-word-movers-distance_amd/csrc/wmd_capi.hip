@@ -29,6 +29,23 @@ void set_err(const char* fmt, ...) {
   g_err = buf;
 }
 
+}  // namespace
+
+// Error text of the file-format readers (wmd_io.cpp) goes to the same
+// thread-local message.
+namespace wmd_io {
+void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+}  // namespace wmd_io
+
+namespace {
+
 #define HIP_TRY(expr)                                                             \
   do {                                                                            \
     hipError_t e_ = (expr);                                                       \
@@ -187,19 +204,24 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   return a;
 }
 
-// Smallest-capacity tile shape that holds q query words and documents of
-// nmax words (log domain: among the shapes with a log-domain instantiation).
+// Whether shape s has a kernel for the domain.
+bool shape_in_domain(int s, bool log_domain) {
+  return log_domain ? wmd::kShapes[s].log != 0 : wmd::kShapes[s].lin != 0;
+}
+
+// Smallest-capacity tile shape of the domain that holds q query words and
+// documents of nmax words.
 int pick_shape(int q, int nmax, bool log_domain) {
   for (int s = 0; s < wmd::kNumShapes; ++s)
     if (wmd::kShapes[s].cap_q() >= q && wmd::kShapes[s].cap_n() >= nmax &&
-        (!log_domain || wmd::kShapes[s].log))
+        shape_in_domain(s, log_domain))
       return s;
   return -1;
 }
 
 bool shape_fits(int s, int q, bool log_domain) {
   return s >= 0 && s < wmd::kNumShapes && wmd::kShapes[s].cap_q() >= q &&
-         (!log_domain || wmd::kShapes[s].log);
+         shape_in_domain(s, log_domain);
 }
 
 // Tile shape for a query of q words against documents of class cls: the
@@ -651,6 +673,14 @@ int wmd_shape_info(int shape, int32_t* waves, int32_t* cap_q, int32_t* cap_n) {
   if (cap_q) *cap_q = wmd::kShapes[shape].cap_q();
   if (cap_n) *cap_n = wmd::kShapes[shape].cap_n();
   return WMD_OK;
+}
+
+int wmd_shape_domains(int shape) {
+  if (shape < 0 || shape >= wmd::kNumShapes) {
+    set_err("shape %d out of range", shape);
+    return WMD_ERR_ARG;
+  }
+  return (wmd::kShapes[shape].lin ? 1 : 0) | (wmd::kShapes[shape].log ? 2 : 0);
 }
 
 int wmd_context_create(int device, wmd_context** out) {
@@ -1194,7 +1224,7 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
     }
     int prev_cap = 0;
     for (int s0 = 0; s0 < wmd::kNumShapes; ++s0) {
-      if (lg && !wmd::kShapes[s0].log) continue;
+      if (!shape_in_domain(s0, lg)) continue;
       const int cap0 = wmd::kShapes[s0].cap_q();
       const std::vector<int32_t>& g = groups[s0];
       if (!g.empty()) {

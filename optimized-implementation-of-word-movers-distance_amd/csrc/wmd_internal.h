@@ -106,59 +106,72 @@ struct SolveArgs {
 // A = 64W/B document-word lanes.  Each launch also fixes a document-length
 // class c (documents of at most kDocClassWidth*(c+1) words), which sets the
 // lane's document-word slot count NE = kDocClassWidth*(c+1)/A.  Capacity:
-// q <= B*NI, n <= kNumDocClasses*kDocClassWidth.  Query capacities step by
-// one slot per lane (B words), so a query pads to at most B-1 extra words.
-// `log` marks the shapes that also have a log-domain instantiation.
+// q <= B*NI, n <= kNumDocClasses*kDocClassWidth.
+// `lin` / `log` mark the shapes with a linear-domain (k_resident) and a
+// log-domain (k_resident_log) instantiation.  Linear shapes keep B <= 16, so
+// a query-word group stays inside a 16-lane row and every cross-row step is
+// a reduce-scatter; multi-wave linear shapes put more document-word lanes on
+// long queries instead of wider query-word groups.
 constexpr int kNumDocClasses = 5;
 constexpr int kDocClassWidth = 32;
 struct TileShape {
-  int W, B, NI, log;
+  int W, B, NI, lin, log;
   constexpr int cap_q() const { return B * NI; }
   constexpr int cap_n() const { return kNumDocClasses * kDocClassWidth; }
 };
 // Sorted by capacity; equal capacities list the narrower query-word group
 // first (more document-word lanes: fewer slots per lane to reduce and divide
 // for the same tile), which is the default the autotuner starts from.
-constexpr int kNumShapes = 31;
+constexpr int kNumShapes = 40;
 inline constexpr TileShape kShapes[kNumShapes] = {
-    // W  B  NI log     cap_q
-    {1, 2, 2, 0},    //   4
-    {1, 2, 3, 0},    //   6
-    {1, 2, 4, 1},    //   8
-    {1, 4, 3, 0},    //  12
-    {1, 4, 4, 0},    //  16
-    {1, 4, 5, 1},    //  20
-    {1, 4, 6, 0},    //  24
-    {1, 8, 3, 0},    //  24
-    {1, 4, 7, 0},    //  28
-    {1, 4, 8, 0},    //  32
-    {1, 8, 4, 0},    //  32
-    {1, 4, 9, 0},    //  36
-    {1, 4, 10, 0},   //  40
-    {1, 8, 5, 1},    //  40
-    {1, 4, 11, 0},   //  44
-    {1, 4, 12, 0},   //  48
-    {1, 8, 6, 0},    //  48
-    {1, 8, 7, 0},    //  56
-    {1, 8, 8, 1},    //  64
-    {2, 16, 5, 0},   //  80
-    {2, 16, 6, 0},   //  96
-    {2, 16, 7, 0},   // 112
-    {2, 16, 8, 1},   // 128
-    {4, 32, 5, 0},   // 160
-    {4, 32, 6, 0},   // 192
-    {4, 32, 7, 0},   // 224
-    {4, 32, 8, 1},   // 256
-    {8, 64, 5, 0},   // 320
-    {8, 64, 6, 0},   // 384
-    {8, 64, 7, 0},   // 448
-    {8, 64, 8, 1},   // 512
+    // W  B  NI lin log   cap_q
+    {1, 2, 2, 1, 0},    //    4
+    {1, 2, 3, 1, 0},    //    6
+    {1, 2, 4, 1, 1},    //    8
+    {1, 4, 3, 1, 0},    //   12
+    {1, 4, 4, 1, 0},    //   16
+    {1, 4, 5, 1, 1},    //   20
+    {1, 4, 6, 1, 0},    //   24
+    {1, 8, 3, 1, 0},    //   24
+    {1, 4, 7, 1, 0},    //   28
+    {1, 4, 8, 1, 0},    //   32
+    {1, 8, 4, 1, 0},    //   32
+    {1, 4, 9, 1, 0},    //   36
+    {1, 4, 10, 1, 0},   //   40
+    {1, 8, 5, 1, 1},    //   40
+    {1, 4, 11, 1, 0},   //   44
+    {1, 4, 12, 1, 0},   //   48
+    {1, 8, 6, 1, 0},    //   48
+    {1, 8, 7, 1, 0},    //   56
+    {1, 8, 8, 1, 1},    //   64
+    {2, 16, 5, 1, 0},   //   80
+    {2, 16, 6, 1, 0},   //   96
+    {2, 16, 7, 1, 0},   //  112
+    {2, 16, 8, 1, 1},   //  128
+    {4, 16, 9, 1, 0},   //  144
+    {4, 16, 10, 1, 0},  //  160
+    {4, 16, 11, 1, 0},  //  176
+    {4, 16, 12, 1, 0},  //  192
+    {4, 16, 13, 1, 0},  //  208
+    {4, 16, 14, 1, 0},  //  224
+    {4, 16, 15, 1, 0},  //  240
+    {4, 16, 16, 1, 0},  //  256
+    {4, 32, 8, 0, 1},   //  256
+    {8, 16, 17, 1, 0},  //  272
+    {8, 16, 18, 1, 0},  //  288
+    {8, 16, 19, 1, 0},  //  304
+    {8, 16, 20, 1, 0},  //  320
+    {8, 16, 24, 1, 0},  //  384
+    {8, 16, 28, 1, 0},  //  448
+    {8, 16, 32, 1, 0},  //  512
+    {8, 64, 8, 0, 1},   //  512
 };
 constexpr int kMaxQ = kShapes[kNumShapes - 1].B * kShapes[kNumShapes - 1].NI;
 
 #define WMD_FOR_EACH_SHAPE(X)                                                                 \
   X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)     \
-  X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30)
+  X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) \
+  X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39)
 
 // ---- learned warm start (wmd_mlp.hip) ----
 constexpr int kMlpHidden = 256;

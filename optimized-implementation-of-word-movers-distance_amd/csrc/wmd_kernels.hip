@@ -523,7 +523,7 @@ hipError_t launch_resident(int shape, int doc_class, bool log_domain, const Solv
   if (a.n_pairs <= 0) return hipSuccess;
   if (shape < 0 || shape >= kNumShapes || doc_class < 0 || doc_class >= kNumDocClasses)
     return hipErrorInvalidValue;
-  if (log_domain && !kShapes[shape].log) return hipErrorInvalidValue;
+  if (log_domain ? !kShapes[shape].log : !kShapes[shape].lin) return hipErrorInvalidValue;
   return kResidentLaunch[shape](doc_class, log_domain, a, s);
 }
 

@@ -14,18 +14,16 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // Sum over every query slot of a pair of per-lane partials (step-major over
-// N values).  W == 1: after the x reduce-scatter the lanes of a row own
-// disjoint slots, so the sum runs over the B lanes and the four rows; W > 1:
-// every lane holds all slots of its query-word lane b, so over the B lanes.
-template <int W, int B, int N>
+// N values).  After the x reduce-scatter the lanes of a 16-lane row own
+// disjoint slots (the same slots in every wave of the pair), so the sum runs
+// over the B query-word lanes of a row and then over the four rows.
+template <int B, int N>
 __device__ __forceinline__ void slot_sum_n(float (&v)[N]) {
   group_sum_n<B>(v);
-  if constexpr (W == 1) {
 #pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
+  for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
 #pragma unroll
-    for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
-  }
+  for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
 }
 
 // State of the u-update rule for the N query slots a lane updates.
@@ -64,7 +62,7 @@ __device__ __forceinline__ void update_state_reset(UpdateState<N>& st, float ome
 //              previous point's plain update, and the history restarts.
 //              Centring pins the gauge (the plan ignores a constant added to
 //              every log2 u_i), so the extrapolation cannot drift along it.
-template <int W, int B, int N>
+template <int B, int N>
 __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState<N>& st,
                                                  const float (&x)[N], const float (&r)[N],
                                                  const float (&lr)[N], float (&u)[N], bool seeded,
@@ -79,7 +77,7 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
       g[n] = (r[n] > 0.f) ? lr[n] - __builtin_amdgcn_logf(x[n]) : 0.f;
       red[1] += g[n];
     }
-    slot_sum_n<W, B>(red);
+    slot_sum_n<B>(red);
     const float res = wave_uniform(red[0]);
     if (!seeded && use_tol && res < a.tol) {
       stop = true;
@@ -105,7 +103,7 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
       }
       float gam = 0.f;
       if (st.prev) {
-        slot_sum_n<W, B>(d);
+        slot_sum_n<B>(d);
         gam = wave_uniform(d[1]) / fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f);
       }
 #pragma unroll
@@ -129,7 +127,7 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
     part += fabsf(ux[n] - r[n]);  // padded slots: 0 * x - 0
   }
   float red[1] = {part};
-  slot_sum_n<W, B>(red);
+  slot_sum_n<B>(red);
   const float res = wave_uniform(red[0]);
   if (st.sor && !seeded) {
     if (res > st.res_prev) st.sor = false;
@@ -149,14 +147,20 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
 
 // ---------------------------------------------------------------------------
 // Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
-// query-word lanes.  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
+// query-word lanes (B <= 16, so a query-word group never leaves a 16-lane
+// row).  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
 // i = b + B*m (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
-//   s_e = sum_i K[e,i] u_i      (partial over own i, butterfly over the B lanes)
+//   s_e = sum_i K[e,i] u_i      (partial over own i, DPP butterfly over the
+//                                B lanes of the row)
 //   o_e = c_e / s_e
-//   x_i = sum_e K[e,i] o_e      (partial over own e, butterfly over the A lanes,
-//                                LDS across waves when W > 1)
+//   x_i = sum_e K[e,i] o_e      (partial over own e; reduce-scatter across the
+//                                four rows, DPP steps inside a row, and for
+//                                W > 1 the other waves' sums of the same
+//                                slots through LDS)
 //   res = sum_i |u_i x_i - r_i| (row-marginal residual of the current plan)
-//   u_i = r_i / x_i             (= 1 / x of the reference's K_over_r form)
+//   u_i = r_i / x_i             (= 1 / x of the reference's K_over_r form),
+//                                for the slots the lane owns; the rest are
+//                                gathered back across the rows
 // then the final half-step and sum_{i,e} u_i K M o_e in FP64 (:339-393), with
 // M recovered from the rescaled kernel as mrow + log(Kt) / lamda.
 //
@@ -165,16 +169,14 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
 // the document's or the query's end hold K = 0 and contribute nothing.
 template <int W, int B, int NI, int NE>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
+  static_assert(B <= 16, "query-word groups stay inside a 16-lane row");
   constexpr int AW = 64 / B;
   constexpr int A = W * AW;
-  // In-wave x reduction over the AW lanes of a query word: its xor masks 32
-  // and 16 cross 16-lane rows and run as a reduce-scatter (XL swap levels),
-  // the masks below 16 as DPP all-reduce steps.  NIP pads NI to a multiple of
-  // 2^XL; after the scatter a lane owns NO of the NIP sums.
-  constexpr int XL = (B <= 16) ? 2 : (B == 32 ? 1 : 0);
-  constexpr int NIP = (NI + (1 << XL) - 1) >> XL << XL;
-  constexpr int NO = NIP >> XL;
-  __shared__ float xsh[(W > 1) ? 2 * W * NIP * B : 1];
+  // After the scatter a lane owns NO = NIP / 4 of the x sums (NIP pads NI to
+  // a multiple of 4): slot k <-> query-word slot own_slot<2>(k, rho).
+  constexpr int NIP = (NI + 3) / 4 * 4;
+  constexpr int NO = NIP / 4;
+  __shared__ float xsh[(W > 1) ? 2 * W * NO * 64 : 1];
   __shared__ double dsh[(W > 1) ? W : 1];
 
   const int lane = threadIdx.x & 63;
@@ -216,20 +218,16 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
     }
   }
-  float u[NI], r[NI];
+  float u[NI];
 #pragma unroll
-  for (int m = 0; m < NI; ++m) {
-    const int i = b + B * m;
-    r[m] = (i < q) ? a.q_r[qb + i] : 0.f;
-    u[m] = (i < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
-  }
-  // One wave per pair: the lane updates only the NO scalings it owns after
-  // the x reduce-scatter (slot k <-> query-word slot own_slot<XL>(k, rho)) and
-  // gathers the rest; uo, ro, lro are its owned u, r, log2 r.
+  for (int m = 0; m < NI; ++m) u[m] = (b + B * m < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
+  // The lane updates only the NO scalings it owns and gathers the rest; uo,
+  // ro, lro are its owned u, r, log2 r (padded slots: r = 0, log2 r = -inf,
+  // which the update rules map to u = 0).
   float uo[NO], ro[NO], lro[NO];
 #pragma unroll
   for (int k = 0; k < NO; ++k) {
-    const int m = own_slot<XL>(k, rho);
+    const int m = own_slot<2>(k, rho);
     const int i = b + B * m;
     const bool live = m < NI && i < q;
     ro[k] = live ? a.q_r[qb + i] : 0.f;
@@ -246,11 +244,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   bool seeded = warm_on;
   bool fell_back = !warm_on;
   float res_win = INFINITY;
-  // log2 r (padded slots: -inf, which the update rules map to u = 0).
-  float lr[NI];
-#pragma unroll
-  for (int m = 0; m < NI; ++m) lr[m] = __builtin_amdgcn_logf(r[m]);
-  UpdateState<(W == 1) ? NO : NI> st;
+  UpdateState<NO> st;
   update_state_reset(st, a.omega);
   int it = 0;
   float res = 0.f;
@@ -291,54 +285,37 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       }
       xs[m] = x;
     }
-    bool stop = false;
-    if constexpr (W == 1) {
-      // x_i for the owned slots: scatter across rows, then the in-row masks.
-      float xo[NO];
-      rows_reduce_scatter<NIP>(xs, xo);
-      bfly_sum_n<B, 8>(xo);
-      // The residual is the same on every lane, so the stop test is a
-      // scalar branch.
-      const float rv = update_scalings<W, B>(a, st, xo, ro, lro, uo, seeded, q, stop);
-      if (!seeded) res = rv;
-      float ug[NIP];
-      rows_gather<NIP>(uo, ug);
+    // x_i for the owned slots: scatter across rows, then the in-row masks.
+    float xo[NO];
+    rows_reduce_scatter<NIP>(xs, xo);
+    bfly_sum_n<B, 8>(xo);
+    if constexpr (W > 1) {
+      // The waves of a pair own the same slots (ownership follows the row):
+      // add the other waves' sums through LDS, in wave order, so every wave
+      // holds bitwise-equal totals.  Double buffering: the next write to
+      // this half happens two iterations later, after every wave has passed
+      // the next barrier.
+      float* buf = xsh + (it & 1) * (W * NO * 64);
 #pragma unroll
-      for (int m = 0; m < NI; ++m) u[m] = ug[m];
-    } else {
-      // Scatter inside the wave, then the W partial sums of every slot
-      // through LDS (each lane writes the slots it owns, reads all).
-      float* buf = xsh + (it & 1) * (W * NIP * B);
-      if constexpr (XL == 2) {
-        float xo[NIP / 4];
-        rows_reduce_scatter<NIP>(xs, xo);
-#pragma unroll
-        for (int k = 0; k < NIP / 4; ++k) buf[(wv * NIP + own_slot<2>(k, rho)) * B + b] = xo[k];
-      } else if constexpr (XL == 1) {
-        float xo[NIP / 2];
-        halves_reduce_scatter<NIP>(xs, xo);
-#pragma unroll
-        for (int k = 0; k < NIP / 2; ++k) buf[(wv * NIP + own_slot<1>(k, rho)) * B + b] = xo[k];
-      } else {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) buf[(wv * NIP + m) * B + b] = xs[m];
-      }
+      for (int k = 0; k < NO; ++k) buf[(wv * NO + k) * 64 + lane] = xo[k];
       __syncthreads();
 #pragma unroll
-      for (int m = 0; m < NI; ++m) {
+      for (int k = 0; k < NO; ++k) {
         float x = 0.f;
 #pragma unroll
-        for (int w = 0; w < W; ++w) x += buf[(w * NIP + m) * B + b];
-        xs[m] = x;
+        for (int w = 0; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
+        xo[k] = x;
       }
-      // Double buffering: the next write to this half happens two
-      // iterations later, after every wave has passed the next barrier.
-      float xv[NI];
-#pragma unroll
-      for (int m = 0; m < NI; ++m) xv[m] = xs[m];
-      const float rv = update_scalings<W, B>(a, st, xv, r, lr, u, seeded, q, stop);
-      if (!seeded) res = rv;
     }
+    // The residual is the same on every lane, so the stop test is a scalar
+    // branch.
+    bool stop = false;
+    const float rv = update_scalings<B>(a, st, xo, ro, lro, uo, seeded, q, stop);
+    if (!seeded) res = rv;
+    float ug[NIP];
+    rows_gather<NIP>(uo, ug);
+#pragma unroll
+    for (int m = 0; m < NI; ++m) u[m] = ug[m];
     ++it;
     if (seeded) {
       seeded = false;
@@ -348,7 +325,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     if (!fell_back && (it & 63) == 0) {
       if (res >= res_win) {
 #pragma unroll
-        for (int m = 0; m < NI; ++m) u[m] = (r[m] > 0.f) ? (float)q : 0.f;
+        for (int m = 0; m < NI; ++m) u[m] = (b + B * m < q) ? (float)q : 0.f;
 #pragma unroll
         for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
         update_state_reset(st, a.omega);
@@ -591,36 +568,40 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
   }
 }
 
-template <int W, int B, int NI, int NE, bool LOG>
+// Launches the linear or the log-domain kernel of a shape; LIN / LOG say
+// which of the two the shape instantiates.
+template <int W, int B, int NI, int NE, bool LIN, bool LOG>
 hipError_t launch_res(bool log_domain, const SolveArgs& a, hipStream_t s) {
   const int64_t blocks = (W == 1) ? (a.n_pairs + 3) / 4 : a.n_pairs;
-  if constexpr (LOG) {
-    if (log_domain) {
+  if (log_domain) {
+    if constexpr (LOG) {
       hipLaunchKernelGGL((k_resident_log<W, B, NI, NE>), dim3((unsigned)blocks),
                          dim3(W == 1 ? 256 : 64 * W), 0, s, a);
       return hipGetLastError();
     }
-  } else {
-    if (log_domain) return hipErrorInvalidValue;
+    return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
-                     dim3(W == 1 ? 256 : 64 * W), 0, s, a);
-  return hipGetLastError();
+  if constexpr (LIN) {
+    hipLaunchKernelGGL((k_resident<W, B, NI, NE>), dim3((unsigned)blocks),
+                       dim3(W == 1 ? 256 : 64 * W), 0, s, a);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
 }
 
 // Document-length class c holds documents of at most kDocClassWidth*(c+1)
 // words: NE = that length over the shape's A document-word lanes.
-template <int W, int B, int NI, bool LOG>
+template <int W, int B, int NI, bool LIN, bool LOG>
 hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
   constexpr int A = 64 * W / B;
   constexpr int C = kDocClassWidth;
   static_assert(kNumDocClasses == 5, "one case per document-length class");
   switch (cls) {
-    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A, LOG>(lg, a, s);
-    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A, LOG>(lg, a, s);
-    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A, LOG>(lg, a, s);
-    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A, LOG>(lg, a, s);
-    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A, LOG>(lg, a, s);
+    case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -630,7 +611,7 @@ hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
 // Defines launch_resident_s<ID> for entry ID of kShapes.
 #define WMD_RESIDENT_SHAPE(ID)                                                             \
   hipError_t launch_resident_s##ID(int cls, bool lg, const SolveArgs& a, hipStream_t s) { \
-    return launch_shape<kShapes[ID].W, kShapes[ID].B, kShapes[ID].NI,                      \
+    return launch_shape<kShapes[ID].W, kShapes[ID].B, kShapes[ID].NI, kShapes[ID].lin != 0, \
                         kShapes[ID].log != 0>(cls, lg, a, s);                               \
   }
 

@@ -148,8 +148,8 @@ def test_forced_shapes_agree(P, dg):
     with engine(P, corpus) as e:
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
         ref = None
-        for s, (_, cap_q, _) in enumerate(P.shapes()):
-            if cap_q < 7:
+        for s, ((_, cap_q, _), (linear, _)) in enumerate(zip(P.shapes(), P.api.shape_domains())):
+            if cap_q < 7 or not linear:
                 continue
             e.run(-10.0, max_iter=15, shape=s)
             w = e.result()[0]
@@ -304,11 +304,14 @@ def test_invalid_arguments(P, dg):
         big = np.arange(0, 2 * (cap0 + 1), 2)  # one word beyond shape 0's capacity
         e.set_queries([0, cap0 + 1], big, np.full(cap0 + 1, 1.0 / (cap0 + 1)))
         assert status(lambda: e.run(-1.0, shape=0, max_iter=3)) == nat.WMD_ERR_UNSUPPORTED
-        # a shape without a log-domain form is refused in the log domain
-        no_log = 1
+        # a shape without a form for the domain is refused
+        doms = P.api.shape_domains()
+        no_log = next(i for i, (lin, lg) in enumerate(doms) if lin and not lg)
+        no_lin = next(i for i, (lin, lg) in enumerate(doms) if lg and not lin)
         e.set_queries([0, 1], [2], [1.0])
         assert status(lambda: e.run(-1.0, shape=no_log, max_iter=3, log_domain=True)) == \
             nat.WMD_ERR_UNSUPPORTED
+        assert status(lambda: e.run(-1.0, shape=no_lin, max_iter=3)) == nat.WMD_ERR_UNSUPPORTED
 
 
 def test_batched_full_size_properties(P, dg, oracle):
