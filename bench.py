@@ -124,7 +124,7 @@ def cpu_port_per_pair(corpus, qs, lamda, tol, max_iter, threads, n_docs):
                       f"{dt:.2f}s"}
 
 
-def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=4):
+def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=6):
     """Train the warm-start MLP on converged duals of n_queries x n_docs
     seeded synthetic pairs over the same embeddings (not part of a step)."""
     tc, tq = dg.training_set(vecs, n_queries=n_queries, n_docs=n_docs, seed=101)
@@ -132,7 +132,7 @@ def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=4):
     with P.WMDEngine(0) as e:
         e.set_corpus(tc.vecs, tc.word_major())
         e.set_queries(tq.q_ptr, tq.words, tq.weights)
-        e.run(lamda, max_iter=20000, tol=1e-6, export_duals=True)
+        e.run(lamda, max_iter=20000, tol=1e-6, export_duals=True, accel=True)
         e.mlp_set(P.mlp_init(tc.D, seed=0))
         losses = e.mlp_train(epochs=epochs, pairs_per_batch=32, lr=1e-3, seed=0)
         params = e.mlp_get()
@@ -248,8 +248,10 @@ def main() -> int:
     ap.add_argument("--warm", choices=("uniform", "learned"), default="uniform")
     ap.add_argument("--accel", choices=("none", "anderson"), default="anderson",
                     help="u-update acceleration (anderson overrides --omega)")
-    ap.add_argument("--train-queries", type=int, default=100)
-    ap.add_argument("--train-docs", type=int, default=200)
+    # 20k training pairs over many distinct queries (the network has to
+    # generalise across queries more than across documents)
+    ap.add_argument("--train-queries", type=int, default=1000)
+    ap.add_argument("--train-docs", type=int, default=20)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="smallest-fitting tile shapes instead of autotuned ones")
