@@ -121,7 +121,11 @@ typedef struct wmd_options {
                           [1, 2).  The fixed point and the residual test are
                           unchanged (v is always the exact update); a pair whose
                           residual rises falls back to omega = 1. */
-  int32_t reserved[4];
+  int32_t graph;       /* resident variant: 1 (default) launches the solver
+                          stage from a HIP graph captured on first use and
+                          replayed while queries, tuning and options stay the
+                          same; 0 launches the kernels directly */
+  int32_t reserved[3];
 } wmd_options;
 
 typedef struct wmd_stats {

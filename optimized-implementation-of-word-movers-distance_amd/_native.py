@@ -56,7 +56,7 @@ class Options(C.Structure):
                 ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
                 ("domain", C.c_int32), ("cdist", C.c_int32), ("warm_start", C.c_int32),
                 ("export_duals", C.c_int32), ("accel", C.c_int32), ("omega", C.c_double),
-                ("reserved", C.c_int32 * 4)]
+                ("graph", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Stats(C.Structure):

@@ -123,7 +123,7 @@ class WMDEngine:
 
     def run(self, lamda, max_iter=15, tol=0.0, variant=nat.VARIANT_RESIDENT, check_every=8,
             shape=-1, summarize=True, log_domain=False, mfma_cdist=True, warm_start=False,
-            export_duals=False, omega=1.0, accel=False) -> dict:
+            export_duals=False, omega=1.0, accel=False, graph=True) -> dict:
         """Solve every (query, doc) pair; returns the run's wmd_stats as a dict
         (with summarize=False only stage times, no device-to-host copy)."""
         o = nat.default_options(max_iter=int(max_iter), tol=float(tol), variant=int(variant),
@@ -134,7 +134,8 @@ class WMDEngine:
                                 warm_start=(nat.WARM_GIVEN if warm_start == "given" else
                                             nat.WARM_LEARNED if warm_start else nat.WARM_UNIFORM),
                                 export_duals=int(bool(export_duals)), omega=float(omega),
-                                accel=nat.ACCEL_ANDERSON if accel else nat.ACCEL_NONE)
+                                accel=nat.ACCEL_ANDERSON if accel else nat.ACCEL_NONE,
+                                graph=int(bool(graph)))
         st = nat.Stats()
         nat.check(self._L.wmd_context_run(self._h, float(lamda), C.byref(o), C.byref(st)),
                   "wmd_context_run")

@@ -87,6 +87,15 @@ struct DevBuf {
 
 int round_up4(int x) { return (x + 3) & ~3; }
 
+// One resident launch: queries bucket[off, off + cnt) against the documents
+// doc_perm[d0, d1) of one length class, with one tile shape.
+struct ResidentLaunch {
+  int shape, cls;
+  size_t off;
+  int64_t cnt, d0, d1;
+  double work;  // padded tile entries, the list-scheduling weight
+};
+
 }  // namespace
 
 // Side streams of a context: the resident launches of a run fan out over
@@ -109,6 +118,17 @@ struct wmd_context {
   // 1 log): filled by wmd_context_autotune or wmd_context_set_tuning; -1 or
   // an empty table means the smallest shape that fits.
   std::vector<int32_t> tune[2];
+  // Resident launch plan (its query buckets live in plan_bucket) and the
+  // solver stage captured from it.  plan_epoch moves whenever corpus,
+  // queries or tuning change; a plan or graph built at an older epoch, for
+  // other shape/domain choices or (graph) other kernel arguments is rebuilt.
+  uint64_t plan_epoch = 1, plan_built = 0, plan_id = 0;
+  int plan_forced = -2;
+  bool plan_log = false;
+  std::vector<ResidentLaunch> plan;
+  hipGraphExec_t solve_exec = nullptr;
+  uint64_t solve_plan_id = 0;
+  wmd::SolveArgs solve_args{};
 
   // corpus
   bool has_corpus = false;
@@ -132,7 +152,7 @@ struct wmd_context {
   std::vector<int32_t> h_q_ptr, h_k_stride;
   std::vector<int64_t> h_k_off, h_u_off;
   int64_t k_total = 0, u_total = 0;
-  DevBuf<int32_t> q_ptr, q_words, k_stride, bucket;
+  DevBuf<int32_t> q_ptr, q_words, k_stride, bucket, plan_bucket;
   DevBuf<float> q_r;
   DevBuf<int64_t> k_off, u_off;
   DevBuf<float> K, mrow;
@@ -236,15 +256,6 @@ int shape_for(const wmd_context* c, int cls, int q, bool log_domain, int forced)
   return pick_shape(q, 0, log_domain);
 }
 
-// One resident launch: queries bucket[off, off + cnt) against the documents
-// doc_perm[d0, d1) of one length class, with one tile shape.
-struct ResidentLaunch {
-  int shape, cls;
-  size_t off;
-  int64_t cnt, d0, d1;
-  double work;  // padded tile entries, the list-scheduling weight
-};
-
 void add_launch(std::vector<ResidentLaunch>* plan, std::vector<int32_t>* flat,
                 const std::vector<int32_t>& queries, int s, int cls, int64_t d0, int64_t d1) {
   if (queries.empty() || d1 == d0) return;
@@ -271,14 +282,15 @@ int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
     }
     for (int s = 0; s < wmd::kNumShapes; ++s) add_launch(plan, &flat, groups[s], s, cls, d0, d1);
   }
-  HIP_TRY(c->bucket.upload(flat.data(), flat.size(), c->stream));
+  HIP_TRY(c->plan_bucket.upload(flat.data(), flat.size(), c->stream));
   return WMD_OK;
 }
 
+// Launch l with the query ids at bucket + l.off.
 hipError_t launch_one(wmd_context* c, const wmd::SolveArgs& base, const ResidentLaunch& l,
-                      bool log_domain, hipStream_t s) {
+                      const int32_t* bucket, bool log_domain, hipStream_t s) {
   wmd::SolveArgs a = base;
-  a.bucket_q = c->bucket.p + l.off;
+  a.bucket_q = bucket + l.off;
   a.doc_list = c->doc_perm.p + l.d0;
   a.n_list = (int)(l.d1 - l.d0);
   a.n_pairs = l.cnt * (l.d1 - l.d0);
@@ -301,7 +313,7 @@ int launch_plan(wmd_context* c, const wmd::SolveArgs& base,
   for (size_t i : order) {
     const int k = (int)(std::min_element(load, load + kSideStreams) - load);
     load[k] += plan[i].work;
-    HIP_TRY(launch_one(c, base, plan[i], log_domain, c->side[k]));
+    HIP_TRY(launch_one(c, base, plan[i], c->plan_bucket.p, log_domain, c->side[k]));
     ++*launches;
   }
   for (int k = 0; k < kSideStreams; ++k) {
@@ -311,12 +323,54 @@ int launch_plan(wmd_context* c, const wmd::SolveArgs& base,
   return WMD_OK;
 }
 
+void drop_solve_graph(wmd_context* c) {
+  if (c->solve_exec) (void)hipGraphExecDestroy(c->solve_exec);
+  c->solve_exec = nullptr;
+}
+
+// The resident solver stage.  The plan (and its bucket upload) is rebuilt
+// only when queries, corpus, tuning or the shape/domain choice changed.  With
+// use_graph the stage's launches, fork and join included, are captured once
+// into a HIP graph and replayed while the plan and the kernel arguments stay
+// the same (the arguments are compared bytewise; base_args zero-fills them).
 int run_resident(wmd_context* c, const wmd::SolveArgs& base, int forced_shape, bool log_domain,
-                 int* launches) {
-  std::vector<ResidentLaunch> plan;
-  int rc = plan_resident(c, forced_shape, log_domain, &plan);
-  if (rc != WMD_OK) return rc;
-  return launch_plan(c, base, plan, log_domain, launches);
+                 bool use_graph, int* launches) {
+  if (c->plan_built != c->plan_epoch || c->plan_forced != forced_shape ||
+      c->plan_log != log_domain) {
+    int rc = plan_resident(c, forced_shape, log_domain, &c->plan);
+    if (rc != WMD_OK) return rc;
+    c->plan_built = c->plan_epoch;
+    c->plan_forced = forced_shape;
+    c->plan_log = log_domain;
+    ++c->plan_id;
+  }
+  if (!use_graph) return launch_plan(c, base, c->plan, log_domain, launches);
+  if (!c->solve_exec || c->solve_plan_id != c->plan_id ||
+      std::memcmp(&c->solve_args, &base, sizeof(base)) != 0) {
+    drop_solve_graph(c);
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int captured = 0;
+    const int rc = launch_plan(c, base, c->plan, log_domain, &captured);
+    hipGraph_t graph = nullptr;
+    const hipError_t end = hipStreamEndCapture(c->stream, &graph);
+    if (rc != WMD_OK || end != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (rc == WMD_OK) set_err("solver-stage capture failed: %s", hipGetErrorString(end));
+      return rc != WMD_OK ? rc : WMD_ERR_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&c->solve_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      c->solve_exec = nullptr;
+      set_err("solver-stage graph instantiation failed: %s", hipGetErrorString(ie));
+      return WMD_ERR_HIP;
+    }
+    c->solve_args = base;
+    c->solve_plan_id = c->plan_id;
+  }
+  HIP_TRY(hipGraphLaunch(c->solve_exec, c->stream));
+  *launches += (int)c->plan.size();
+  return WMD_OK;
 }
 
 // Streaming variants: iterations launched from a captured HIP graph.
@@ -658,6 +712,7 @@ void wmd_options_default(wmd_options* opt) {
   opt->export_duals = 0;
   opt->accel = WMD_ACCEL_NONE;
   opt->omega = 1.0;
+  opt->graph = 1;
 }
 
 const char* wmd_last_error(void) { return g_err.c_str(); }
@@ -732,6 +787,8 @@ void wmd_context_destroy(wmd_context* c) {
   c->At.release(); c->Bd.release(); c->Cq.release(); c->pred.release(); c->duals.release();
   c->fallbacks.release(); c->warm_vals.release();
   c->status.release();
+  c->plan_bucket.release();
+  drop_solve_graph(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (int i = 0; i < kSideStreams; ++i) {
@@ -836,6 +893,7 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   c->nmax = nmax;
   c->split_ready = false;
   c->has_corpus = true;
+  ++c->plan_epoch;
   c->has_queries = false;  // query blocks depend on Vt and N
   c->ran = false;
   return WMD_OK;
@@ -916,6 +974,7 @@ int wmd_context_set_queries(wmd_context* c, uint64_t n, const int64_t* qp, const
   c->k_total = ktot;
   c->u_total = utot;
   c->has_queries = true;
+  ++c->plan_epoch;
   c->duals_valid = false;
   c->warm_given = false;
   c->ran = false;
@@ -1115,7 +1174,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   }
   HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   if (variant == WMD_VARIANT_RESIDENT)
-    rc = run_resident(c, a, o.shape, log_domain, &launches);
+    rc = run_resident(c, a, o.shape, log_domain, o.graph != 0, &launches);
   else
     rc = run_streaming(c, a, variant == WMD_VARIANT_FUSED, o, &launches);
   if (rc != WMD_OK) return rc;
@@ -1241,9 +1300,9 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
           if (!shape_fits(s, qmax, lg) || wmd::kShapes[s].cap_q() > 2 * cap0) continue;
           ++tried;
           l.shape = s;
-          HIP_TRY(launch_one(c, base, l, lg, c->stream));  // code load, caches
+          HIP_TRY(launch_one(c, base, l, c->bucket.p, lg, c->stream));  // code load, caches
           HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-          HIP_TRY(launch_one(c, base, l, lg, c->stream));
+          HIP_TRY(launch_one(c, base, l, c->bucket.p, lg, c->stream));
           HIP_TRY(hipEventRecord(c->ev[1], c->stream));
           HIP_TRY(hipEventSynchronize(c->ev[1]));
           float ms = 0.f;
@@ -1259,6 +1318,7 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
     }
   }
   c->tune[lg ? 1 : 0] = table;
+  ++c->plan_epoch;
   c->ran = false;  // the tuning launches overwrote the results
   if (ms_out)
     *ms_out = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -1288,6 +1348,7 @@ int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table)
   const bool lg = domain == WMD_DOMAIN_LOG;
   if (!table) {
     c->tune[lg ? 1 : 0].clear();
+    ++c->plan_epoch;
     return WMD_OK;
   }
   for (int cls = 0; cls < wmd::kNumDocClasses; ++cls)
@@ -1300,6 +1361,7 @@ int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table)
       }
     }
   c->tune[lg ? 1 : 0].assign(table, table + (size_t)wmd::kNumDocClasses * kTuneStride);
+  ++c->plan_epoch;
   return WMD_OK;
 }
 

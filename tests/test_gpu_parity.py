@@ -158,6 +158,69 @@ def test_forced_shapes_agree(P, dg):
             assert rel_err(w, ref) < 2e-6, s
 
 
+def test_tile_class_boundaries_and_maximum_sizes(P, dg, oracle):
+    # documents of 1 word and of exactly each length-class boundary up to the
+    # 160-word maximum; queries of 1 word and of the 512-word maximum
+    say("[case] class boundaries, n = 1..160, q = 1..512")
+    rng = np.random.default_rng(17)
+    V, D = 20_000, 300
+    vecs = dg.make_embeddings(rng, V, D, norm_lognormal=(0.7, 0.3))
+    cdf = dg.zipf_cdf(V)
+    lens = [1, 2, 31, 32, 33, 63, 64, 65, 95, 96, 97, 127, 128, 129, 159, 160] * 3
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    words = np.concatenate([dg.distinct_zipf(rng, cdf, n) for n in lens])
+    vals = np.repeat(1.0 / np.array(lens, dtype=np.float64), lens)
+    corpus = dg.Corpus(vecs, ptr, words, vals)
+    qs = dg.make_queries(rng, V, [1, 7, 64, 65, 300, 512])
+    v64 = vecs.astype(np.float64)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        st = e.run(-10.0, max_iter=15)
+        assert st["variant"] == RES
+        w15 = e.result()[0]
+        e.run(-10.0, max_iter=20000, tol=1e-4, accel=True)
+        wt, itt, rt = e.result()
+    wo15, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, v64, ptr, words, vals,
+                                     -10.0, 15)
+    truth, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, v64, ptr, words, vals,
+                                      -10.0, 100000, tol=1e-10)
+    for k, q in enumerate(qs.lengths()):
+        e15, et = rel_err(w15[k], wo15[k]), rel_err(wt[k], truth[k])
+        say(f"  q={q:3d}: 15 iterations {e15:.2e}; Anderson to 1e-4 {et:.2e} "
+            f"(iters max {itt[k].max()})")
+        assert e15 <= 1e-5
+        assert et <= 2e-4
+    assert np.all(rt < 1e-4)
+
+
+def test_solver_graph_replays_match_direct_launches(P, dg):
+    # the solver stage captured in a HIP graph: every replay (with direct
+    # launches, other options and synchronisations in between) must equal
+    # the directly launched result bitwise
+    say("[case] solver-stage graph replays vs direct launches")
+    corpus, _, _ = dg.workload("batched", seed=4, N=3000, n_queries=1)
+    qs = _batch_queries(dg, corpus.V, [4, 17, 33, 60, 100, 200, 290], seed=8)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        kw = dict(max_iter=10000, tol=1e-4, accel=True)
+        e.run(-10.0, graph=False, **kw)
+        ref, it_ref, _ = e.result()
+        for rep in range(3):
+            st = e.run(-10.0, graph=True, **kw)
+            w, it, _ = e.result()
+            assert np.array_equal(w, ref) and np.array_equal(it, it_ref), rep
+            e.run(-10.0, max_iter=15, graph=False)          # other work in between
+            e.run(-10.0, max_iter=15, graph=True)           # another captured graph
+            assert st["launches"] > 0
+        # new queries invalidate the plan and the graph
+        qs2 = _batch_queries(dg, corpus.V, [9, 70], seed=9)
+        e.set_queries(qs2.q_ptr, qs2.words, qs2.weights)
+        e.run(-10.0, graph=False, **kw)
+        ref2 = e.result()[0]
+        e.run(-10.0, graph=True, **kw)
+        assert np.array_equal(e.result()[0], ref2)
+
+
 def test_autotuned_shapes_agree(P, dg):
     say("[case] autotuner: tuned table, same results, table round trip")
     corpus, _, _ = dg.workload("batched", seed=2, N=2000, n_queries=1)
