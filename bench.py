@@ -294,7 +294,8 @@ def main() -> int:
     tune_ms = tune_changed = None
     if not args.no_autotune:
         default_table = eng.get_tuning()
-        tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma)
+        tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma,
+                               accel=args.accel == "anderson", omega=args.omega)
         tune_changed = int((eng.get_tuning() != default_table).sum())
         log(f"tile shapes autotuned in {tune_ms:.0f} ms ({tune_changed} table entries differ "
             f"from the smallest-fitting shape)")

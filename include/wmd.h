@@ -203,6 +203,13 @@ int wmd_context_result(wmd_context* ctx, double* wmd, int32_t* iters, float* res
  * set_queries / destroy), for callers that keep results on the GPU. */
 const double* wmd_context_device_result(const wmd_context* ctx);
 
+/* The k nearest documents of every query of the last run (no reference
+ * counterpart: the reference writes all N scores, sinkhorn_wmd_fused.cpp:
+ * 614-622; retrieval is the use its README names): doc_ids[n_queries][k] and
+ * wmd[n_queries][k], ascending distance, ties in document order, documents
+ * without a distance (NaN) after every other.  1 <= k <= N. */
+int wmd_context_topk(wmd_context* ctx, int32_t k, int32_t* doc_ids, double* wmd);
+
 /* ---- Learned warm start (no reference counterpart).  A 3D -> 256 -> 256 -> 1
  * MLP with tanh-form GELU hidden layers predicts, for every (query, document
  * word), the converged log v_e from [word embedding; document mean embedding;
@@ -299,6 +306,44 @@ int wmd_write_mtx(const char* path, uint64_t V, uint64_t N, const uint64_t* row_
                   const uint64_t* col_inds, const double* vals);
 int wmd_write_vecs(const char* path, uint64_t V, uint64_t D, const double* vecs);
 int wmd_write_r(const char* path, uint64_t V, const double* r);
+
+/* ---- Corpus preparation from raw text (prep.py:17-50 load_vecs + docs2mat,
+ * prep.py:88-91 reading the document file).  Host code; no device needed. */
+enum wmd_text_format {
+  /* comma-separated records ('"' quoting, '""' escapes); a document is the
+   * second field, one space, the third field (DBpedia "class,title,content",
+   * prep.py:89-90) */
+  WMD_TEXT_CSV = 0,
+  /* one document per line ('\n' or "\r\n" endings) */
+  WMD_TEXT_LINES = 1
+};
+enum wmd_text_which { WMD_TEXT_TOKENS = 0, WMD_TEXT_DOCS = 1 };
+typedef struct wmd_prepared wmd_prepared;
+/* Reads the first n_toks embeddings of a fastText .vec file (header "count
+ * dim"; a token repeated later maps to its last line) and the documents of
+ * text_path; keeps the documents with at least min_toks distinct vocabulary
+ * tokens (tokens split on single spaces); builds the V x N word-major CSR
+ * with weight 1/len(document) per distinct token, documents ascending in each
+ * row (the matrix prep.py:50 saves).  *out is released by wmd_prepared_free. */
+int wmd_prepare_corpus(const char* vec_path, const char* text_path, int text_format,
+                       uint64_t n_toks, uint64_t min_toks, wmd_prepared** out);
+int wmd_prepared_dims(const wmd_prepared* p, uint64_t* V, uint64_t* D, uint64_t* N,
+                      uint64_t* nnz);
+/* Copies out V x D embeddings (row-major), the CSR (row_ptr V+1, col_inds and
+ * vals nnz), per kept document its record index in text_path (records
+ * counted from 0; blank lines are not records in the CSV format), and N+1
+ * byte offsets into the WMD_TEXT_DOCS text: document j is the bytes
+ * [doc_offsets[j], doc_offsets[j+1] - 1) (a document may itself hold line
+ * breaks).  Any output may be null. */
+int wmd_prepared_copy(const wmd_prepared* p, double* vecs, uint64_t* row_ptr,
+                      uint64_t* col_inds, double* vals, uint64_t* kept_records,
+                      uint64_t* doc_offsets);
+/* The tokens (which = WMD_TEXT_TOKENS) or the kept documents' text
+ * (WMD_TEXT_DOCS), '\n'-joined (prep.py:98-99); *needed = its byte length;
+ * buf may be null to ask for the length only. */
+int wmd_prepared_text(const wmd_prepared* p, int which, char* buf, uint64_t cap,
+                      uint64_t* needed);
+void wmd_prepared_free(wmd_prepared* p);
 
 /* Human-readable description of the last error on this thread. */
 const char* wmd_last_error(void);

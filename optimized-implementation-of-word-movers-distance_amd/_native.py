@@ -48,7 +48,13 @@ EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_
            "wmd_mlp_loss_grad", "wmd_context_set_warm", "wmd_context_autotune",
            "wmd_tuning_dims", "wmd_context_get_tuning", "wmd_context_set_tuning",
            "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims", "wmd_read_vecs", "wmd_read_r",
-           "wmd_write_scores", "wmd_write_mtx", "wmd_write_vecs", "wmd_write_r")
+           "wmd_write_scores", "wmd_write_mtx", "wmd_write_vecs", "wmd_write_r",
+           "wmd_context_topk", "wmd_prepare_corpus", "wmd_prepared_dims", "wmd_prepared_copy",
+           "wmd_prepared_text", "wmd_prepared_free")
+TEXT_CSV = 0
+TEXT_LINES = 1
+TEXT_TOKENS = 0
+TEXT_DOCS = 1
 
 
 class Options(C.Structure):
@@ -141,7 +147,16 @@ def load():
     L.wmd_write_mtx.argtypes = [S, U64, U64, P, P, P]
     L.wmd_write_vecs.argtypes = [S, U64, U64, P]
     L.wmd_write_r.argtypes = [S, U64, P]
-    for name in ("wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
+    L.wmd_context_topk.argtypes = [P, C.c_int32, P, P]
+    L.wmd_prepare_corpus.argtypes = [S, S, C.c_int, U64, U64, C.POINTER(C.c_void_p)]
+    L.wmd_prepared_dims.argtypes = [P, P, P, P, P]
+    L.wmd_prepared_copy.argtypes = [P, P, P, P, P, P, P]
+    L.wmd_prepared_text.argtypes = [P, C.c_int, P, U64, P]
+    L.wmd_prepared_free.argtypes = [P]
+    L.wmd_prepared_free.restype = None
+    for name in ("wmd_prepare_corpus", "wmd_prepared_dims", "wmd_prepared_copy",
+                 "wmd_prepared_text",
+                 "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
                  "wmd_context_create", "wmd_context_set_corpus", "wmd_context_set_queries",
                  "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info",
                  "wmd_shape_domains",
@@ -150,7 +165,7 @@ def load():
                  "wmd_context_autotune", "wmd_tuning_dims", "wmd_context_get_tuning",
                  "wmd_context_set_tuning", "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims",
                  "wmd_read_vecs", "wmd_read_r", "wmd_write_scores", "wmd_write_mtx",
-                 "wmd_write_vecs", "wmd_write_r"):
+                 "wmd_write_vecs", "wmd_write_r", "wmd_context_topk"):
         getattr(L, name).restype = C.c_int
     _lib = L
     return L

@@ -141,12 +141,25 @@ class WMDEngine:
                   "wmd_context_run")
         return st.as_dict()
 
+    def topk(self, k):
+        """(doc_ids [nq, k] int32, wmd [nq, k]) of the k nearest documents of
+        every query of the last run, ascending, ties in document order."""
+        ids = np.empty((self.n_queries, int(k)), dtype=np.int32)
+        dist = np.empty((self.n_queries, int(k)), dtype=np.float64)
+        nat.check(self._L.wmd_context_topk(self._h, int(k), ids.ctypes.data, dist.ctypes.data),
+                  "wmd_context_topk")
+        return ids, dist
+
     # ---- tile-shape autotuner ----
-    def autotune(self, lamda, iters=16, log_domain=False, mfma_cdist=True) -> float:
-        """Time candidate tile shapes on the current corpus and queries and keep
-        the fastest per (document class, query-length bucket); returns ms."""
+    def autotune(self, lamda, iters=16, log_domain=False, mfma_cdist=True, accel=False,
+                 omega=1.0) -> float:
+        """Time candidate tile shapes on the current corpus and queries, with
+        the update rule later runs use, and keep the fastest per (document
+        class, query-length bucket); returns ms."""
         o = nat.default_options(domain=nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR,
-                                cdist=nat.CDIST_MFMA if mfma_cdist else nat.CDIST_FP32)
+                                cdist=nat.CDIST_MFMA if mfma_cdist else nat.CDIST_FP32,
+                                accel=nat.ACCEL_ANDERSON if accel else nat.ACCEL_NONE,
+                                omega=float(omega))
         ms = C.c_double()
         nat.check(self._L.wmd_context_autotune(self._h, float(lamda), C.byref(o), int(iters),
                                                C.byref(ms)), "wmd_context_autotune")

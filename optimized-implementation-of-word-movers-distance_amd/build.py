@@ -21,7 +21,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(PKG_DIR)
 LIB_PATH = os.path.join(PKG_DIR, "libwmd.so")
 UNITS = ("wmd_kernels.hip", *(f"wmd_resident_{p}.hip" for p in range(8)), "wmd_mlp.hip",
-         "wmd_capi.hip", "wmd_io.cpp")
+         "wmd_topk.hip", "wmd_capi.hip", "wmd_io.cpp")
 SOURCES = [os.path.join(PKG_DIR, "csrc", f) for f in UNITS]
 # The reference program's command line (reads its data files, writes scores).
 CLI_SOURCE = os.path.join(PKG_DIR, "csrc", "wmd_b.cpp")

@@ -1248,10 +1248,15 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
     return WMD_ERR_ARG;
   }
   // The distance stage of a zero-iteration run leaves K and mrow for lamda in
-  // the options' domain (and validates corpus, queries and options).
+  // the options' domain (and validates corpus, queries and options).  The
+  // candidates are then timed with the options' update rule (plain,
+  // over-relaxed or Anderson), whose per-iteration cost differs.
+  const int accel = o.accel;
+  const double omega = o.omega;
   o.variant = WMD_VARIANT_RESIDENT;
   o.max_iter = 0;
   o.tol = 0.0;
+  o.accel = WMD_ACCEL_NONE;
   o.shape = -1;
   o.warm_start = WMD_WARM_UNIFORM;
   o.export_duals = 0;
@@ -1266,9 +1271,13 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
   if (rc != WMD_OK) return rc;
   const bool lg = o.domain == WMD_DOMAIN_LOG;
   o.max_iter = iters;
-  o.omega = 1.0;
+  o.omega = omega;
+  o.accel = accel;
   wmd::SolveArgs base = base_args(c, lamda, o);
   base.fallbacks = c->fallbacks.p;
+  // A tolerance no pair reaches keeps every candidate at `iters` iterations
+  // (the accelerated rule needs one to run).
+  if (accel != WMD_ACCEL_NONE) base.tol = 1e-30f;
 
   std::vector<int32_t> table = default_tuning(lg);
   std::vector<int32_t> ids;
@@ -1391,6 +1400,60 @@ int wmd_context_result(wmd_context* c, double* out, int32_t* iters, float* resid
 
 const double* wmd_context_device_result(const wmd_context* c) {
   return (c && c->ran) ? c->wmd.p : nullptr;
+}
+
+int wmd_context_topk(wmd_context* c, int32_t k, int32_t* doc_ids, double* wmd_out) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->ran) {
+    set_err("run must complete before topk");
+    return WMD_ERR_STATE;
+  }
+  if (k < 1 || k > c->N || !doc_ids || !wmd_out) {
+    set_err("k must be in [1, N = %lld] and outputs non-null", (long long)c->N);
+    return WMD_ERR_ARG;
+  }
+  const size_t total = (size_t)c->nq * c->N, nk = (size_t)c->nq * k;
+  if (total > (size_t)INT32_MAX) {
+    set_err("queries x documents beyond 2^31 for the segmented sort");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  DevBuf<double> keys_in, keys_out, dist;
+  DevBuf<int32_t> vals_in, vals_out, offsets, idx;
+  DevBuf<uint8_t> temp;
+  wmd::TopkArgs t{};
+  t.temp_bytes = wmd::topk_temp_bytes(c->nq, (int)c->N);
+  HIP_TRY(keys_in.ensure(total));
+  HIP_TRY(keys_out.ensure(total));
+  HIP_TRY(vals_in.ensure(total));
+  HIP_TRY(vals_out.ensure(total));
+  HIP_TRY(offsets.ensure((size_t)c->nq + 1));
+  HIP_TRY(temp.ensure(t.temp_bytes));
+  HIP_TRY(dist.ensure(nk));
+  HIP_TRY(idx.ensure(nk));
+  t.wmd = c->wmd.p;
+  t.nq = c->nq;
+  t.N = (int)c->N;
+  t.k = k;
+  t.keys_in = keys_in.p;
+  t.keys_out = keys_out.p;
+  t.vals_in = vals_in.p;
+  t.vals_out = vals_out.p;
+  t.offsets = offsets.p;
+  t.temp = temp.p;
+  t.dist = dist.p;
+  t.idx = idx.p;
+  hipError_t e = wmd::launch_topk(t, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(doc_ids, idx.p, nk * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(wmd_out, dist.p, nk * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  keys_in.release(); keys_out.release(); vals_in.release(); vals_out.release();
+  offsets.release(); temp.release(); dist.release(); idx.release();
+  if (e != hipSuccess) {
+    set_err("topk failed: %s", hipGetErrorString(e));
+    return WMD_ERR_HIP;
+  }
+  return WMD_OK;
 }
 
 int wmd_sinkhorn_ex(const double* r, const double* vecs, double* WMD, double lamda, uint64_t V,

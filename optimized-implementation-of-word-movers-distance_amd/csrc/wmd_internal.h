@@ -231,6 +231,23 @@ hipError_t launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t 
 hipError_t launch_mlp_infer(const MlpInferArgs& a, int n_cu, hipStream_t s);
 size_t mlp_infer_lds_bytes();
 
+// Nearest documents per query (wmd_topk.hip).
+struct TopkArgs {
+  const double* wmd;   // nq x N results
+  int nq, N, k;
+  double* keys_in;     // nq x N scratch
+  double* keys_out;    // nq x N scratch
+  int32_t* vals_in;    // nq x N scratch
+  int32_t* vals_out;   // nq x N scratch
+  int32_t* offsets;    // nq + 1 scratch
+  void* temp;          // topk_temp_bytes(nq, N) bytes
+  size_t temp_bytes;
+  double* dist;        // nq x k out
+  int32_t* idx;        // nq x k out
+};
+size_t topk_temp_bytes(int nq, int N);
+hipError_t launch_topk(const TopkArgs& t, hipStream_t s);
+
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
 hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s);
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s);

@@ -11,15 +11,18 @@
 //
 // Files are read whole and parsed with strtoull/strtod.  Where the reference
 // prints a message and exits, these return WMD_ERR_ARG with the message in
-// wmd_last_error().
+// wmd_last_error().  The second half builds the doc-term matrix itself from
+// raw text, as prep.py does (wmd_prepare_corpus).
 #include "../../include/wmd.h"
 
+#include <algorithm>
 #include <cerrno>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace wmd_io {
@@ -304,6 +307,295 @@ int wmd_write_r(const char* path, uint64_t V, const double* r) {
   for (uint64_t v = 0; v < V; ++v) std::fprintf(f, "%.18e\n", r[v]);
   return close_out(f, path);
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Corpus preparation from text (prep.py:17-50): embeddings of the first
+// n_toks tokens of a fastText .vec file ("count dim" header, then "token v1 ..
+// vdim" lines); documents are either the records of a comma-separated file
+// whose second and third fields, joined by one space, make the text (the
+// DBpedia "class,title,content" layout prep.py:89-90 reads), or the lines of a
+// plain text file; tokens split on single spaces; each document keeps its
+// distinct in-vocabulary token ids, sorted, and is dropped with fewer than
+// min_toks of them; weights are binary, normalised per document (1 / its
+// length); the result is the V x N word-major CSR.
+struct wmd_prepared {
+  uint64_t V = 0, D = 0, N = 0;
+  std::vector<double> vecs;
+  std::vector<uint64_t> row_ptr, col_inds, kept;
+  std::vector<double> vals;
+  std::string tokens_text, docs_text;  // '\n'-joined tokens and kept documents
+  std::vector<uint64_t> doc_start;     // byte offset of each kept document in docs_text
+};
+
+namespace {
+
+// Splits a line on single spaces (empty pieces included), like str.split(' ').
+template <typename F>
+void split_spaces(const char* p, const char* end, F&& f) {
+  const char* s = p;
+  for (const char* q = p; q <= end; ++q) {
+    if (q == end || *q == ' ') {
+      f(s, q);
+      s = q + 1;
+    }
+  }
+}
+
+const char* line_end(const char* p, const char* end) {
+  const char* q = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+  return q ? q : end;
+}
+
+// The ASCII characters str.rstrip() removes.
+bool py_space(char ch) {
+  return ch == ' ' || (ch >= '\t' && ch <= '\r') || (ch >= '\x1c' && ch <= '\x1f');
+}
+
+// Steps over one line ending ('\n', '\r' or "\r\n") at c, if there is one.
+bool eat_eol(const char*& c, const char* e) {
+  if (c < e && *c == '\n') {
+    ++c;
+    return true;
+  }
+  if (c < e && *c == '\r') {
+    c += (c + 1 < e && c[1] == '\n') ? 2 : 1;
+    return true;
+  }
+  return false;
+}
+
+// One record of a comma-separated file with '"' quoting ('""' inside quotes
+// is a quote; a quoted field may span lines; a quote later in an unquoted
+// field is literal); records end at '\n', '\r' or "\r\n" (pandas' reader
+// takes all three); blank lines are skipped.  Returns false at the end.
+bool csv_record(const char*& c, const char* e, std::vector<std::string>* fields) {
+  while (eat_eol(c, e)) {
+  }
+  if (c >= e) return false;
+  fields->clear();
+  fields->emplace_back();
+  bool quoted = false, in_field_quote = false;
+  while (c < e) {
+    const char ch = *c;
+    if (in_field_quote) {
+      if (ch == '"') {
+        if (c + 1 < e && c[1] == '"') {
+          fields->back().push_back('"');
+          c += 2;
+          continue;
+        }
+        in_field_quote = false;
+      } else {
+        fields->back().push_back(ch);
+      }
+      ++c;
+      continue;
+    }
+    if (ch == '"' && fields->back().empty() && !quoted) {
+      quoted = in_field_quote = true;
+      ++c;
+      continue;
+    }
+    if (ch == ',') {
+      fields->emplace_back();
+      quoted = false;
+      ++c;
+      continue;
+    }
+    if (eat_eol(c, e)) return true;
+    fields->back().push_back(ch);
+    ++c;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wmd_prepare_corpus(const char* vec_path, const char* text_path, int text_format,
+                       uint64_t n_toks, uint64_t min_toks, wmd_prepared** out) {
+  if (!out || n_toks == 0 || (text_format != WMD_TEXT_CSV && text_format != WMD_TEXT_LINES)) {
+    set_error("null output, n_toks = 0 or unknown text format %d", text_format);
+    return WMD_ERR_ARG;
+  }
+  *out = nullptr;
+  std::string vtext, dtext;
+  if (!slurp(vec_path, &vtext) || !slurp(text_path, &dtext)) return WMD_ERR_ARG;
+  auto* p = new wmd_prepared();
+  auto fail = [&](void) {
+    delete p;
+    return WMD_ERR_ARG;
+  };
+  // --- embeddings (prep.py:17-29)
+  const char* c = vtext.data();
+  const char* e = c + vtext.size();
+  const char* le = line_end(c, e);
+  unsigned long long count = 0, dim = 0;
+  if (std::sscanf(std::string(c, le).c_str(), "%llu %llu", &count, &dim) != 2 || dim == 0) {
+    set_error("%s: header 'count dim' missing", vec_path);
+    return fail();
+  }
+  p->D = dim;
+  std::unordered_map<std::string, uint64_t> lookup;
+  std::vector<double> row;
+  for (uint64_t v = 0; v < n_toks; ++v) {
+    if (le == e) {
+      set_error("%s: %llu embeddings, %llu requested", vec_path, (unsigned long long)v,
+                (unsigned long long)n_toks);
+      return fail();
+    }
+    c = le + 1;
+    le = line_end(c, e);
+    const char* stop = le;
+    while (stop > c && py_space(stop[-1])) --stop;
+    std::string token;
+    row.clear();
+    bool first = true, bad = false;
+    split_spaces(c, stop, [&](const char* a, const char* b) {
+      if (first) {
+        token.assign(a, b);
+        first = false;
+        return;
+      }
+      char* q = nullptr;
+      const std::string s(a, b);
+      const double x = std::strtod(s.c_str(), &q);
+      if (s.empty() || *q != '\0') bad = true;
+      row.push_back(x);
+    });
+    if (bad || row.size() != dim) {
+      set_error("%s: line %llu does not hold a token and %llu values", vec_path,
+                (unsigned long long)v + 2, (unsigned long long)dim);
+      return fail();
+    }
+    p->vecs.insert(p->vecs.end(), row.begin(), row.end());
+    lookup[token] = v;  // a repeated token maps to its last line, as dict(zip(...)) does
+    p->tokens_text.append(token);
+    if (v + 1 < n_toks) p->tokens_text.push_back('\n');
+  }
+  p->V = n_toks;
+  // --- documents (prep.py:32-50, 88-91)
+  std::vector<std::vector<uint64_t>> docs;
+  c = dtext.data();
+  e = c + dtext.size();
+  uint64_t record = 0;
+  size_t nfields = 0;
+  std::vector<uint64_t> ids;
+  std::vector<std::string> fields;
+  std::string doc;
+  while (c < e) {
+    if (text_format == WMD_TEXT_CSV) {
+      if (!csv_record(c, e, &fields)) break;
+      if (nfields == 0) nfields = fields.size();
+      if (fields.size() < 3 || fields.size() != nfields) {
+        set_error("%s: record %llu has %llu fields (3 or more, as many as the first)", text_path,
+                  (unsigned long long)record + 1, (unsigned long long)fields.size());
+        return fail();
+      }
+      doc = fields[1] + ' ' + fields[2];
+    } else {
+      le = line_end(c, e);
+      const char* stop = (le > c && le[-1] == '\r') ? le - 1 : le;
+      doc.assign(c, stop);
+      c = (le == e) ? e : le + 1;
+    }
+    ids.clear();
+    split_spaces(doc.data(), doc.data() + doc.size(), [&](const char* a, const char* b) {
+      auto it = lookup.find(std::string(a, b));
+      if (it != lookup.end()) ids.push_back(it->second);
+    });
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    if (ids.size() >= min_toks) {
+      docs.push_back(ids);
+      p->kept.push_back(record);
+      if (docs.size() > 1) p->docs_text.push_back('\n');
+      p->doc_start.push_back(p->docs_text.size());
+      p->docs_text.append(doc);
+    }
+    ++record;
+  }
+  p->N = docs.size();
+  if (p->N == 0) {
+    set_error("%s: no document with %llu or more vocabulary tokens", text_path,
+              (unsigned long long)min_toks);
+    return fail();
+  }
+  // Transpose to V x N with documents ascending in every row.
+  std::vector<uint64_t> deg(p->V + 1, 0);
+  for (const auto& d : docs)
+    for (uint64_t t : d) ++deg[t];
+  p->row_ptr.assign(p->V + 1, 0);
+  for (uint64_t t = 0; t < p->V; ++t) p->row_ptr[t + 1] = p->row_ptr[t] + deg[t];
+  p->col_inds.resize(p->row_ptr[p->V]);
+  p->vals.resize(p->row_ptr[p->V]);
+  std::vector<uint64_t> fill(p->row_ptr.begin(), p->row_ptr.end() - 1);
+  for (uint64_t j = 0; j < p->N; ++j) {
+    const double w = 1.0 / (double)docs[j].size();
+    for (uint64_t t : docs[j]) {
+      const uint64_t k = fill[t]++;
+      p->col_inds[k] = j;
+      p->vals[k] = w;
+    }
+  }
+  *out = p;
+  return WMD_OK;
+}
+
+int wmd_prepared_dims(const wmd_prepared* p, uint64_t* V, uint64_t* D, uint64_t* N,
+                      uint64_t* nnz) {
+  if (!p || !V || !D || !N || !nnz) {
+    set_error("null argument");
+    return WMD_ERR_ARG;
+  }
+  *V = p->V;
+  *D = p->D;
+  *N = p->N;
+  *nnz = p->row_ptr.empty() ? 0 : p->row_ptr.back();
+  return WMD_OK;
+}
+
+int wmd_prepared_copy(const wmd_prepared* p, double* vecs, uint64_t* row_ptr, uint64_t* col_inds,
+                      double* vals, uint64_t* kept_records, uint64_t* doc_offsets) {
+  if (!p) {
+    set_error("null corpus");
+    return WMD_ERR_ARG;
+  }
+  if (vecs) std::memcpy(vecs, p->vecs.data(), p->vecs.size() * sizeof(double));
+  if (row_ptr) std::memcpy(row_ptr, p->row_ptr.data(), p->row_ptr.size() * sizeof(uint64_t));
+  if (col_inds) std::memcpy(col_inds, p->col_inds.data(), p->col_inds.size() * sizeof(uint64_t));
+  if (vals) std::memcpy(vals, p->vals.data(), p->vals.size() * sizeof(double));
+  if (kept_records) std::memcpy(kept_records, p->kept.data(), p->kept.size() * sizeof(uint64_t));
+  if (doc_offsets) {
+    std::memcpy(doc_offsets, p->doc_start.data(), p->doc_start.size() * sizeof(uint64_t));
+    doc_offsets[p->doc_start.size()] = p->docs_text.size() + 1;
+  }
+  return WMD_OK;
+}
+
+int wmd_prepared_text(const wmd_prepared* p, int which, char* buf, uint64_t cap,
+                      uint64_t* needed) {
+  if (!p || (which != WMD_TEXT_TOKENS && which != WMD_TEXT_DOCS)) {
+    set_error("null corpus or unknown text %d", which);
+    return WMD_ERR_ARG;
+  }
+  const std::string& s = which == WMD_TEXT_TOKENS ? p->tokens_text : p->docs_text;
+  if (needed) *needed = s.size();
+  if (buf) {
+    if (cap < s.size()) {
+      set_error("buffer of %llu bytes, %llu needed", (unsigned long long)cap,
+                (unsigned long long)s.size());
+      return WMD_ERR_ARG;
+    }
+    std::memcpy(buf, s.data(), s.size());
+  }
+  return WMD_OK;
+}
+
+void wmd_prepared_free(wmd_prepared* p) { delete p; }
 
 int wmd_write_scores(const char* path, const double* wmd, uint64_t N) {
   if (!path || !wmd) {

@@ -221,6 +221,32 @@ def test_solver_graph_replays_match_direct_launches(P, dg):
         assert np.array_equal(e.result()[0], ref2)
 
 
+def test_topk_nearest_documents(P, dg):
+    say("[case] k nearest documents per query vs a stable sort of the full result")
+    corpus, _, _ = dg.workload("batched", seed=6, N=2500, n_queries=1)
+    # one empty document: its distance is NaN and it must come last
+    lens = np.diff(corpus.doc_ptr)
+    lens[7] = 0
+    ptr = np.concatenate([[0], np.cumsum(lens)])
+    keep = np.concatenate([np.arange(corpus.doc_ptr[j], corpus.doc_ptr[j] + lens[j])
+                           for j in range(corpus.N)])
+    corpus = dg.Corpus(corpus.vecs, ptr, corpus.doc_words[keep], corpus.doc_vals[keep])
+    qs = _batch_queries(dg, corpus.V, [3, 20, 50, 140], seed=10)
+    with engine(P, corpus) as e:
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=10000, tol=1e-4, accel=True)
+        w = e.result(allow_nonfinite=True)[0]
+        for k in (1, 10, corpus.N):
+            ids, dist = e.topk(k)
+            key = np.where(np.isnan(w), np.inf, w)
+            want = np.argsort(key, axis=1, kind="stable")[:, :k]
+            assert np.array_equal(ids, want), k
+            assert np.array_equal(dist, np.take_along_axis(w, want, axis=1), equal_nan=True)
+        assert np.all(ids[:, -1] == 7)  # the empty document, last
+        with pytest.raises(P.WMDError):
+            e.topk(corpus.N + 1)
+
+
 def test_autotuned_shapes_agree(P, dg):
     say("[case] autotuner: tuned table, same results, table round trip")
     corpus, _, _ = dg.workload("batched", seed=2, N=2000, n_queries=1)
