@@ -98,6 +98,21 @@ enum wmd_accel {
   WMD_ACCEL_ANDERSON = 1
 };
 
+/* Reuse of distances across the queries of a batch (sinkhorn_wmd_fused.cpp:
+ * 197-223 computes them per query).  In the shared form the distances of the
+ * batch's distinct query words to the corpus words are computed once and
+ * every query's row-rescaled kernel block is gathered from them; outputs are
+ * bitwise equal to the per-query form. */
+enum wmd_share {
+  /* shared when the batch has at least 5 query words per 4 distinct ones,
+   * no query is longer than 512 words and there are at most 16384 distinct
+   * words */
+  WMD_SHARE_AUTO = 0,
+  WMD_SHARE_OFF = 1,   /* per-query distances */
+  WMD_SHARE_ON = 2     /* shared whenever the limits above allow, else
+                          WMD_ERR_UNSUPPORTED */
+};
+
 typedef struct wmd_options {
   int32_t max_iter;    /* iteration cap; the reference runs exactly 15 (:12) */
   int32_t variant;     /* enum wmd_variant */
@@ -125,7 +140,8 @@ typedef struct wmd_options {
                           stage from a HIP graph captured on first use and
                           replayed while queries, tuning and options stay the
                           same; 0 launches the kernels directly */
-  int32_t reserved[3];
+  int32_t share;       /* enum wmd_share: distance reuse across a batch */
+  int32_t reserved[2];
 } wmd_options;
 
 typedef struct wmd_stats {
@@ -140,7 +156,9 @@ typedef struct wmd_stats {
   int32_t variant;     /* variant that ran */
   int32_t fallbacks;   /* warm-started pairs restarted from the uniform start */
   double ms_warm;      /* learned warm start: MLP inference stage */
-  int32_t reserved[4];
+  int32_t shared;      /* 1: the distance stage ran in the shared form */
+  int32_t distinct;    /* distinct query words of the batch */
+  int32_t reserved[2];
 } wmd_stats;
 
 /* Default options: max_iter 15, resident variant, tol 0, automatic shape,

@@ -35,6 +35,9 @@ WARM_LEARNED = 1
 WARM_GIVEN = 2
 ACCEL_NONE = 0
 ACCEL_ANDERSON = 1
+SHARE_AUTO = 0
+SHARE_OFF = 1
+SHARE_ON = 2
 MLP_HIDDEN = 256
 
 # Every symbol include/wmd.h declares.
@@ -62,7 +65,7 @@ class Options(C.Structure):
                 ("check_every", C.c_int32), ("shape", C.c_int32), ("summarize", C.c_int32),
                 ("domain", C.c_int32), ("cdist", C.c_int32), ("warm_start", C.c_int32),
                 ("export_duals", C.c_int32), ("accel", C.c_int32), ("omega", C.c_double),
-                ("graph", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("graph", C.c_int32), ("share", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class Stats(C.Structure):
@@ -70,7 +73,7 @@ class Stats(C.Structure):
                 ("pairs", C.c_int64), ("iters_max", C.c_int32), ("launches", C.c_int32),
                 ("iters_mean", C.c_double), ("resid_max", C.c_double),
                 ("variant", C.c_int32), ("fallbacks", C.c_int32), ("ms_warm", C.c_double),
-                ("reserved", C.c_int32 * 4)]
+                ("shared", C.c_int32), ("distinct", C.c_int32), ("reserved", C.c_int32 * 2)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -67,6 +67,9 @@ def sinkhorn_wmd(r, c, vecs, lamb, max_iter):
     return sinkhorn_wmd_fused(r, vecs, c, lamda=lamb, max_iter=max_iter)
 
 
+_SHARE = {"auto": nat.SHARE_AUTO, "off": nat.SHARE_OFF, "on": nat.SHARE_ON}
+
+
 class WMDEngine:
     """A corpus resident on one GPU; solves query batches against it."""
 
@@ -123,9 +126,11 @@ class WMDEngine:
 
     def run(self, lamda, max_iter=15, tol=0.0, variant=nat.VARIANT_RESIDENT, check_every=8,
             shape=-1, summarize=True, log_domain=False, mfma_cdist=True, warm_start=False,
-            export_duals=False, omega=1.0, accel=False, graph=True) -> dict:
+            export_duals=False, omega=1.0, accel=False, graph=True, share="auto") -> dict:
         """Solve every (query, doc) pair; returns the run's wmd_stats as a dict
-        (with summarize=False only stage times, no device-to-host copy)."""
+        (with summarize=False only stage times, no device-to-host copy).
+        share: "auto", "off" or "on" -- distances of the batch's distinct
+        query words computed once and reused by every query."""
         o = nat.default_options(max_iter=int(max_iter), tol=float(tol), variant=int(variant),
                                 check_every=int(check_every), shape=int(shape),
                                 summarize=int(bool(summarize)),
@@ -135,7 +140,7 @@ class WMDEngine:
                                             nat.WARM_LEARNED if warm_start else nat.WARM_UNIFORM),
                                 export_duals=int(bool(export_duals)), omega=float(omega),
                                 accel=nat.ACCEL_ANDERSON if accel else nat.ACCEL_NONE,
-                                graph=int(bool(graph)))
+                                graph=int(bool(graph)), share=_SHARE[share])
         st = nat.Stats()
         nat.check(self._L.wmd_context_run(self._h, float(lamda), C.byref(o), C.byref(st)),
                   "wmd_context_run")

@@ -156,6 +156,13 @@ struct wmd_context {
   DevBuf<float> q_r;
   DevBuf<int64_t> k_off, u_off;
   DevBuf<float> K, mrow;
+  // shared distance stage: the batch's distinct query words (ascending), the
+  // distinct index of every query word, pseudo-queries of 64 distinct words
+  // (row stride Sdp into Dd) and the Vt x Sdp raw distances
+  int Sd = 0, Sdp = 0, nqd = 0;
+  DevBuf<int32_t> d_words, slot, dq_ptr, dk_stride;
+  DevBuf<int64_t> dk_off;
+  DevBuf<float> Dd;
   DevBuf<double> wmd;
   DevBuf<int32_t> iters;
   DevBuf<float> resid;
@@ -788,6 +795,8 @@ void wmd_context_destroy(wmd_context* c) {
   c->fallbacks.release(); c->warm_vals.release();
   c->status.release();
   c->plan_bucket.release();
+  c->d_words.release(); c->slot.release(); c->dq_ptr.release(); c->dk_stride.release();
+  c->dk_off.release(); c->Dd.release();
   drop_solve_graph(c);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
@@ -953,6 +962,25 @@ int wmd_context_set_queries(wmd_context* c, uint64_t n, const int64_t* qp, const
     words[k] = (int32_t)qw[k];
     weights[k] = (float)qv[k];
   }
+  // Distinct query words of the batch and each query word's index among them.
+  std::vector<int32_t> dwords(words);
+  std::sort(dwords.begin(), dwords.end());
+  dwords.erase(std::unique(dwords.begin(), dwords.end()), dwords.end());
+  std::vector<int32_t> slot(total);
+  for (int64_t k = 0; k < total; ++k)
+    slot[k] = (int32_t)(std::lower_bound(dwords.begin(), dwords.end(), words[k]) - dwords.begin());
+  const int Sd = (int)dwords.size();
+  const int Sdp = (Sd + 63) & ~63;
+  const int nqd = Sdp / 64;
+  std::vector<int32_t> dqp(nqd + 1), dks(nqd, Sdp);
+  std::vector<int64_t> dko(nqd);
+  for (int i = 0; i <= nqd; ++i) dqp[i] = std::min(64 * i, Sd);
+  for (int i = 0; i < nqd; ++i) dko[i] = 64 * (int64_t)i;
+  HIP_TRY(c->d_words.upload(dwords.data(), dwords.size(), c->stream));
+  HIP_TRY(c->slot.upload(slot.data(), slot.size(), c->stream));
+  HIP_TRY(c->dq_ptr.upload(dqp.data(), dqp.size(), c->stream));
+  HIP_TRY(c->dk_stride.upload(dks.data(), dks.size(), c->stream));
+  HIP_TRY(c->dk_off.upload(dko.data(), dko.size(), c->stream));
   HIP_TRY(c->q_ptr.upload(hq.data(), hq.size(), c->stream));
   HIP_TRY(c->q_words.upload(words.data(), words.size(), c->stream));
   HIP_TRY(c->q_r.upload(weights.data(), weights.size(), c->stream));
@@ -967,6 +995,9 @@ int wmd_context_set_queries(wmd_context* c, uint64_t n, const int64_t* qp, const
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->nq = (int)n;
   c->qmax = qmax;
+  c->Sd = Sd;
+  c->Sdp = Sdp;
+  c->nqd = nqd;
   c->h_q_ptr = hq;
   c->h_k_stride = hs;
   c->h_k_off = ko;
@@ -1025,6 +1056,22 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     return WMD_ERR_ARG;
   }
   const bool mfma = o.cdist == WMD_CDIST_MFMA;
+  if (o.share != WMD_SHARE_AUTO && o.share != WMD_SHARE_OFF && o.share != WMD_SHARE_ON) {
+    set_err("unknown share mode %d", o.share);
+    return WMD_ERR_ARG;
+  }
+  const int sq_total = c->h_q_ptr[c->nq];
+  const bool share_ok = c->qmax <= wmd::kShareMaxQ &&
+                        (size_t)c->Sdp * sizeof(float) <= (size_t)wmd::kShareLds;
+  if (o.share == WMD_SHARE_ON && !share_ok) {
+    set_err("shared distances need queries of at most %d words and at most %d distinct words "
+            "(longest query %d, %d distinct)", wmd::kShareMaxQ,
+            wmd::kShareLds / (int)sizeof(float), c->qmax, c->Sd);
+    return WMD_ERR_UNSUPPORTED;
+  }
+  const bool share = o.share == WMD_SHARE_ON ||
+                     (o.share == WMD_SHARE_AUTO && share_ok && c->nq > 1 &&
+                      5ll * c->Sd <= 4ll * sq_total);
   if (o.warm_start != WMD_WARM_UNIFORM && o.warm_start != WMD_WARM_LEARNED &&
       o.warm_start != WMD_WARM_GIVEN) {
     set_err("unknown warm_start %d", o.warm_start);
@@ -1102,29 +1149,41 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     }
   }
 
+  // Shared form: the distance kernels run in raw mode over pseudo-queries of
+  // 64 distinct words into Dd, then k_share_rescale builds every query's block.
+  const int32_t* cd_words = share ? c->d_words.p : c->q_words.p;
+  const int32_t* cd_ptr = share ? c->dq_ptr.p : c->q_ptr.p;
+  const int64_t* cd_off = share ? c->dk_off.p : c->k_off.p;
+  const int32_t* cd_stride = share ? c->dk_stride.p : c->k_stride.p;
+  const int cd_nq = share ? c->nqd : c->nq;
+  const int cd_rows = share ? c->Sd : sq_total;
+  if (share) HIP_TRY(c->Dd.ensure((size_t)c->Vt * (size_t)c->Sdp));
+  float* cd_out = share ? c->Dd.p : c->K.p;
+
   wmd::CdistArgs ca{};
   ca.E = c->E.p;
   ca.touched = c->touched.p;
-  ca.q_ptr = c->q_ptr.p;
-  ca.q_words = c->q_words.p;
-  ca.k_off = c->k_off.p;
-  ca.k_stride = c->k_stride.p;
-  ca.K = c->K.p;
+  ca.q_ptr = cd_ptr;
+  ca.q_words = cd_words;
+  ca.k_off = cd_off;
+  ca.k_stride = cd_stride;
+  ca.K = cd_out;
   ca.mrow = c->mrow.p;
   ca.D = (int)c->D;
   ca.Vt = (int)c->Vt;
   ca.lamda = (float)lamda;
   ca.log_out = log_domain ? 1 : 0;
+  ca.raw = share ? 1 : 0;
 
   int launches = 0;
   HIP_TRY(hipEventRecord(c->ev[0], c->stream));
   if (mfma) {
-    const int sq = c->h_q_ptr[c->nq];
+    const int sq = cd_rows;
     HIP_TRY(c->Qs.ensure((size_t)3 * sq * Dp));
     HIP_TRY(c->nqw.ensure((size_t)sq));
     wmd::SplitArgs sa{};
     sa.E = c->E.p;
-    sa.ids = c->q_words.p;
+    sa.ids = cd_words;
     sa.rows = sq;
     sa.D = (int)c->D;
     sa.Dp = Dp;
@@ -1138,11 +1197,11 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     ma.nq = c->nqw.p;
     ma.E = c->E.p;
     ma.touched = c->touched.p;
-    ma.q_words = c->q_words.p;
-    ma.q_ptr = c->q_ptr.p;
-    ma.k_off = c->k_off.p;
-    ma.k_stride = c->k_stride.p;
-    ma.K = c->K.p;
+    ma.q_words = cd_words;
+    ma.q_ptr = cd_ptr;
+    ma.k_off = cd_off;
+    ma.k_stride = cd_stride;
+    ma.K = cd_out;
     ma.mrow = c->mrow.p;
     ma.D = (int)c->D;
     ma.Dp = Dp;
@@ -1150,10 +1209,29 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     ma.Sq = sq;
     ma.lamda = (float)lamda;
     ma.log_out = log_domain ? 1 : 0;
-    HIP_TRY(wmd::launch_cdist_mfma(ma, c->nq, c->stream));
+    ma.raw = share ? 1 : 0;
+    HIP_TRY(wmd::launch_cdist_mfma(ma, cd_nq, c->stream));
     launches += 2;
   } else {
-    HIP_TRY(wmd::launch_cdist(ca, c->nq, c->stream));
+    HIP_TRY(wmd::launch_cdist(ca, cd_nq, c->stream));
+    ++launches;
+  }
+  if (share) {
+    wmd::ShareArgs sh{};
+    sh.Dd = c->Dd.p;
+    sh.slot = c->slot.p;
+    sh.q_ptr = c->q_ptr.p;
+    sh.k_off = c->k_off.p;
+    sh.k_stride = c->k_stride.p;
+    sh.K = c->K.p;
+    sh.mrow = c->mrow.p;
+    sh.Vt = (int)c->Vt;
+    sh.Sdp = c->Sdp;
+    sh.nq = c->nq;
+    sh.rows = std::max(1, std::min(16, wmd::kShareLds / (c->Sdp * (int)sizeof(float))));
+    sh.lamda = (float)lamda;
+    sh.log_out = log_domain ? 1 : 0;
+    HIP_TRY(wmd::launch_share_rescale(sh, c->stream));
     ++launches;
   }
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
@@ -1199,6 +1277,8 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     stats->pairs = (int64_t)c->nq * c->N;
     stats->launches = launches;
     stats->variant = variant;
+    stats->shared = share ? 1 : 0;
+    stats->distinct = c->Sd;
     if (!o.summarize) return WMD_OK;
     const size_t np = (size_t)c->nq * c->N;
     std::vector<int32_t> it(np);

@@ -28,6 +28,7 @@ struct CdistArgs {
   int Vt;
   float lamda;             // K = exp(lamda * M), lamda < 0
   int log_out;             // 1: store base-2 logits lamda (M - mrow) log2(e) instead
+  int raw;                 // 1: store raw distances only (no rescale, no mrow)
 };
 
 // BF16 three-way split of embedding rows for the matrix-core distance stage.
@@ -58,7 +59,28 @@ struct MfmaCdistArgs {
   int D, Dp, Vt, Sq;
   float lamda;
   int log_out;
+  int raw;                 // as in CdistArgs
 };
+
+// Shared distance stage of a query batch: the raw distances of the batch's
+// distinct query words (computed once, by either distance kernel in raw mode
+// over pseudo-queries of 64 distinct words) are turned into every query's
+// row-rescaled block.
+struct ShareArgs {
+  const float* Dd;         // Vt x Sdp raw distances, row-major
+  const int32_t* slot;     // sum q: distinct-word index of every query word
+  const int32_t* q_ptr;
+  const int64_t* k_off;
+  const int32_t* k_stride;
+  float* K;
+  float* mrow;
+  int Vt, Sdp, nq;
+  int rows;                // touched words per block (rows x Sdp floats of LDS)
+  float lamda;
+  int log_out;
+};
+constexpr int kShareMaxQ = 512;          // longest query of the shared form
+constexpr int kShareLds = 64 * 1024;     // LDS bytes per block of the shared epilogue
 
 // Sinkhorn stage.  A pair is (query, document); pairs of one launch are
 // bucket_q[p / N] x (p % N), query-major.
@@ -251,6 +273,7 @@ hipError_t launch_topk(const TopkArgs& t, hipStream_t s);
 hipError_t launch_cdist(const CdistArgs& a, int nq, hipStream_t s);
 hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s);
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s);
+hipError_t launch_share_rescale(const ShareArgs& a, hipStream_t s);
 hipError_t launch_resident(int shape, int doc_class, bool log_domain, const SolveArgs& a,
                            hipStream_t s);
 // Per-shape resident launchers (wmd_resident_*.hip); launch_resident dispatches.

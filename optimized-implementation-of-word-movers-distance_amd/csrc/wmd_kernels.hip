@@ -113,10 +113,11 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
         dist[x][y] = dd;
         if (i < q && t < a.Vt) {
           rmin[x] = fminf(rmin[x], dd);
-          if (nqc > 1) Kq[(int64_t)t * ks + i] = dd;  // raw distance, rescaled below
+          if (nqc > 1 || a.raw) Kq[(int64_t)t * ks + i] = dd;  // raw distance
         }
       }
   }
+  if (a.raw) return;  // uniform over the block
 #pragma unroll
   for (int x = 0; x < 4; ++x) red[tx * 4 + x][ty] = rmin[x];
   __syncthreads();
@@ -286,6 +287,7 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
       if ((lane & 31) == 0) atomicMin(&smin[wy * 32 + i], __float_as_uint(mn));
     }
   }
+  if (a.raw) return;  // uniform over the block
   __syncthreads();
   const float l2 = a.lamda * 1.4426950408889634f;
   for (int qc = 0; qc < ks; qc += 64) {
@@ -306,6 +308,64 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
     }
   }
   if (tid < 64 && t0 + tid < a.Vt) mq[t0 + tid] = __uint_as_float(smin[tid]);
+}
+
+// ---------------------------------------------------------------------------
+// Shared distance epilogue.  Block: a.rows consecutive touched words, whose
+// Dd rows (contiguous in memory) are staged once in LDS; wave w serves
+// queries w, w + 4, ...; lane l of chunk c takes query word 64 c + l (query
+// lengths up to kShareMaxQ, so at most 8 chunks held in registers).  Per
+// (query, word row): the row minimum over the query's words, then the same
+// epilogue as the per-query kernels (bitwise-equal outputs).
+constexpr int kShareChunks = kShareMaxQ / 64;
+
+__global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
+  extern __shared__ float4 lds4[];
+  const float* rows = reinterpret_cast<const float*>(lds4);
+  const int t0 = blockIdx.x * a.rows;
+  const int nr = min(a.rows, a.Vt - t0);
+  const float4* src = reinterpret_cast<const float4*>(a.Dd + (int64_t)t0 * a.Sdp);
+  const int n4 = nr * (a.Sdp >> 2);
+  for (int k = threadIdx.x; k < n4; k += 256) lds4[k] = src[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const float l2 = a.lamda * 1.4426950408889634f;
+  const float pad = a.log_out ? kLogZero : 0.f;
+  for (int Q = w; Q < a.nq; Q += 4) {
+    const int qb = a.q_ptr[Q];
+    const int q = a.q_ptr[Q + 1] - qb;
+    const int ks = a.k_stride[Q];
+    const int nc = (ks + 63) >> 6;
+    int sl[kShareChunks];
+#pragma unroll
+    for (int c = 0; c < kShareChunks; ++c) {
+      const int j = c * 64 + lane;
+      sl[c] = (c < nc && j < q) ? a.slot[qb + j] : -1;
+    }
+    float* Kq = a.K + a.k_off[Q];
+    for (int r = 0; r < nr; ++r) {
+      const int t = t0 + r;
+      float d[kShareChunks];
+      float m = INFINITY;
+#pragma unroll
+      for (int c = 0; c < kShareChunks; ++c) {
+        d[c] = sl[c] >= 0 ? rows[r * a.Sdp + sl[c]] : INFINITY;
+        m = fminf(m, d[c]);
+      }
+      m = bfly_red<1, 32, kMin>(m);
+#pragma unroll
+      for (int c = 0; c < kShareChunks; ++c) {
+        const int j = c * 64 + lane;
+        if (c < nc && j < ks) {
+          float kv = pad;
+          if (j < q) kv = a.log_out ? l2 * (d[c] - m) : expf(a.lamda * (d[c] - m));
+          Kq[(int64_t)t * ks + j] = kv;
+        }
+      }
+      if (lane == 0) a.mrow[(int64_t)Q * a.Vt + t] = m;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -499,6 +559,15 @@ hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s) {
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
   if (a.Vt <= 0 || nq <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_cdist_mfma, dim3((a.Vt + 63) / 64, nq), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_share_rescale(const ShareArgs& a, hipStream_t s) {
+  if (a.Vt <= 0 || a.nq <= 0) return hipSuccess;
+  if (a.rows <= 0 || (a.Sdp & 63) != 0 || (size_t)a.rows * a.Sdp * sizeof(float) > kShareLds)
+    return hipErrorInvalidValue;
+  const size_t lds = (size_t)a.rows * a.Sdp * sizeof(float);
+  hipLaunchKernelGGL(k_share_rescale, dim3((a.Vt + a.rows - 1) / a.rows), dim3(256), lds, s, a);
   return hipGetLastError();
 }
 

@@ -221,6 +221,69 @@ def test_solver_graph_replays_match_direct_launches(P, dg):
         assert np.array_equal(e.result()[0], ref2)
 
 
+def _overlapping_queries(lengths, pool, seed):
+    rng = np.random.default_rng(seed)
+    words = [np.sort(rng.choice(pool, size=q, replace=False)) for q in lengths]
+    weights = [rng.uniform(0.5, 2.0, size=q) for q in lengths]
+    weights = [w / w.sum() for w in weights]
+    q_ptr = np.concatenate([[0], np.cumsum(lengths)])
+    return q_ptr, np.concatenate(words), np.concatenate(weights)
+
+
+def test_shared_distances_match_per_query_bitwise(P, dg):
+    # distances of the batch's distinct query words computed once and reused:
+    # every output must equal the per-query distance stage bitwise (same
+    # per-pair arithmetic), for both distance kernels, both domains and the
+    # streaming variant
+    corpus, _, _ = dg.workload("batched", seed=12, N=1500, n_queries=1)
+    pool = np.random.default_rng(3).choice(corpus.V, size=900, replace=False)
+    log_cap = max(cq for (_, cq, _), (_, lg) in zip(P.shapes(), P.api.shape_domains()) if lg)
+    lengths = [5, 40, 64, 65, 130, 300, 512]
+    cases = [("mfma", True, False, RES, lengths), ("fp32", False, False, RES, lengths),
+             ("mfma log", True, True, RES, [q for q in lengths if q <= log_cap]),
+             ("mfma fused", True, False, FUS, [5, 40, 65])]
+    for name, mfma, log_domain, variant, lens in cases:
+        say(f"[case] shared distances {name} q={lens}")
+        q_ptr, words, weights = _overlapping_queries(lens, pool, seed=len(lens))
+        with engine(P, corpus) as e:
+            e.set_queries(q_ptr, words, weights)
+            kw = dict(max_iter=15 if variant == FUS else 10000, tol=0.0 if variant == FUS else 1e-4,
+                      mfma_cdist=mfma, log_domain=log_domain, variant=variant)
+            st_off = e.run(-10.0, share="off", **kw)
+            w_off, it_off, r_off = e.result()
+            st_on = e.run(-10.0, share="on", **kw)
+            w_on, it_on, r_on = e.result()
+        assert st_off["shared"] == 0 and st_on["shared"] == 1
+        assert st_on["distinct"] == len(np.unique(words))
+        assert np.array_equal(w_on, w_off) and np.array_equal(it_on, it_off)
+        assert np.array_equal(r_on, r_off)
+        say(f"  bitwise equal; cdist {st_off['ms_cdist']:.2f} ms per query, "
+            f"{st_on['ms_cdist']:.2f} ms shared ({st_on['distinct']} distinct of {len(words)})")
+
+
+def test_shared_distances_automatic_choice_and_limits(P, dg):
+    say("[case] shared distances: automatic choice and limits")
+    corpus, _, _ = dg.workload("batched", seed=13, N=400, n_queries=1)
+    pool = np.arange(200, 300)
+    with engine(P, corpus) as e:
+        # heavy overlap: shared
+        e.set_queries(*_overlapping_queries([50, 60, 70, 80], pool, seed=1))
+        assert e.run(-10.0)["shared"] == 1
+        # one query: nothing to share
+        e.set_queries(*_overlapping_queries([50], pool, seed=2))
+        assert e.run(-10.0)["shared"] == 0
+        assert e.run(-10.0, share="on")["shared"] == 1
+        # disjoint queries: per query
+        e.set_queries(np.array([0, 3, 6]), np.array([1, 2, 3, 4, 5, 6]), np.full(6, 1 / 3))
+        assert e.run(-10.0)["shared"] == 0
+        # a query past 512 words: per query (streaming), and "on" is refused
+        e.set_queries(*_overlapping_queries([600, 600], np.arange(1000, 2000), seed=3))
+        assert e.run(-10.0, max_iter=3)["shared"] == 0
+        with pytest.raises(P.WMDError) as ei:
+            e.run(-10.0, max_iter=3, share="on")
+        assert ei.value.status == P._native.WMD_ERR_UNSUPPORTED
+
+
 def test_topk_nearest_documents(P, dg):
     say("[case] k nearest documents per query vs a stable sort of the full result")
     corpus, _, _ = dg.workload("batched", seed=6, N=2500, n_queries=1)
