@@ -134,8 +134,8 @@ struct SolveArgs {
 // a query-word group stays inside a 16-lane row and every cross-row step is
 // a reduce-scatter; multi-wave linear shapes put more document-word lanes on
 // long queries instead of wider query-word groups.
-constexpr int kNumDocClasses = 5;
-constexpr int kDocClassWidth = 32;
+constexpr int kNumDocClasses = 10;
+constexpr int kDocClassWidth = 16;
 struct TileShape {
   int W, B, NI, lin, log;
   constexpr int cap_q() const { return B * NI; }

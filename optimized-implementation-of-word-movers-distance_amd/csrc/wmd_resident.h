@@ -595,13 +595,18 @@ template <int W, int B, int NI, bool LIN, bool LOG>
 hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
   constexpr int A = 64 * W / B;
   constexpr int C = kDocClassWidth;
-  static_assert(kNumDocClasses == 5, "one case per document-length class");
+  static_assert(kNumDocClasses == 10, "one case per document-length class");
   switch (cls) {
     case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 2: return launch_res<W, B, NI, (3 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 3: return launch_res<W, B, NI, (4 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 4: return launch_res<W, B, NI, (5 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 5: return launch_res<W, B, NI, (6 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 6: return launch_res<W, B, NI, (7 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 7: return launch_res<W, B, NI, (8 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 8: return launch_res<W, B, NI, (9 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 9: return launch_res<W, B, NI, (10 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -166,7 +166,9 @@ def test_tile_class_boundaries_and_maximum_sizes(P, dg, oracle):
     V, D = 20_000, 300
     vecs = dg.make_embeddings(rng, V, D, norm_lognormal=(0.7, 0.3))
     cdf = dg.zipf_cdf(V)
-    lens = [1, 2, 31, 32, 33, 63, 64, 65, 95, 96, 97, 127, 128, 129, 159, 160] * 3
+    lens = [1, 2] + [n for k in range(1, 11) for n in (16 * k - 1, 16 * k, 16 * k + 1)
+                     if n <= 160]
+    lens = lens * 2
     ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     words = np.concatenate([dg.distinct_zipf(rng, cdf, n) for n in lens])
     vals = np.repeat(1.0 / np.array(lens, dtype=np.float64), lens)
