@@ -92,9 +92,11 @@ enum wmd_accel {
    * (centred) and f = g - l, the next l is l + f - gamma (dl + df), gamma =
    * (df . f) / (df . df), dl and df the changes since the previous step.  A
    * step whose marginal residual rises (or is not finite) is rejected and
-   * replaced by the previous point's plain update.  Same fixed point and the
-   * same stop test as plain Sinkhorn; needs tol > 0 (fixed-iteration runs
-   * reproduce the reference's plain iterates); omega is ignored. */
+   * replaced by the previous point's plain update.  A pair whose best
+   * residual does not improve over a 64-iteration window continues with
+   * plain steps.  Same fixed point and the same stop test as plain
+   * Sinkhorn; needs tol > 0 (fixed-iteration runs reproduce the reference's
+   * plain iterates); omega is ignored. */
   WMD_ACCEL_ANDERSON = 1
 };
 

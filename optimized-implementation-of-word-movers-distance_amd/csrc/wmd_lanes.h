@@ -24,9 +24,12 @@ constexpr float kLogZero = -1.0e30f;
 //   xor 32     v_permlane32_swap (gfx950)
 // A swap of two copies of v returns (row pair low half, row pair high half)
 // in its two results, so their sum is the pair sum in one fixed order.
+// Every control used here (quad_perm, row_ror, row_half_mirror) reads a valid
+// lane for every lane, so bound_ctrl changes no value; setting it lets the
+// compiler fold the move into the consuming add (v_add_f32_dpp).
 template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 // Reduction operators of the cross-lane helpers.

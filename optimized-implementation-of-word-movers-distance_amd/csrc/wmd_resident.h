@@ -17,13 +17,31 @@ namespace {
 // N values).  After the x reduce-scatter the lanes of a 16-lane row own
 // disjoint slots (the same slots in every wave of the pair), so the sum runs
 // over the B query-word lanes of a row and then over the four rows.
+// Pairs of values take the row steps as a reduce-scatter: one
+// v_permlane16_swap sums the first value over each row pair in the even rows
+// and the second in the odd rows, the 32-lane step then runs once for both,
+// and a swap of two copies hands each lane both totals (the same association
+// as the plain butterfly, so the same bits).
 template <int B, int N>
 __device__ __forceinline__ void slot_sum_n(float (&v)[N]) {
   group_sum_n<B>(v);
+  if constexpr (N % 2 == 0) {
 #pragma unroll
-  for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
+    for (int j = 0; j < N / 2; ++j) {
+      const auto p = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[2 * j]),
+                                                      __float_as_uint(v[2 * j + 1]), false, false);
+      const float s = xor_step<32>(__uint_as_float(p[0]) + __uint_as_float(p[1]));
+      const auto g = __builtin_amdgcn_permlane16_swap(__float_as_uint(s), __float_as_uint(s),
+                                                      false, false);
+      v[2 * j] = __uint_as_float(g[0]);
+      v[2 * j + 1] = __uint_as_float(g[1]);
+    }
+  } else {
 #pragma unroll
-  for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
+    for (int n = 0; n < N; ++n) v[n] = xor_step<16>(v[n]);
+#pragma unroll
+    for (int n = 0; n < N; ++n) v[n] = xor_step<32>(v[n]);
+  }
 }
 
 // State of the u-update rule for the N query slots a lane updates.
@@ -35,6 +53,14 @@ struct UpdateState {
   // up to the scaling gauge) and the previous accepted point and its f.
   float l[N], pl[N], pf[N];
   bool prev;         // pl, pf hold a point
+  // Progress guard of the accelerated rule: the best residual so far, and
+  // its value at the last 64-iteration mark.  A window that does not improve
+  // the best residual switches the pair to plain steps for good (in FP32 the
+  // extrapolation of a slowly converging pair can stall, or cycle through
+  // rejected non-finite points, where plain steps still make steady
+  // progress).
+  float res_best, res_mark;
+  bool plain, check_due;
 };
 
 template <int N>
@@ -42,6 +68,8 @@ __device__ __forceinline__ void update_state_reset(UpdateState<N>& st, float ome
   st.sor = omega > 1.f;
   st.res_prev = INFINITY;
   st.prev = false;
+  st.res_best = st.res_mark = INFINITY;
+  st.plain = st.check_due = false;
 #pragma unroll
   for (int n = 0; n < N; ++n) st.l[n] = st.pl[n] = st.pf[n] = 0.f;
 }
@@ -66,9 +94,9 @@ template <int B, int N>
 __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState<N>& st,
                                                  const float (&x)[N], const float (&r)[N],
                                                  const float (&lr)[N], float (&u)[N], bool seeded,
-                                                 int q, bool& stop) {
+                                                 float inv_q, int it, bool& stop) {
   const bool use_tol = a.tol > 0.f;
-  if (a.accel) {
+  if (a.accel && !st.plain) {
     float g[N];
     float red[2] = {0.f, 0.f};
 #pragma unroll
@@ -83,6 +111,25 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
       stop = true;
       return res;
     }
+    if (!seeded) {
+      // Progress guard, due at the end of every 64-iteration window and
+      // taken at the first finite point from there: the switch takes this
+      // point's plain update (u = 2^g = r / x), never an unverified
+      // extrapolation.
+      st.res_best = fminf(st.res_best, res);
+      if ((it & 63) == 63) st.check_due = true;
+      if (st.check_due && res < INFINITY) {
+        st.check_due = false;
+        if (st.res_best >= st.res_mark) {
+          st.plain = true;
+          st.sor = false;
+#pragma unroll
+          for (int n = 0; n < N; ++n) u[n] = (r[n] > 0.f) ? __builtin_amdgcn_exp2f(g[n]) : 0.f;
+          return res;
+        }
+        st.res_mark = st.res_best;
+      }
+    }
     // NaN compares false: a non-finite residual is rejected as well.
     const bool ok = seeded || !st.prev || (res <= st.res_prev && res < INFINITY);
     if (!ok) {
@@ -90,7 +137,7 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
       for (int n = 0; n < N; ++n) st.l[n] = st.pl[n] + st.pf[n];
       st.prev = false;
     } else {
-      const float mean = wave_uniform(red[1]) / (float)q;
+      const float mean = wave_uniform(red[1]) * inv_q;
       float f[N], dl[N], df[N];
       float d[2] = {0.f, 0.f};
 #pragma unroll
@@ -104,7 +151,7 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
       float gam = 0.f;
       if (st.prev) {
         slot_sum_n<B>(d);
-        gam = wave_uniform(d[1]) / fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f);
+        gam = fdiv(wave_uniform(d[1]), fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f));
       }
 #pragma unroll
       for (int n = 0; n < N; ++n) {
@@ -246,6 +293,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   float res_win = INFINITY;
   UpdateState<NO> st;
   update_state_reset(st, a.omega);
+  const float inv_q = 1.f / (float)q;  // centring of the accelerated update
   int it = 0;
   float res = 0.f;
   float o[NE];
@@ -310,7 +358,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     // The residual is the same on every lane, so the stop test is a scalar
     // branch.
     bool stop = false;
-    const float rv = update_scalings<B>(a, st, xo, ro, lro, uo, seeded, q, stop);
+    const float rv = update_scalings<B>(a, st, xo, ro, lro, uo, seeded, inv_q, it, stop);
     if (!seeded) res = rv;
     float ug[NIP];
     rows_gather<NIP>(uo, ug);
