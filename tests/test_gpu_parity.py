@@ -535,3 +535,34 @@ def test_anderson_acceleration_reaches_the_same_fixed_point(P, dg, oracle):
     assert np.all(resa < 1e-4)
     assert ea <= 2e-4
     assert ita.sum() < its.sum()
+
+
+def test_anderson_progress_guard_on_hard_pairs(P, dg, oracle):
+    # pairs on which the FP32 extrapolation stalled (q ~ n ~ 131, a long
+    # plateau) or cycled through rejected non-finite points: the progress
+    # guard must hand them to plain steps, so they converge within about the
+    # plain iteration count, to the same fixed point
+    say("[case] Anderson progress guard on hard pairs")
+    corpus0, qs0, _ = dg.workload("batched", seed=0)
+    c6, _, _ = dg.workload("batched", seed=6, N=2500, n_queries=1)
+    q6 = _batch_queries(dg, c6.V, [3, 20, 50, 140], seed=10)
+    cases = [(corpus0, qs0, 186, 37005), (corpus0, qs0, 186, 45175), (c6, q6, 2, 1066)]
+    for corpus, qs, Q, j in cases:
+        qw, qv = qs.query(Q)
+        b, e_ = corpus.doc_ptr[j], corpus.doc_ptr[j + 1]
+        dw, dv = corpus.doc_words[b:e_], corpus.doc_vals[b:e_]
+        sub = dg.Corpus(corpus.vecs, np.array([0, e_ - b]), dw, dv)
+        with engine(P, sub) as e:
+            e.set_queries(np.array([0, len(qw)]), qw, qv)
+            e.run(-10.0, max_iter=20000, tol=1e-4)
+            _, itp, _ = e.result()
+            e.run(-10.0, max_iter=20000, tol=1e-4, accel=True)
+            wa, ita, ra = e.result()
+        truth, _, _ = oracle.perdoc(qw, qv, corpus.vecs.astype(np.float64), np.array([0, e_ - b]),
+                                    dw, dv, -10.0, 100000, tol=1e-10)
+        err = rel_err(wa[0], truth)
+        say(f"  Q={Q} j={j} q={len(qw)} n={e_ - b}: plain {itp[0, 0]} iterations, Anderson "
+            f"{ita[0, 0]}; error vs converged FP64 {err:.2e}")
+        assert np.isfinite(wa[0, 0]) and ra[0, 0] < 1e-4
+        assert ita[0, 0] <= itp[0, 0] + 192
+        assert err <= 2e-4
