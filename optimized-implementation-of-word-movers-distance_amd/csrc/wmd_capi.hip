@@ -93,14 +93,31 @@ struct ResidentLaunch {
   int shape, cls;
   size_t off;
   int64_t cnt, d0, d1;
-  double work;  // padded tile entries, the list-scheduling weight
+  double work;  // the list-scheduling weight (estimated lane instructions)
 };
+
+// Per-wave instruction estimate of one iteration of a pair in shape s with
+// NE document-word slots per lane: the two tile products, the B-lane
+// butterfly and quotient per slot row, and the per-slot update.
+double iter_cost(int s, int cls) {
+  const wmd::TileShape& t = wmd::kShapes[s];
+  const int A = 64 * t.W / t.B;
+  const int ne = (wmd::kDocClassWidth * (cls + 1) + A - 1) / A;
+  int lb = 0;
+  while ((1 << lb) < t.B) ++lb;
+  const int no = (t.NI + 3) / 4;
+  return (double)t.W * (ne * (2.0 * t.NI + lb + 2) + 40.0 * no + 40.0);
+}
 
 }  // namespace
 
-// Side streams of a context: the resident launches of a run fan out over
-// them so one launch's tail (its slowest pairs) overlaps the next launch.
-constexpr int kSideStreams = 4;
+// Side streams of a context.  The resident launches of a run fan out over
+// four lanes, the context stream and three side streams: the runtime maps
+// consecutive streams of a process to its (by default four) hardware queues
+// in turn, so the four lanes run concurrently, and one launch's tail (its
+// slowest pairs) overlaps other launches.
+constexpr int kSideStreams = 3;
+constexpr int kLanes = kSideStreams + 1;
 constexpr int kTuneStride = wmd::kMaxQ + 1;
 // Shapes the autotuner times per (document class, query bucket): the default
 // and the next ones in capacity order, up to twice the default's capacity.
@@ -126,6 +143,8 @@ struct wmd_context {
   int plan_forced = -2;
   bool plan_log = false;
   std::vector<ResidentLaunch> plan;
+  std::vector<int32_t> h_plan_flat;  // host copy of plan_bucket
+  bool plan_measured = false;        // plan weights come from a finished run
   hipGraphExec_t solve_exec = nullptr;
   uint64_t solve_plan_id = 0;
   wmd::SolveArgs solve_args{};
@@ -143,6 +162,7 @@ struct wmd_context {
   DevBuf<uint16_t> Qs;                  // BF16 planes of query words
   DevBuf<double> nqw;                   // FP64 squared norms of query words
   std::vector<int64_t> h_class_off;     // kNumDocClasses + 1 offsets into doc_perm
+  std::vector<int32_t> h_doc_perm;      // host copy of doc_perm
   DevBuf<float> doc_val;
 
   // queries
@@ -263,11 +283,15 @@ int shape_for(const wmd_context* c, int cls, int q, bool log_domain, int forced)
   return pick_shape(q, 0, log_domain);
 }
 
+// Until a run has measured them, every pair is weighted with this many
+// iterations.
+constexpr double kPriorIters = 20.0;
+
 void add_launch(std::vector<ResidentLaunch>* plan, std::vector<int32_t>* flat,
                 const std::vector<int32_t>& queries, int s, int cls, int64_t d0, int64_t d1) {
   if (queries.empty() || d1 == d0) return;
   ResidentLaunch l{s, cls, flat->size(), (int64_t)queries.size(), d0, d1, 0.0};
-  l.work = (double)l.cnt * (double)(d1 - d0) * wmd::kShapes[s].cap_q() * (cls + 1);
+  l.work = (double)l.cnt * (double)(d1 - d0) * kPriorIters * iter_cost(s, cls);
   flat->insert(flat->end(), queries.begin(), queries.end());
   plan->push_back(l);
 }
@@ -290,7 +314,28 @@ int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
     for (int s = 0; s < wmd::kNumShapes; ++s) add_launch(plan, &flat, groups[s], s, cls, d0, d1);
   }
   HIP_TRY(c->plan_bucket.upload(flat.data(), flat.size(), c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->h_plan_flat = std::move(flat);
+  c->plan_measured = false;
   return WMD_OK;
+}
+
+// Re-weights the cached plan with the iterations its pairs took in the run
+// that just finished (it: nq x N counts) and bumps the plan id, so the next
+// run (and its captured graph) balances the lanes on measured work.  Done
+// once per plan; the launch order does not change any result.
+void measure_plan(wmd_context* c, const std::vector<int32_t>& it) {
+  if (c->plan_measured || c->plan.empty() || c->h_doc_perm.empty()) return;
+  for (ResidentLaunch& l : c->plan) {
+    double iters = 0.0;
+    for (int64_t k = 0; k < l.cnt; ++k) {
+      const int32_t* row = it.data() + (size_t)c->h_plan_flat[l.off + k] * c->N;
+      for (int64_t d = l.d0; d < l.d1; ++d) iters += row[c->h_doc_perm[d]];
+    }
+    l.work = iters * iter_cost(l.shape, l.cls);
+  }
+  c->plan_measured = true;
+  ++c->plan_id;
 }
 
 // Launch l with the query ids at bucket + l.off.
@@ -304,10 +349,10 @@ hipError_t launch_one(wmd_context* c, const wmd::SolveArgs& base, const Resident
   return wmd::launch_resident(l.shape, l.cls, log_domain, a, s);
 }
 
-// Launches a plan on the side streams: largest first, each to the stream
-// with the least queued work (longest-processing-time list scheduling), so a
-// launch's tail of slow pairs overlaps other launches.  The context stream
-// forks before and joins after.
+// Launches a plan on the four lanes: largest first, each to the lane with
+// the least queued work (longest-processing-time list scheduling), so a
+// launch's tail of slow pairs overlaps other launches.  The side streams
+// fork from the context stream before and join it after.
 int launch_plan(wmd_context* c, const wmd::SolveArgs& base,
                 const std::vector<ResidentLaunch>& plan, bool log_domain, int* launches) {
   std::vector<size_t> order(plan.size());
@@ -316,11 +361,12 @@ int launch_plan(wmd_context* c, const wmd::SolveArgs& base,
                    [&](size_t x, size_t y) { return plan[x].work > plan[y].work; });
   HIP_TRY(hipEventRecord(c->fork, c->stream));
   for (int k = 0; k < kSideStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->side[k], c->fork, 0));
-  double load[kSideStreams] = {};
+  double load[kLanes] = {};
   for (size_t i : order) {
-    const int k = (int)(std::min_element(load, load + kSideStreams) - load);
+    const int k = (int)(std::min_element(load, load + kLanes) - load);
     load[k] += plan[i].work;
-    HIP_TRY(launch_one(c, base, plan[i], c->plan_bucket.p, log_domain, c->side[k]));
+    HIP_TRY(launch_one(c, base, plan[i], c->plan_bucket.p, log_domain,
+                       k == 0 ? c->stream : c->side[k - 1]));
     ++*launches;
   }
   for (int k = 0; k < kSideStreams; ++k) {
@@ -888,6 +934,7 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   HIP_TRY(c->doc_of.upload(doc_of.data(), doc_of.size(), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->h_class_off = class_off;
+  c->h_doc_perm = perm;
   c->h_doc_ptr = dptr;
   c->h_doc_t = dt;
   c->h_touched = touched;
@@ -1285,6 +1332,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     std::vector<float> rs(np);
     HIP_TRY(hipMemcpy(it.data(), c->iters.p, np * sizeof(int32_t), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(rs.data(), c->resid.p, np * sizeof(float), hipMemcpyDeviceToHost));
+    if (variant == WMD_VARIANT_RESIDENT) measure_plan(c, it);
     double sum = 0.0, rmax = 0.0;
     int imax = 0;
     for (size_t k = 0; k < np; ++k) {
