@@ -385,26 +385,33 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     }
   }
 
-  // Final half-step: v = c / (K^T u), then sum u o ((K o M) v) in FP64.
+  // Final half-step: v = c / (K^T u), then the distance
+  //   sum_{e,i} u_i K_ei M_ei o_e = sum_e o_e (s_e mrow_e + T_e / lamda)
+  // with M = mrow + ln(K) / lamda, s_e = sum_i u_i K_ei and
+  // T_e = ln 2 sum_i u_i K_ei log2 K_ei (every term <= 0, so the FP32
+  // partials are well conditioned; K + FLT_MIN keeps the padded entries'
+  // 0 * log2 finite and changes no normal K); the row terms are accumulated
+  // in FP64, s_e mrow_e once per row (by the row's first query-word lane).
+  const double ln2_l = 0.6931471805599453 / (double)a.lamda;
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < NE; ++k) {
-    float s = 0.f;
+    float s = 0.f, t = 0.f;
 #pragma unroll
-    for (int m = 0; m < NI; ++m) s = fmaf(Kt[k][m], u[m], s);
+    for (int m = 0; m < NI; ++m) {
+      const float kv = Kt[k][m];
+      s = fmaf(kv, u[m], s);
+      t = fmaf(u[m] * kv, __builtin_amdgcn_logf(kv + kFltMin), t);
+    }
     s = group_sum<B>(s);
     const int e = ea + A * k;
     if (e < n) {
       const float oe = fdiv(c[k], s);
       if (a.duals) a.duals[(int64_t)Q * a.nnz + db + e] = __logf(oe);
-      const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
-#pragma unroll
-      for (int m = 0; m < NI; ++m) {
-        const float kv = Kt[k][m];
-        if (kv >= kFltMin) {
-          const float Mv = mr + __logf(kv) * a.inv_lamda;
-          acc += (double)u[m] * (double)kv * (double)Mv * (double)oe;
-        }
+      acc = fma((double)oe * (double)t, ln2_l, acc);
+      if (b == 0) {
+        const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+        acc = fma((double)oe * (double)s, (double)mr, acc);
       }
     }
   }
