@@ -202,30 +202,33 @@ def drop_in_rate(P, dg, reps: int = 3):
                         f"15 fixed iterations, FP64 host arrays in/out"}
 
 
-def trace_span(path: str, kernel: str = "k_resident", gap_ms: float = 20.0) -> int:
+def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") -> int:
     """Per-step wall span of one kernel family in a rocprofv3 kernel_trace.csv.
 
     The solver stage's launches run concurrently on several streams, so the
-    sum of their durations exceeds the stage's time; the union of their
-    [start, end] intervals, split into steps at idle gaps longer than gap_ms,
-    is what the HIP events around the stage measure."""
+    sum of their durations exceeds the stage's time; the span from the first
+    launch's start to the last one's end, per step, is what the HIP events
+    around the stage measure.  Steps are delimited by the distance-stage
+    kernels (names containing `stage`) that start every run."""
+    import bisect
     import csv
-    iv = []
+    iv, marks = [], []
     with open(path) as f:
         for row in csv.DictReader(f):
             if kernel in row["Kernel_Name"]:
                 iv.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"])))
+            elif stage in row["Kernel_Name"]:
+                marks.append(int(row["Start_Timestamp"]))
     iv.sort()
-    steps, busy = [], []
+    marks.sort()
+    steps = {}
     for s, e in iv:
-        if steps and s - steps[-1][1] <= gap_ms * 1e6:
-            steps[-1][1] = max(steps[-1][1], e)
-            busy[-1] += e - s
-        else:
-            steps.append([s, e])
-            busy.append(e - s)
-    for k, ((s, e), b) in enumerate(zip(steps, busy)):
-        print(json.dumps({"step": k, "span_ms": (e - s) / 1e6, "sum_kernel_ms": b / 1e6,
+        k = bisect.bisect_right(marks, s)
+        lo, hi, busy = steps.get(k, (s, e, 0))
+        steps[k] = (min(lo, s), max(hi, e), busy + e - s)
+    for n, k in enumerate(sorted(steps)):
+        s, e, b = steps[k]
+        print(json.dumps({"step": n, "span_ms": (e - s) / 1e6, "sum_kernel_ms": b / 1e6,
                           "overlap": b / max(e - s, 1)}))
     return 0
 
@@ -255,6 +258,9 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="smallest-fitting tile shapes instead of autotuned ones")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the solver kernels directly instead of replaying a HIP graph "
+                         "(per-dispatch counter runs)")
     ap.add_argument("--hbm-copy-mib", type=int, default=0,
                     help="only time a device copy of this many MiB (counter calibration)")
     ap.add_argument("--trace-span", metavar="KERNEL_TRACE_CSV",
@@ -300,7 +306,8 @@ def main() -> int:
         log(f"tile shapes autotuned in {tune_ms:.0f} ms ({tune_changed} table entries differ "
             f"from the smallest-fitting shape)")
     kw = dict(max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=args.omega,
-              warm_start=args.warm == "learned", accel=args.accel == "anderson")
+              warm_start=args.warm == "learned", accel=args.accel == "anderson",
+              graph=not args.no_graph)
 
     warm = None
     for w in range(max(1, args.warmup)):

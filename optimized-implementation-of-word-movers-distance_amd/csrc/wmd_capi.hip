@@ -1275,7 +1275,9 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     sh.Vt = (int)c->Vt;
     sh.Sdp = c->Sdp;
     sh.nq = c->nq;
-    sh.rows = std::max(1, std::min(16, wmd::kShareLds / (c->Sdp * (int)sizeof(float))));
+    // Rows per block: as many as fit in 20 KB of LDS (eight blocks per CU),
+    // at least one.
+    sh.rows = std::max(1, std::min(16, 20 * 1024 / (c->Sdp * (int)sizeof(float))));
     sh.lamda = (float)lamda;
     sh.log_out = log_domain ? 1 : 0;
     HIP_TRY(wmd::launch_share_rescale(sh, c->stream));

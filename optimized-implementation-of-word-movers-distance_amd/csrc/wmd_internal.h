@@ -60,6 +60,7 @@ struct MfmaCdistArgs {
   float lamda;
   int log_out;
   int raw;                 // as in CdistArgs
+  int nquery;              // queries of the launch (set by the launcher)
 };
 
 // Shared distance stage of a query batch: the raw distances of the batch's

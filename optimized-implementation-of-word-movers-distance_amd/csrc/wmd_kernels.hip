@@ -218,8 +218,15 @@ __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p, bool valid) {
 // to Kt first and the row-minimum rescale runs after the last chunk, each
 // thread re-reading only what it wrote.
 __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
-  const int Q = blockIdx.y;
-  const int t0 = blockIdx.x * 64;
+  // One-dimensional grid of (word tile, query) blocks, queries fastest and
+  // XCD-aware: XCD x (blocks b with b mod 8 = x) takes a contiguous run of
+  // the linear order, so the query blocks of one word tile run on one XCD
+  // and re-read the tile's BF16 planes from its L2.
+  const int nb = gridDim.x;
+  const int xcd = blockIdx.x & 7;
+  const int lin = xcd * (nb >> 3) + min(xcd, nb & 7) + (blockIdx.x >> 3);
+  const int Q = lin % a.nquery;
+  const int t0 = (lin / a.nquery) * 64;
   const int qb = a.q_ptr[Q];
   const int q = a.q_ptr[Q + 1] - qb;
   const int ks = a.k_stride[Q];
@@ -322,7 +329,13 @@ constexpr int kShareChunks = kShareMaxQ / 64;
 __global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
   extern __shared__ float4 lds4[];
   const float* rows = reinterpret_cast<const float*>(lds4);
-  const int t0 = blockIdx.x * a.rows;
+  // XCD-aware tile order: blocks are dispatched to the 8 XCDs in turn, so
+  // block b takes tile (b mod 8)'s share start + b / 8 -- each XCD writes a
+  // contiguous run of rows of every query's block (whole lines in its L2).
+  const int nb = gridDim.x;
+  const int xcd = blockIdx.x & 7;
+  const int tile = xcd * (nb >> 3) + min(xcd, nb & 7) + (blockIdx.x >> 3);
+  const int t0 = tile * a.rows;
   const int nr = min(a.rows, a.Vt - t0);
   const float4* src = reinterpret_cast<const float4*>(a.Dd + (int64_t)t0 * a.Sdp);
   const int n4 = nr * (a.Sdp >> 2);
@@ -558,7 +571,11 @@ hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s) {
 
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
   if (a.Vt <= 0 || nq <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_cdist_mfma, dim3((a.Vt + 63) / 64, nq), dim3(256), 0, s, a);
+  const int64_t blocks = (int64_t)((a.Vt + 63) / 64) * nq;
+  if (blocks > INT32_MAX) return hipErrorInvalidValue;
+  MfmaCdistArgs b = a;
+  b.nquery = nq;
+  hipLaunchKernelGGL(k_cdist_mfma, dim3((unsigned)blocks), dim3(256), 0, s, b);
   return hipGetLastError();
 }
 
