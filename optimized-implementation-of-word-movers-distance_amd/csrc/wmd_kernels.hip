@@ -166,6 +166,7 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
 // FP64, and pairs whose d^2 falls below kNearTau (|a|^2 + |b|^2) — near-
 // identical words, where the cancellation eats the precision — are
 // recomputed as FP64 direct differences.
+constexpr int kCdSuper = 16;  // word tiles (and queries) per super-tile side
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
@@ -218,15 +219,21 @@ __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p, bool valid) {
 // to Kt first and the row-minimum rescale runs after the last chunk, each
 // thread re-reading only what it wrote.
 __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
-  // One-dimensional grid of (word tile, query) blocks, queries fastest and
+  // One-dimensional grid of (word tile, query) blocks in super-tiles of
+  // kCdSuper word tiles x sup_q queries (sup_q = min(kCdSuper, queries)),
   // XCD-aware: XCD x (blocks b with b mod 8 = x) takes a contiguous run of
-  // the linear order, so the query blocks of one word tile run on one XCD
-  // and re-read the tile's BF16 planes from its L2.
+  // the linear order, so a super-tile's blocks run on one XCD and its word
+  // tiles' and queries' BF16 planes (about 4 MB at most) are fetched once
+  // into that XCD's L2.  Blocks past the edges return.
   const int nb = gridDim.x;
   const int xcd = blockIdx.x & 7;
   const int lin = xcd * (nb >> 3) + min(xcd, nb & 7) + (blockIdx.x >> 3);
-  const int Q = lin % a.nquery;
-  const int t0 = (lin / a.nquery) * 64;
+  const int sq = a.sup_q;
+  const int nsq = (a.nquery + sq - 1) / sq;
+  const int sup = lin / (kCdSuper * sq), in = lin % (kCdSuper * sq);
+  const int Q = (sup % nsq) * sq + in % sq;
+  const int t0 = ((sup / nsq) * kCdSuper + in / sq) * 64;
+  if (Q >= a.nquery || t0 >= a.Vt) return;  // uniform over the block
   const int qb = a.q_ptr[Q];
   const int q = a.q_ptr[Q + 1] - qb;
   const int ks = a.k_stride[Q];
@@ -326,6 +333,46 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
 // epilogue as the per-query kernels (bitwise-equal outputs).
 constexpr int kShareChunks = kShareMaxQ / 64;
 
+// One query's rows of the block; NC = its 64-word chunks (compile-time, so
+// the chunk values stay in registers and short queries do no padded work).
+template <int NC>
+__device__ __forceinline__ void share_query(const ShareArgs& a, const float* rows, int t0, int nr,
+                                            int Q, int lane) {
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  const float l2 = a.lamda * 1.4426950408889634f;
+  const float pad = a.log_out ? kLogZero : 0.f;
+  int sl[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const int j = c * 64 + lane;
+    sl[c] = j < q ? a.slot[qb + j] : -1;
+  }
+  float* Kq = a.K + a.k_off[Q];
+  for (int r = 0; r < nr; ++r) {
+    const int t = t0 + r;
+    float d[NC];
+    float m = INFINITY;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      d[c] = sl[c] >= 0 ? rows[r * a.Sdp + sl[c]] : INFINITY;
+      m = fminf(m, d[c]);
+    }
+    m = bfly_red<1, 32, kMin>(m);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int j = c * 64 + lane;
+      if (j < ks) {
+        float kv = pad;
+        if (j < q) kv = a.log_out ? l2 * (d[c] - m) : expf(a.lamda * (d[c] - m));
+        Kq[(int64_t)t * ks + j] = kv;
+      }
+    }
+    if (lane == 0) a.mrow[(int64_t)Q * a.Vt + t] = m;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
   extern __shared__ float4 lds4[];
   const float* rows = reinterpret_cast<const float*>(lds4);
@@ -343,40 +390,17 @@ __global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const float l2 = a.lamda * 1.4426950408889634f;
-  const float pad = a.log_out ? kLogZero : 0.f;
+  static_assert(kShareChunks == 8, "one case per chunk count");
   for (int Q = w; Q < a.nq; Q += 4) {
-    const int qb = a.q_ptr[Q];
-    const int q = a.q_ptr[Q + 1] - qb;
-    const int ks = a.k_stride[Q];
-    const int nc = (ks + 63) >> 6;
-    int sl[kShareChunks];
-#pragma unroll
-    for (int c = 0; c < kShareChunks; ++c) {
-      const int j = c * 64 + lane;
-      sl[c] = (c < nc && j < q) ? a.slot[qb + j] : -1;
-    }
-    float* Kq = a.K + a.k_off[Q];
-    for (int r = 0; r < nr; ++r) {
-      const int t = t0 + r;
-      float d[kShareChunks];
-      float m = INFINITY;
-#pragma unroll
-      for (int c = 0; c < kShareChunks; ++c) {
-        d[c] = sl[c] >= 0 ? rows[r * a.Sdp + sl[c]] : INFINITY;
-        m = fminf(m, d[c]);
-      }
-      m = bfly_red<1, 32, kMin>(m);
-#pragma unroll
-      for (int c = 0; c < kShareChunks; ++c) {
-        const int j = c * 64 + lane;
-        if (c < nc && j < ks) {
-          float kv = pad;
-          if (j < q) kv = a.log_out ? l2 * (d[c] - m) : expf(a.lamda * (d[c] - m));
-          Kq[(int64_t)t * ks + j] = kv;
-        }
-      }
-      if (lane == 0) a.mrow[(int64_t)Q * a.Vt + t] = m;
+    switch ((a.k_stride[Q] + 63) >> 6) {
+      case 1: share_query<1>(a, rows, t0, nr, Q, lane); break;
+      case 2: share_query<2>(a, rows, t0, nr, Q, lane); break;
+      case 3: share_query<3>(a, rows, t0, nr, Q, lane); break;
+      case 4: share_query<4>(a, rows, t0, nr, Q, lane); break;
+      case 5: share_query<5>(a, rows, t0, nr, Q, lane); break;
+      case 6: share_query<6>(a, rows, t0, nr, Q, lane); break;
+      case 7: share_query<7>(a, rows, t0, nr, Q, lane); break;
+      default: share_query<8>(a, rows, t0, nr, Q, lane); break;
     }
   }
 }
@@ -571,10 +595,14 @@ hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s) {
 
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
   if (a.Vt <= 0 || nq <= 0) return hipSuccess;
-  const int64_t blocks = (int64_t)((a.Vt + 63) / 64) * nq;
+  const int sq = nq < kCdSuper ? nq : kCdSuper;
+  const int64_t nst = ((a.Vt + 63) / 64 + kCdSuper - 1) / kCdSuper;
+  const int64_t nsq = (nq + sq - 1) / sq;
+  const int64_t blocks = nst * nsq * kCdSuper * sq;
   if (blocks > INT32_MAX) return hipErrorInvalidValue;
   MfmaCdistArgs b = a;
   b.nquery = nq;
+  b.sup_q = sq;
   hipLaunchKernelGGL(k_cdist_mfma, dim3((unsigned)blocks), dim3(256), 0, s, b);
   return hipGetLastError();
 }
