@@ -87,6 +87,17 @@ struct DevBuf {
 
 int round_up4(int x) { return (x + 3) & ~3; }
 
+// A temporary device allocation freed when it goes out of scope.
+struct ScopedDev {
+  void* p = nullptr;
+  ScopedDev() = default;
+  ScopedDev(const ScopedDev&) = delete;
+  ScopedDev& operator=(const ScopedDev&) = delete;
+  ~ScopedDev() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 // One resident launch: queries bucket[off, off + cnt) against the documents
 // doc_perm[d0, d1) of one length class, with one tile shape.
 struct ResidentLaunch {
@@ -895,19 +906,32 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
       dt[pos] = compact[w];
       dv[pos] = (float)vals[e];
     }
-  std::vector<float> ef;
-  const float* eptr = vecs_f32;
-  if (!eptr) {
-    ef.resize(V * D);
-    for (uint64_t k = 0; k < V * D; ++k) ef[k] = (float)vecs_f64[k];
-    eptr = ef.data();
+  // The embeddings go to the device as given (FP64 input into a temporary);
+  // the FP32 conversion and the finiteness check run there, so the host does
+  // no pass over V x D.  From here on a failure leaves the context without a
+  // corpus.
+  c->has_corpus = false;
+  c->has_queries = false;
+  ScopedDev e64, flag;
+  HIP_TRY(c->E.ensure(V * D));
+  HIP_TRY(hipMalloc(&flag.p, sizeof(unsigned long long)));
+  if (vecs_f64) {
+    HIP_TRY(hipMalloc(&e64.p, V * D * sizeof(double)));
+    HIP_TRY(hipMemcpyAsync(e64.p, vecs_f64, V * D * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  } else {
+    HIP_TRY(hipMemcpyAsync(c->E.p, vecs_f32, V * D * sizeof(float), hipMemcpyHostToDevice,
+                           c->stream));
   }
-  for (uint64_t k = 0; k < V * D; ++k)
-    if (!std::isfinite(eptr[k])) {
-      set_err("embedding element %llu is not finite", (unsigned long long)k);
-      return WMD_ERR_ARG;
-    }
-  HIP_TRY(c->E.upload(eptr, V * D, c->stream));
+  HIP_TRY(wmd::launch_embed(static_cast<const double*>(e64.p), c->E.p, V * D,
+                            static_cast<unsigned long long*>(flag.p), c->stream));
+  unsigned long long first_bad = 0;
+  HIP_TRY(hipMemcpyAsync(&first_bad, flag.p, sizeof(first_bad), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (first_bad != ~0ull) {
+    set_err("embedding element %llu is not finite", first_bad);
+    return WMD_ERR_ARG;
+  }
   HIP_TRY(c->touched.upload(touched.data(), touched.size(), c->stream));
   HIP_TRY(c->doc_ptr.upload(dptr.data(), dptr.size(), c->stream));
   HIP_TRY(c->doc_t.upload(dt.data(), dt.size(), c->stream));

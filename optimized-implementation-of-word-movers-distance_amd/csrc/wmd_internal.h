@@ -289,5 +289,7 @@ hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
+hipError_t launch_embed(const double* src64, float* dst, uint64_t n, unsigned long long* first_bad,
+                        hipStream_t s);
 
 }  // namespace wmd

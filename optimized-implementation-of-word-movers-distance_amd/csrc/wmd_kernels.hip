@@ -682,6 +682,33 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Embedding upload: the FP32 copy of an FP64 (src64) or FP32 (src32, in
+// place: dst == src32) matrix, and the index of its first element that is
+// not finite in FP32 (*first_bad, which the caller sets to ~0 beforehand).
+__global__ __launch_bounds__(256) void k_embed(const double* src64, float* dst, uint64_t n,
+                                               unsigned long long* first_bad) {
+  for (uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x; k < n;
+       k += (uint64_t)gridDim.x * 256) {
+    const float x = src64 ? (float)src64[k] : dst[k];
+    if (src64) dst[k] = x;
+    if (!isfinite(x)) atomicMin(first_bad, (unsigned long long)k);
+  }
+}
+
+__global__ void k_set_u64(unsigned long long* p, unsigned long long v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+hipError_t launch_embed(const double* src64, float* dst, uint64_t n, unsigned long long* first_bad,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(64), 0, s, first_bad, ~0ull);
+  if (n == 0) return hipGetLastError();
+  const uint64_t want = (n + 255) / 256;
+  const uint64_t blocks = want < 8192 ? want : 8192;
+  hipLaunchKernelGGL(k_embed, dim3((unsigned)blocks), dim3(256), 0, s, src64, dst, n, first_bad);
+  return hipGetLastError();
+}
+
 hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, status);
   const int64_t n = a.n_pairs;
