@@ -13,6 +13,7 @@
 // cores (BF16 inputs, FP32 accumulation; W2 resident in LDS), GELU and the
 // output dot product (k_mlp_infer).
 #include "wmd_internal.h"
+#include "wmd_lanes.h"
 
 namespace wmd {
 namespace {
@@ -34,11 +35,15 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * kGeluC * fmaf(3.f * kGeluA, x * x, 1.f);
 }
 
-// Same function through exp: x * sigmoid(2z) = x / (1 + exp(-2z)); used on
-// the inference path (one v_exp_f32 and one v_rcp_f32).
+// Same function through exp: x * sigmoid(2z) = x / (1 + exp(-2z)) =
+// x / (1 + 2^(x (c0 + c1 x^2))) with c0 = -2 sqrt(2/pi) log2(e) and
+// c1 = c0 * 0.044715; used on the inference path (one v_exp_f32, one
+// v_rcp_f32 and five other instructions).
+constexpr float kGeluE0 = -2.f * 0.7978845608028654f * 1.4426950408889634f;
+constexpr float kGeluE1 = kGeluE0 * 0.044715f;
 __device__ __forceinline__ float gelu_fast(float x) {
-  const float z = kGeluC * fmaf(kGeluA * x * x, x, x);
-  return x * __builtin_amdgcn_rcpf(1.f + __expf(-2.f * z));
+  const float p = fmaf(kGeluE1, x * x, kGeluE0);
+  return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * p));
 }
 
 // ---------------------------------------------------------------------------
@@ -247,7 +252,8 @@ __global__ void k_to_bf16(const float* x, int64_t n, uint16_t* out) {
 // ---------------------------------------------------------------------------
 // Inference over every (query, nonzero) sample s = Q * nnz + pos:
 //   h1 = gelu(At[t] + Bd[j] + Cq[Q]), h2 = gelu(W2 h1 + b2), y = w3 . h2 + b3.
-// Persistent blocks of 4 waves; W2 (BF16, rows padded by 8) lives in LDS.
+// Persistent blocks of kInferWaves waves, one per CU; W2 (BF16, rows padded
+// by 8) lives in LDS, shared by the block's waves.
 // Wave tile = 32 samples: lane l computes h1 for sample l % 32, features
 // 16 s + 8 (l / 32) .. +7 of each 16-wide step (exactly the A-operand
 // fragments of v_mfma_f32_32x32x16_bf16), then 8 column tiles x 16 steps of
@@ -257,23 +263,27 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 constexpr int kW2Pitch = H + 8;
 
-__global__ __launch_bounds__(256) void k_mlp_infer(MlpInferArgs a) {
+constexpr int kInferWaves = 12;  // waves per block, all sharing the block's LDS copy of W2
+
+__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(MlpInferArgs a) {
   extern __shared__ uint16_t w2s[];  // H x kW2Pitch
-  for (int i = threadIdx.x; i < H * H; i += 256) w2s[(i / H) * kW2Pitch + (i % H)] = a.W2b[i];
+  for (int i = threadIdx.x; i < H * H; i += 64 * kInferWaves)
+    w2s[(i / H) * kW2Pitch + (i % H)] = a.W2b[i];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int64_t total = (int64_t)a.nq * a.nnz;
-  const int64_t ntiles = (total + 31) / 32;
+  // Tile = 32 consecutive nonzeros x one query, queries fastest: the waves
+  // of a block work on the same 32 nonzeros for different queries, so their
+  // At / Bd rows come from the CU's cache.
+  const int64_t npb = (a.nnz + 31) / 32;
+  const int64_t ntiles = npb * a.nq;
   const int half = lane >> 5;
-  for (int64_t tile = (int64_t)blockIdx.x * 4 + wv; tile < ntiles; tile += (int64_t)gridDim.x * 4) {
-    const int64_t s = tile * 32 + (lane & 31);
-    const bool valid = s < total;
-    int Q = 0, pos = 0;
-    if (valid) {
-      Q = (int)(s / a.nnz);
-      pos = (int)(s % a.nnz);
-    }
+  for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wv; tile < ntiles;
+       tile += (int64_t)gridDim.x * kInferWaves) {
+    const int Q = (int)(tile % a.nq);
+    const int64_t pos0 = (tile / a.nq) * 32;
+    const int64_t pl = pos0 + (lane & 31);
+    const int pos = (int)(pl < a.nnz ? pl : a.nnz - 1);  // tail lanes: a valid row, not stored
     const float* at = a.At + (int64_t)a.doc_t[pos] * H;
     const float* bd = a.Bd + (int64_t)a.doc_of[pos] * H;
     const float* cq = a.Cq + (int64_t)Q * H;
@@ -281,10 +291,10 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpInferArgs a) {
 #pragma unroll
     for (int st = 0; st < H / 16; ++st) {
       const int k0 = 16 * st + 8 * half;
-      u16x8 v;
+      // (__bf16) rounds to nearest even; pairs compile to v_cvt_pk_bf16_f32.
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = to_bf16(gelu_fast(at[k0 + e] + bd[k0 + e] + cq[k0 + e]));
-      h1[st] = __builtin_bit_cast(bf16x8, v);
+      for (int e = 0; e < 8; ++e)
+        h1[st][e] = (__bf16)gelu_fast(at[k0 + e] + bd[k0 + e] + cq[k0 + e]);
     }
     float yp[16] = {};
 #pragma unroll 1
@@ -301,19 +311,16 @@ __global__ __launch_bounds__(256) void k_mlp_infer(MlpInferArgs a) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) yp[r] = fmaf(w3, gelu_fast(acc[r] + b2), yp[r]);
     }
-    // Column sums: the 32 lanes of each half hold the 32 columns of a row.
+    // Column sums: the 32 lanes of each half hold the 32 columns of a row
+    // (butterflies over xor 1..16 stay inside the half).
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      float v = yp[r];
-#pragma unroll
-      for (int m = 1; m < 32; m <<= 1) v += __shfl_xor(v, m, 64);
-      yp[r] = v;
-    }
+    for (int r = 0; r < 16; ++r) yp[r] = bfly_sum<1, 16>(yp[r]);
     if ((lane & 31) == 0) {
+      float* out = a.pred + (int64_t)Q * a.nnz;
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t so = tile * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (so < total) a.pred[so] = yp[r] + a.b3;
+        const int64_t po = pos0 + 8 * (r >> 2) + 4 * half + (r & 3);
+        if (po < a.nnz) out[po] = yp[r] + a.b3;
       }
     }
   }
@@ -413,14 +420,14 @@ hipError_t launch_mlp_infer(const MlpInferArgs& a, int n_cu, hipStream_t s) {
   const int64_t total = (int64_t)a.nq * a.nnz;
   if (total <= 0) return hipSuccess;
   const int64_t tiles = (total + 31) / 32;
-  int64_t blocks = (tiles + 3) / 4;
-  const int64_t cap = (int64_t)n_cu * 2;
+  int64_t blocks = (tiles + kInferWaves - 1) / kInferWaves;
+  const int64_t cap = (int64_t)n_cu;  // one block per CU: W2 fills most of the LDS
   if (blocks > cap) blocks = cap;
   const size_t lds = mlp_infer_lds_bytes();
   hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_infer),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(64 * kInferWaves), lds, s, a);
   return hipGetLastError();
 }
 
