@@ -422,19 +422,28 @@ int wmd_prepare_corpus(const char* vec_path, const char* text_path, int text_for
     return WMD_ERR_ARG;
   }
   *out = nullptr;
-  std::string vtext, dtext;
-  if (!slurp(vec_path, &vtext) || !slurp(text_path, &dtext)) return WMD_ERR_ARG;
+  // The embeddings file is read line by line, only as far as the n_toks
+  // lines used (a fastText file holds millions more); the documents whole.
+  std::string dtext;
+  if (!slurp(text_path, &dtext)) return WMD_ERR_ARG;
+  FILE* vf = vec_path ? std::fopen(vec_path, "rb") : nullptr;
+  if (!vf) {
+    set_error("could not open %s: %s", vec_path ? vec_path : "(null)", std::strerror(errno));
+    return WMD_ERR_ARG;
+  }
+  char* buf = nullptr;
+  size_t cap = 0;
   auto* p = new wmd_prepared();
   auto fail = [&](void) {
     delete p;
+    std::free(buf);
+    if (vf) std::fclose(vf);
     return WMD_ERR_ARG;
   };
   // --- embeddings (prep.py:17-29)
-  const char* c = vtext.data();
-  const char* e = c + vtext.size();
-  const char* le = line_end(c, e);
+  ssize_t len = getline(&buf, &cap, vf);
   unsigned long long count = 0, dim = 0;
-  if (std::sscanf(std::string(c, le).c_str(), "%llu %llu", &count, &dim) != 2 || dim == 0) {
+  if (len < 0 || std::sscanf(buf, "%llu %llu", &count, &dim) != 2 || dim == 0) {
     set_error("%s: header 'count dim' missing", vec_path);
     return fail();
   }
@@ -442,15 +451,15 @@ int wmd_prepare_corpus(const char* vec_path, const char* text_path, int text_for
   std::unordered_map<std::string, uint64_t> lookup;
   std::vector<double> row;
   for (uint64_t v = 0; v < n_toks; ++v) {
-    if (le == e) {
+    len = getline(&buf, &cap, vf);
+    if (len < 0) {
       set_error("%s: %llu embeddings, %llu requested", vec_path, (unsigned long long)v,
                 (unsigned long long)n_toks);
       return fail();
     }
-    c = le + 1;
-    le = line_end(c, e);
-    const char* stop = le;
-    while (stop > c && py_space(stop[-1])) --stop;
+    const char* c = buf;
+    const char* stop = buf + len;
+    while (stop > c && py_space(stop[-1])) --stop;  // rstrip (the newline included)
     std::string token;
     row.clear();
     bool first = true, bad = false;
@@ -477,10 +486,15 @@ int wmd_prepare_corpus(const char* vec_path, const char* text_path, int text_for
     if (v + 1 < n_toks) p->tokens_text.push_back('\n');
   }
   p->V = n_toks;
+  std::free(buf);
+  buf = nullptr;
+  std::fclose(vf);
+  vf = nullptr;
   // --- documents (prep.py:32-50, 88-91)
   std::vector<std::vector<uint64_t>> docs;
-  c = dtext.data();
-  e = c + dtext.size();
+  const char* c = dtext.data();
+  const char* e = c + dtext.size();
+  const char* le = nullptr;
   uint64_t record = 0;
   size_t nfields = 0;
   std::vector<uint64_t> ids;
