@@ -468,6 +468,43 @@ def test_invalid_arguments(P, dg):
         assert status(lambda: e.run(-1.0, shape=no_lin, max_iter=3)) == nat.WMD_ERR_UNSUPPORTED
 
 
+def test_non_finite_embeddings_are_rejected_on_the_device(P, dg):
+    # the embeddings are converted to FP32 and checked on the device: a NaN,
+    # an infinity or an FP64 value beyond FP32's range is reported with its
+    # element index, and a failed upload leaves the context without a corpus
+    say("[case] non-finite embeddings")
+    corpus, qs, p = dg.workload("tiny", seed=0)
+    nat = P._native
+    r = qs.dense_r(0, corpus.V)
+    csr = corpus.word_major()
+    D = corpus.vecs.shape[1]
+    for value, (row, col) in ((np.nan, (5, 7)), (np.inf, (0, 0)), (1e39, (corpus.V - 1, D - 1))):
+        vecs = corpus.vecs.astype(np.float64)
+        vecs[row, col] = value
+        with pytest.raises(P.WMDError) as ei:
+            P.sinkhorn_wmd_fused(r, vecs, csr, lamda=-1.0)
+        assert ei.value.status == nat.WMD_ERR_ARG
+        assert f"element {row * D + col} is not finite" in str(ei.value)
+    bad32 = corpus.vecs.astype(np.float32).copy()
+    bad32[3, 2] = np.inf
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, csr)
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        with pytest.raises(P.WMDError) as ei:
+            e.set_corpus(bad32, csr)
+        assert ei.value.status == nat.WMD_ERR_ARG and f"element {3 * D + 2}" in str(ei.value)
+        with pytest.raises(P.WMDError) as ei:
+            e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        assert ei.value.status == nat.WMD_ERR_STATE
+        e.set_corpus(corpus.vecs, csr)  # a valid corpus makes the context usable again
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=15)
+        w = e.result()[0]
+    ref = P.sinkhorn_wmd_fused(qs.dense_r(0, corpus.V), corpus.vecs.astype(np.float64), csr,
+                               lamda=-10.0)
+    assert rel_err(w[0], ref) < 1e-6
+
+
 def test_batched_full_size_properties(P, dg, oracle):
     # the batched workload at its full size: 256 queries x 50,000 documents
     say("[case] batched full size, residual 1e-4")
