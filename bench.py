@@ -202,6 +202,40 @@ def drop_in_rate(P, dg, reps: int = 3):
                         f"15 fixed iterations, FP64 host arrays in/out"}
 
 
+def plain_shape(P, dg, reps: int = 5):
+    """The reference's own shape (one 19-word query x 5,000 documents,
+    V = 100,000, 15 fixed iterations) with the corpus resident: device time of
+    the distance stage and of the solve per variant, and the sparse stage's
+    algorithmic bytes per iteration B = 4(V+1) + 8 nnz + 4 Vt q + 8 N q (CSR
+    once, kernel rows once, u read and x written once) over the solve time,
+    against the 8 TB/s HBM peak.  The working set (about 10 MB) sits in the
+    256 MB MALL and 5,000 pairs fill only part of the chip, so this shape is
+    latency-bound."""
+    corpus, qs, prm = dg.workload("plain", seed=0)
+    q = int(qs.lengths()[0])
+    vt = int(np.unique(corpus.doc_words).shape[0])
+    b_iter = 4.0 * (corpus.V + 1) + 8.0 * corpus.nnz + 4.0 * vt * q + 8.0 * corpus.N * q
+    out = {"workload": f"plain: 1 query (q={q}) x {corpus.N} docs, V={corpus.V}, nnz {corpus.nnz}, "
+                       f"15 fixed iterations, corpus resident",
+           "alg_bytes_per_iter": b_iter, "hbm_peak_gbs": 8000.0}
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        for name, variant in (("resident", P.VARIANT_RESIDENT), ("fused", P.VARIANT_FUSED),
+                              ("nonfused", P.VARIANT_NONFUSED)):
+            e.run(prm["lamda"], max_iter=15, variant=variant, summarize=False)  # warm
+            cd = sv = 0.0
+            for _ in range(reps):
+                st = e.run(prm["lamda"], max_iter=15, variant=variant, summarize=False)
+                cd += st["ms_cdist"]
+                sv += st["ms_solve"]
+            cd, sv = cd / reps, sv / reps
+            gbs = 15 * b_iter / (sv * 1e-3) / 1e9
+            out[name] = {"ms_cdist": cd, "ms_solve": sv, "pairs_per_s": corpus.N / ((cd + sv) * 1e-3),
+                         "alg_gbs": gbs, "hbm_frac": gbs / 8000.0}
+    return out
+
+
 def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") -> int:
     """Per-step wall span of one kernel family in a rocprofv3 kernel_trace.csv.
 
@@ -402,11 +436,15 @@ def main() -> int:
         log(f"plain GPU method: {plain['value']:.4g} pairs/s on {nq} queries")
     eng.close()
     traffic = measured_traffic()
-    drop_in = None
+    drop_in = plain_cfg = None
     if not args.no_plain:
         drop_in = drop_in_rate(P, dg)
         log(f"drop-in entry (host buffers, PCIe-inclusive): {drop_in['value']:.4g} pairs/s, "
             f"{drop_in['ms_per_call']:.1f} ms per call")
+        plain_cfg = plain_shape(P, dg)
+        log("reference shape, corpus resident: " + ", ".join(
+            f"{k} {plain_cfg[k]['ms_solve']:.3f} ms solve ({plain_cfg[k]['alg_gbs']:.0f} GB/s alg.)"
+            for k in ("resident", "fused", "nonfused")))
 
     cpu = cpu_port = None
     if not args.no_cpu_baseline:
@@ -464,6 +502,7 @@ def main() -> int:
         "cpu_port_per_pair_stop": cpu_port,
         "plain_gpu": plain,
         "drop_in": drop_in,
+        "reference_shape": plain_cfg,
     }
     print(json.dumps(line), flush=True)
     return 0
