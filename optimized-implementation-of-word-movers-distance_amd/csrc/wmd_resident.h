@@ -284,13 +284,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 
   // Learned warm start: the first pass takes v from the predicted log v
   // (one half-step, counted as an iteration) instead of from u = q.  A
-  // warm-started pair whose residual stalls (no decrease over a 64-iteration
-  // window) restarts once from the uniform start.  Slow but steady progress
-  // is not a stall: such pairs are slow from any start.
+  // warm-started pair whose best residual stalls (no decrease over a
+  // 64-iteration window; the best, because the accelerated rule's residual
+  // is not monotone) restarts once from the uniform start.  Slow but steady
+  // progress is not a stall: such pairs are slow from any start.
   const bool warm_on = a.warm != nullptr;
   bool seeded = warm_on;
   bool fell_back = !warm_on;
-  float res_win = INFINITY;
+  float res_win = INFINITY, res_low = INFINITY;
   UpdateState<NO> st;
   update_state_reset(st, a.omega);
   const float inv_q = 1.f / (float)q;  // centring of the accelerated update
@@ -370,8 +371,9 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       continue;
     }
     if (stop) break;
+    res_low = fminf(res_low, res);
     if (!fell_back && (it & 63) == 0) {
-      if (res >= res_win) {
+      if (res_low >= res_win) {
 #pragma unroll
         for (int m = 0; m < NI; ++m) u[m] = (b + B * m < q) ? (float)q : 0.f;
 #pragma unroll
@@ -380,7 +382,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
         fell_back = true;
         if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
       } else {
-        res_win = res;
+        res_win = res_low;
       }
     }
   }
