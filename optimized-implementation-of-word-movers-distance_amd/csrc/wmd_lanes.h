@@ -204,6 +204,41 @@ __device__ __forceinline__ void rows_gather(const float (&own)[N / 4], float (&v
 // Fast reciprocal quotient a / b (v_rcp_f32, 1 ulp, then one rounding).
 __device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
+// Wave-wide FP64 sum over the same moves as group_red<64>, applied to both
+// 32-bit halves: no LDS permutes, and every lane ends with the same bits
+// (each step adds the lower lane's or row's copy first on both partners).
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov_d(double v) {
+  const uint64_t u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+  v += dpp_mov_d<0xB1>(v);   // xor 1
+  v += dpp_mov_d<0x4E>(v);   // xor 2
+  v += dpp_mov_d<0x141>(v);  // row_half_mirror: quads uniform, pairs the two quads
+  v += dpp_mov_d<0x128>(v);  // row_ror:8 == xor 8
+  {
+    const uint64_t u = __double_as_longlong(v);
+    const auto l = __builtin_amdgcn_permlane16_swap((uint32_t)u, (uint32_t)u, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32),
+                                                    false, false);
+    v = __longlong_as_double((int64_t)(((uint64_t)h[0] << 32) | l[0])) +
+        __longlong_as_double((int64_t)(((uint64_t)h[1] << 32) | l[1]));
+  }
+  {
+    const uint64_t u = __double_as_longlong(v);
+    const auto l = __builtin_amdgcn_permlane32_swap((uint32_t)u, (uint32_t)u, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap((uint32_t)(u >> 32), (uint32_t)(u >> 32),
+                                                    false, false);
+    v = __longlong_as_double((int64_t)(((uint64_t)h[0] << 32) | l[0])) +
+        __longlong_as_double((int64_t)(((uint64_t)h[1] << 32) | l[1]));
+  }
+  return v;
+}
+
 template <int LO, int HI>
 __device__ __forceinline__ double bfly_sum_d(double v) {
   if constexpr (LO <= HI) {

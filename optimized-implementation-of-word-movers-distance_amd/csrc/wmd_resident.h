@@ -193,10 +193,25 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
 }
 
 // ---------------------------------------------------------------------------
+// Query-word slot of tile column m of query-word lane b (B lanes, NI columns).
+// The columns come in runs that one vector load fills: the first NI4 in runs
+// of four consecutive slots (16-byte loads), then at most one run of two,
+// then single slots, so slot_of is a bijection of [0, B) x [0, NI) onto
+// [0, B NI).  Kernel rows are 16-byte aligned with a stride of q rounded up
+// to four, so a run that starts below q stays inside its row.
+template <int B, int NI>
+__device__ __forceinline__ int slot_of(int b, int m) {
+  constexpr int NI4 = NI & ~3;
+  constexpr int R2 = (NI - NI4) & 2;
+  if (m < NI4) return 4 * (b + B * (m >> 2)) + (m & 3);
+  if (m < NI4 + R2) return B * NI4 + 2 * b + (m - NI4);
+  return B * (NI4 + R2) + b + B * (m - NI4 - R2);
+}
+
 // Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
 // query-word lanes (B <= 16, so a query-word group never leaves a 16-lane
 // row).  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
-// i = b + B*m (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
+// i = slot_of(b, m) (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
 //   s_e = sum_i K[e,i] u_i      (partial over own i, DPP butterfly over the
 //                                B lanes of the row)
 //   o_e = c_e / s_e
@@ -247,6 +262,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 
   float Kt[NE][NI];
   float c[NE];
+  // The row-minimum part of the distance, sum_e o_e s_e mrow_e, is
+  // sum_e c_e mrow_e once the final half-step has set o = c / s: gathered with
+  // the tile, by the first query-word lane of each row, in FP64.
+  double cm = 0.0;
 #pragma unroll
   for (int k = 0; k < NE; ++k) {
     const int e = ea + A * k;
@@ -256,18 +275,39 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     if (ve) {
       t = a.doc_t[db + e];
       cv = a.doc_val[db + e];
+      if (b == 0) cm = fma((double)cv, (double)a.mrow[(int64_t)Q * a.Vt + t], cm);
     }
     c[k] = cv;
     const float* row = Kq + (int64_t)t * ks;
+    constexpr int NI4 = NI & ~3;
+    constexpr int R2 = (NI - NI4) & 2;
 #pragma unroll
-    for (int m = 0; m < NI; ++m) {
-      const int i = b + B * m;
+    for (int m = 0; m < NI4; m += 4) {
+      const int i = slot_of<B, NI>(b, m);
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (ve && i < q) v = *reinterpret_cast<const float4*>(row + i);
+      Kt[k][m] = v.x;
+      Kt[k][m + 1] = (i + 1 < q) ? v.y : 0.f;
+      Kt[k][m + 2] = (i + 2 < q) ? v.z : 0.f;
+      Kt[k][m + 3] = (i + 3 < q) ? v.w : 0.f;
+    }
+    if constexpr (R2 == 2) {
+      const int i = slot_of<B, NI>(b, NI4);
+      float2 v = make_float2(0.f, 0.f);
+      if (ve && i < q) v = *reinterpret_cast<const float2*>(row + i);
+      Kt[k][NI4] = v.x;
+      Kt[k][NI4 + 1] = (i + 1 < q) ? v.y : 0.f;
+    }
+#pragma unroll
+    for (int m = NI4 + R2; m < NI; ++m) {
+      const int i = slot_of<B, NI>(b, m);
       Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
     }
   }
   float u[NI];
 #pragma unroll
-  for (int m = 0; m < NI; ++m) u[m] = (b + B * m < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
+  for (int m = 0; m < NI; ++m)
+    u[m] = (slot_of<B, NI>(b, m) < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
   // The lane updates only the NO scalings it owns and gathers the rest; uo,
   // ro, lro are its owned u, r, log2 r (padded slots: r = 0, log2 r = -inf,
   // which the update rules map to u = 0).
@@ -275,7 +315,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 #pragma unroll
   for (int k = 0; k < NO; ++k) {
     const int m = own_slot<2>(k, rho);
-    const int i = b + B * m;
+    const int i = slot_of<B, NI>(b, m);
     const bool live = m < NI && i < q;
     ro[k] = live ? a.q_r[qb + i] : 0.f;
     lro[k] = __builtin_amdgcn_logf(ro[k]);
@@ -375,7 +415,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     if (!fell_back && (it & 63) == 0) {
       if (res_low >= res_win) {
 #pragma unroll
-        for (int m = 0; m < NI; ++m) u[m] = (b + B * m < q) ? (float)q : 0.f;
+        for (int m = 0; m < NI; ++m) u[m] = (slot_of<B, NI>(b, m) < q) ? (float)q : 0.f;
 #pragma unroll
         for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
         update_state_reset(st, a.omega);
@@ -388,12 +428,11 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   }
 
   // Final half-step: v = c / (K^T u), then the distance
-  //   sum_{e,i} u_i K_ei M_ei o_e = sum_e o_e (s_e mrow_e + T_e / lamda)
-  // with M = mrow + ln(K) / lamda, s_e = sum_i u_i K_ei and
-  // T_e = ln 2 sum_i u_i K_ei log2 K_ei (every term <= 0, so the FP32
-  // partials are well conditioned; K + FLT_MIN keeps the padded entries'
-  // 0 * log2 finite and changes no normal K); the row terms are accumulated
-  // in FP64, s_e mrow_e once per row (by the row's first query-word lane).
+  //   sum_{e,i} u_i K_ei M_ei o_e = sum_e c_e mrow_e + (ln 2 / lamda) sum_e o_e T_e
+  // with M = mrow + ln(K) / lamda, o_e = c_e / s_e, s_e = sum_i u_i K_ei and
+  // T_e = sum_i u_i K_ei log2 K_ei (every term <= 0, so the FP32 partials are
+  // well conditioned; K + FLT_MIN keeps the padded entries' 0 * log2 finite
+  // and changes no normal K).  The products o_e T_e are summed in FP64.
   const double ln2_l = 0.6931471805599453 / (double)a.lamda;
   double acc = 0.0;
 #pragma unroll
@@ -410,14 +449,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     if (e < n) {
       const float oe = fdiv(c[k], s);
       if (a.duals) a.duals[(int64_t)Q * a.nnz + db + e] = __logf(oe);
-      acc = fma((double)oe * (double)t, ln2_l, acc);
-      if (b == 0) {
-        const float mr = a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
-        acc = fma((double)oe * (double)s, (double)mr, acc);
-      }
+      acc += (double)(oe * t);
     }
   }
-  acc = bfly_sum_d<1, 32>(acc);
+  acc = wave_sum_d(fma(acc, ln2_l, cm));
   if constexpr (W > 1) {
     if (lane == 0) dsh[wv] = acc;
     __syncthreads();
