@@ -44,6 +44,72 @@ __device__ __forceinline__ void slot_sum_n(float (&v)[N]) {
   }
 }
 
+// Quotients o_k = c_k / s_k of a 16-lane row from the row's per-lane
+// partials s_k (k < N), without every lane summing and dividing every k:
+// two reduce-scatter steps (partner lane^8 by DPP row_ror:8, then the mirror
+// lane of the 8-lane half by row_half_mirror, each lane keeping the half of
+// the values its lane bit selects) leave quad g = (lane >> 2) & 3 with the
+// partial sums of values g*P/4 .. g*P/4 + P/4 - 1 (P = N rounded up to four,
+// the padding summed as 0) over four lanes that together cover the row; two
+// quad butterflies finish the sums, the quad divides its P/4 values (cq holds
+// the matching c_k), and DPP row_newbcast hands each quotient from the first
+// lane of its quad to the whole row, so every lane of the row holds the same
+// bits of every o_k.
+template <int N>
+__device__ __forceinline__ void row_quotients(const float (&s)[N], const float (&cq)[(N + 3) / 4],
+                                              float (&o)[N], int lane) {
+  constexpr int P = (N + 3) / 4 * 4;
+  constexpr int H = P / 2, Q = P / 4;
+  const bool up8 = (lane & 8) != 0;
+  const bool up4 = (lane & 4) != 0;
+  float h[H];
+#pragma unroll
+  for (int j = 0; j < H; ++j) {
+    const float lo = (j < N) ? s[j] : 0.f;
+    const float hi = (j + H < N) ? s[j + H] : 0.f;
+    const float keep = up8 ? hi : lo;
+    const float send = up8 ? lo : hi;
+    h[j] = keep + dpp_mov<0x128>(send);  // row_ror:8 == lane ^ 8
+  }
+  float w[Q];
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const float keep = up4 ? h[j + Q] : h[j];
+    const float send = up4 ? h[j] : h[j + Q];
+    w[j] = keep + dpp_mov<0x141>(send);  // row_half_mirror: lane i <-> 7 - i
+  }
+#pragma unroll
+  for (int j = 0; j < Q; ++j) w[j] = xor_step<2>(xor_step<1>(w[j]));
+#pragma unroll
+  for (int j = 0; j < Q; ++j) w[j] = fdiv(cq[j], w[j]);
+  // row_newbcast:n (DPP 0x150 + n): every lane of a 16-lane row reads lane n.
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    switch (k / Q) {
+      case 0: o[k] = dpp_mov<0x150>(w[k % Q]); break;
+      case 1: o[k] = dpp_mov<0x154>(w[k % Q]); break;
+      case 2: o[k] = dpp_mov<0x158>(w[k % Q]); break;
+      default: o[k] = dpp_mov<0x15C>(w[k % Q]); break;
+    }
+  }
+}
+
+// The c_k that quad (lane >> 2) & 3 divides by in row_quotients.
+template <int N>
+__device__ __forceinline__ void row_quotient_c(const float (&c)[N], float (&cq)[(N + 3) / 4],
+                                               int lane) {
+  constexpr int Q = (N + 3) / 4;
+  const int g = (lane >> 2) & 3;
+#pragma unroll
+  for (int j = 0; j < Q; ++j) {
+    const float c0 = c[j];
+    const float c1 = (Q + j < N) ? c[Q + j] : 0.f;
+    const float c2 = (2 * Q + j < N) ? c[2 * Q + j] : 0.f;
+    const float c3 = (3 * Q + j < N) ? c[3 * Q + j] : 0.f;
+    cq[j] = (g == 0) ? c0 : (g == 1) ? c1 : (g == 2) ? c2 : c3;
+  }
+}
+
 // State of the u-update rule for the N query slots a lane updates.
 template <int N>
 struct UpdateState {
@@ -304,6 +370,11 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
     }
   }
+  // Rows that are one query-word group divide each o_e once per quad
+  // (row_quotients) instead of in every lane.
+  constexpr bool kRowQuot = B == 16 && NE >= 3;
+  float cq[(NE + 3) / 4];
+  if constexpr (kRowQuot) row_quotient_c<NE>(c, cq, lane);
   float u[NI];
 #pragma unroll
   for (int m = 0; m < NI; ++m)
@@ -360,9 +431,13 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
         for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
         s[k] = acc;
       }
-      group_sum_n<B>(s);
+      if constexpr (kRowQuot) {
+        row_quotients<NE>(s, cq, o, lane);
+      } else {
+        group_sum_n<B>(s);
 #pragma unroll
-      for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
+        for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
+      }
     }
     float xs[NIP];
 #pragma unroll
