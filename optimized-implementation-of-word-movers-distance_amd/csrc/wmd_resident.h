@@ -110,15 +110,32 @@ __device__ __forceinline__ void row_quotient_c(const float (&c)[N], float (&cq)[
   }
 }
 
-// State of the u-update rule for the N query slots a lane updates.
+// Select between two values under a wave-uniform condition as a bitwise
+// blend (v_bfi_b32).  The empty asm hides
+// the mask's two possible values from the optimiser, which would otherwise
+// turn each select into a branch and make the loop carry register copies
+// between the two paths.
+__device__ __forceinline__ uint32_t opaque_mask(bool cond) {
+  uint32_t m = cond ? 0xFFFFFFFFu : 0u;
+  asm volatile("" : "+v"(m));
+  return m;
+}
+
+__device__ __forceinline__ float mask_select(uint32_t m, float a, float b) {
+  return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
+}
+
+// State of the u-update rules for the N query slots a lane updates.
 template <int N>
 struct UpdateState {
   bool sor;          // over-relaxed steps still on
   float res_prev;    // residual of the previous (accepted) step
   // Anderson acceleration, history 1: centred base-2 potentials l (u = 2^l
-  // up to the scaling gauge) and the previous accepted point and its f.
-  float l[N], pl[N], pf[N];
-  bool prev;         // pl, pf hold a point
+  // up to the scaling gauge), the step dl that led to l from the previous
+  // accepted point (which is l - dl), and that point's f.  Every field is
+  // rewritten in place each step, so the loop carries no register copies.
+  float l[N], dl[N], pf[N];
+  bool prev;         // dl, pf describe a point
   // Progress guard of the accelerated rule: the best residual so far, and
   // its value at the last 64-iteration mark.  A window that does not improve
   // the best residual switches the pair to plain steps for good (in FP32 the
@@ -126,7 +143,7 @@ struct UpdateState {
   // rejected non-finite points, where plain steps still make steady
   // progress).
   float res_best, res_mark;
-  bool plain, check_due;
+  bool check_due;
 };
 
 template <int N>
@@ -135,103 +152,105 @@ __device__ __forceinline__ void update_state_reset(UpdateState<N>& st, float ome
   st.res_prev = INFINITY;
   st.prev = false;
   st.res_best = st.res_mark = INFINITY;
-  st.plain = st.check_due = false;
+  st.check_due = false;
 #pragma unroll
-  for (int n = 0; n < N; ++n) st.l[n] = st.pl[n] = st.pf[n] = 0.f;
+  for (int n = 0; n < N; ++n) st.l[n] = st.dl[n] = st.pf[n] = 0.f;
 }
 
-// One u-update of the N slots this lane updates, from their x = K o, r and
-// log2 r; returns the row-marginal residual sum_i |u_i x_i - r_i| of the
-// current plan (every lane the same value).  stop is set when the pair meets
-// the tolerance: the plain and over-relaxed rules still take their step
-// (the iterate the reference would hold next), the accelerated rule keeps the
-// verified point.
+// One accelerated u-update of the N slots this lane updates, from their
+// x = K o, r and log2 r; returns the row-marginal residual
+// sum_i |u_i x_i - r_i| of the current plan (every lane the same value).
+//   g = centre(log2 r - log2 x), f = g - l,
+//   l <- l + f - gamma (dl + df), gamma = (df . f) / (df . df),
+//   dl, df the changes since the previous accepted step; a step whose
+//   residual rises (or is not finite) is replaced by the previous point's
+//   plain update, and the history restarts.  Centring pins the gauge (the
+//   plan ignores a constant added to every log2 u_i), so the extrapolation
+//   cannot drift along it.
+// stop: the pair meets the tolerance, and u keeps the verified point (its
+// value before this update).  to_plain: the progress guard fired; u is this
+// point's plain update (2^(log2 r - log2 x) = r / x) and the pair continues
+// with plain steps.  Both outcomes of the acceptance test are formed and
+// blended, so the state never differs between paths.
+template <int B, int N>
+__device__ __forceinline__ float accel_update(const SolveArgs& a, UpdateState<N>& st,
+                                              const float (&x)[N], const float (&r)[N],
+                                              const float (&lr)[N], float (&u)[N], bool seeded,
+                                              float inv_q, int it, bool& stop, bool& to_plain) {
+  float g[N];
+  float red[2] = {0.f, 0.f};
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    red[0] += fabsf(u[n] * x[n] - r[n]);
+    g[n] = (r[n] > 0.f) ? lr[n] - __builtin_amdgcn_logf(x[n]) : 0.f;
+    red[1] += g[n];
+  }
+  slot_sum_n<B>(red);
+  const float res = wave_uniform(red[0]);
+  stop = !seeded && a.tol > 0.f && res < a.tol;
+  // Progress guard, due at the end of every 64-iteration window and taken at
+  // the first finite point from there.
+  bool fire = false;
+  if (!seeded) {
+    st.res_best = fminf(st.res_best, res);
+    if ((it & 63) == 63) st.check_due = true;
+    if (st.check_due && res < INFINITY) {
+      st.check_due = false;
+      fire = st.res_best >= st.res_mark;
+      st.res_mark = st.res_best;
+    }
+  }
+  to_plain = fire && !stop;
+  // NaN compares false: a non-finite residual is rejected as well.
+  const bool ok = seeded || !st.prev || (res <= st.res_prev && res < INFINITY);
+  const float mean = wave_uniform(red[1]) * inv_q;
+  float f[N], df[N];
+  float d[2] = {0.f, 0.f};
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    f[n] = (r[n] > 0.f) ? g[n] - mean - st.l[n] : 0.f;
+    df[n] = f[n] - st.pf[n];
+    d[0] = fmaf(df[n], df[n], d[0]);
+    d[1] = fmaf(df[n], f[n], d[1]);
+  }
+  slot_sum_n<B>(d);
+  const float gam = (ok && st.prev) ? fdiv(wave_uniform(d[1]),
+                                           fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f))
+                                    : 0.f;
+  //   accepted  l <- l + step, step = f - gamma (dl + df), dl <- step, pf <- f
+  //   rejected  l <- (l - dl) + pf, the previous point's plain update
+  const uint32_t keep = opaque_mask(ok);
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const float step = f[n] - gam * (st.dl[n] + df[n]);
+    const float back = (st.l[n] - st.dl[n]) + st.pf[n];
+    st.l[n] = mask_select(keep, st.l[n] + step, back);
+    st.dl[n] = mask_select(keep, step, st.dl[n]);
+    st.pf[n] = mask_select(keep, f[n], st.pf[n]);
+  }
+  st.prev = ok && !seeded;
+  st.res_prev = ok ? res : st.res_prev;
+  const uint32_t plain = opaque_mask(to_plain);
+  const uint32_t held = opaque_mask(stop);
+#pragma unroll
+  for (int n = 0; n < N; ++n) {
+    const float un = (r[n] > 0.f) ? __builtin_amdgcn_exp2f(mask_select(plain, g[n], st.l[n])) : 0.f;
+    u[n] = mask_select(held, u[n], un);
+  }
+  return res;
+}
+
+// One plain or over-relaxed u-update (same arguments and residual as
+// accel_update); stop is set when the pair meets the tolerance, and the step
+// is still taken (the iterate the reference would hold next).
 //   plain      u = r / x
 //   omega > 1  u <- u (r / (u x))^omega = u exp2(omega (log2 r - log2 (u x)));
 //              a rise in the residual switches the pair to plain steps
-//   Anderson   g = centre(log2 r - log2 x), f = g - l,
-//              l <- l + f - gamma (dl + df), gamma = (df . f) / (df . df),
-//              dl, df the changes since the previous accepted step; a step
-//              whose residual rises (or is not finite) is replaced by the
-//              previous point's plain update, and the history restarts.
-//              Centring pins the gauge (the plan ignores a constant added to
-//              every log2 u_i), so the extrapolation cannot drift along it.
 template <int B, int N>
-__device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState<N>& st,
-                                                 const float (&x)[N], const float (&r)[N],
-                                                 const float (&lr)[N], float (&u)[N], bool seeded,
-                                                 float inv_q, int it, bool& stop) {
-  const bool use_tol = a.tol > 0.f;
-  if (a.accel && !st.plain) {
-    float g[N];
-    float red[2] = {0.f, 0.f};
-#pragma unroll
-    for (int n = 0; n < N; ++n) {
-      red[0] += fabsf(u[n] * x[n] - r[n]);
-      g[n] = (r[n] > 0.f) ? lr[n] - __builtin_amdgcn_logf(x[n]) : 0.f;
-      red[1] += g[n];
-    }
-    slot_sum_n<B>(red);
-    const float res = wave_uniform(red[0]);
-    if (!seeded && use_tol && res < a.tol) {
-      stop = true;
-      return res;
-    }
-    if (!seeded) {
-      // Progress guard, due at the end of every 64-iteration window and
-      // taken at the first finite point from there: the switch takes this
-      // point's plain update (u = 2^g = r / x), never an unverified
-      // extrapolation.
-      st.res_best = fminf(st.res_best, res);
-      if ((it & 63) == 63) st.check_due = true;
-      if (st.check_due && res < INFINITY) {
-        st.check_due = false;
-        if (st.res_best >= st.res_mark) {
-          st.plain = true;
-          st.sor = false;
-#pragma unroll
-          for (int n = 0; n < N; ++n) u[n] = (r[n] > 0.f) ? __builtin_amdgcn_exp2f(g[n]) : 0.f;
-          return res;
-        }
-        st.res_mark = st.res_best;
-      }
-    }
-    // NaN compares false: a non-finite residual is rejected as well.
-    const bool ok = seeded || !st.prev || (res <= st.res_prev && res < INFINITY);
-    if (!ok) {
-#pragma unroll
-      for (int n = 0; n < N; ++n) st.l[n] = st.pl[n] + st.pf[n];
-      st.prev = false;
-    } else {
-      const float mean = wave_uniform(red[1]) * inv_q;
-      float f[N], dl[N], df[N];
-      float d[2] = {0.f, 0.f};
-#pragma unroll
-      for (int n = 0; n < N; ++n) {
-        f[n] = (r[n] > 0.f) ? g[n] - mean - st.l[n] : 0.f;
-        dl[n] = st.l[n] - st.pl[n];
-        df[n] = f[n] - st.pf[n];
-        d[0] = fmaf(df[n], df[n], d[0]);
-        d[1] = fmaf(df[n], f[n], d[1]);
-      }
-      float gam = 0.f;
-      if (st.prev) {
-        slot_sum_n<B>(d);
-        gam = fdiv(wave_uniform(d[1]), fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f));
-      }
-#pragma unroll
-      for (int n = 0; n < N; ++n) {
-        st.pl[n] = st.l[n];
-        st.pf[n] = f[n];
-        st.l[n] += f[n] - gam * (dl[n] + df[n]);
-      }
-      st.prev = !seeded;
-      st.res_prev = res;
-    }
-#pragma unroll
-    for (int n = 0; n < N; ++n) u[n] = (r[n] > 0.f) ? __builtin_amdgcn_exp2f(st.l[n]) : 0.f;
-    return res;
-  }
+__device__ __forceinline__ float plain_update(const SolveArgs& a, UpdateState<N>& st,
+                                              const float (&x)[N], const float (&r)[N],
+                                              const float (&lr)[N], float (&u)[N], bool seeded,
+                                              bool& stop) {
   float ux[N];
   float part = 0.f;
 #pragma unroll
@@ -254,7 +273,7 @@ __device__ __forceinline__ float update_scalings(const SolveArgs& a, UpdateState
 #pragma unroll
     for (int n = 0; n < N; ++n) u[n] = fdiv(r[n], x[n]);
   }
-  stop = use_tol && !seeded && res < a.tol;
+  stop = a.tol > 0.f && !seeded && res < a.tol;
   return res;
 }
 
@@ -408,13 +427,15 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   const float inv_q = 1.f / (float)q;  // centring of the accelerated update
   int it = 0;
   float res = 0.f;
-  float o[NE];
+
+  // One sweep: o = c / (K^T u) (or the seeded v), then the owned x = K o.
   // Every partial sum starts at kFltMin instead of 0: a padded slot (c = 0,
   // r = 0, kernel entries 0) then ends with a sum of a few kFltMin, so its
   // quotient is exactly 0 without a select, while real sums (bounded below by
   // the largest entry of a rescaled row, 1, times a scaling) are unchanged
   // except at exact rounding ties.
-  while (it < a.max_iter) {
+  auto sweep = [&](float (&xo)[NO]) {
+    float o[NE];
     if (seeded) {
 #pragma unroll
       for (int k = 0; k < NE; ++k) {
@@ -450,7 +471,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       xs[m] = x;
     }
     // x_i for the owned slots: scatter across rows, then the in-row masks.
-    float xo[NO];
     rows_reduce_scatter<NIP>(xs, xo);
     bfly_sum_n<B, 8>(xo);
     if constexpr (W > 1) {
@@ -465,17 +485,16 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < NO; ++k) {
-        float x = 0.f;
+        float x = buf[k * 64 + lane];
 #pragma unroll
-        for (int w = 0; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
+        for (int w = 1; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
         xo[k] = x;
       }
     }
-    // The residual is the same on every lane, so the stop test is a scalar
-    // branch.
-    bool stop = false;
-    const float rv = update_scalings<B>(a, st, xo, ro, lro, uo, seeded, inv_q, it, stop);
-    if (!seeded) res = rv;
+  };
+  // After an update: gather the scalings, count the iteration, and run the
+  // stall check of a warm-started pair; true when the pair restarts.
+  auto advance = [&]() -> bool {
     float ug[NIP];
     rows_gather<NIP>(uo, ug);
 #pragma unroll
@@ -483,23 +502,70 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     ++it;
     if (seeded) {
       seeded = false;
-      continue;
+      return false;
     }
-    if (stop) break;
     res_low = fminf(res_low, res);
     if (!fell_back && (it & 63) == 0) {
-      if (res_low >= res_win) {
-#pragma unroll
-        for (int m = 0; m < NI; ++m) u[m] = (slot_of<B, NI>(b, m) < q) ? (float)q : 0.f;
-#pragma unroll
-        for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
-        update_state_reset(st, a.omega);
-        fell_back = true;
-        if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
-      } else {
-        res_win = res_low;
+      if (res_low >= res_win) return true;
+      res_win = res_low;
+    }
+    return false;
+  };
+
+  // The accelerated rule and the plain rules run in loops of their own (an
+  // accelerated pair whose progress guard fires continues in the plain
+  // loop), so each loop carries only its own state.  The residual, and with
+  // it every branch below, is the same on every lane of every wave of the
+  // pair.
+  bool done = false;
+  for (;;) {
+    bool restart = false;
+    bool plain = !a.accel;
+    if (!plain) {
+      while (it < a.max_iter) {
+        float xo[NO];
+        sweep(xo);
+        bool stop, to_plain;
+        const float rv = accel_update<B>(a, st, xo, ro, lro, uo, seeded, inv_q, it, stop, to_plain);
+        if (!seeded) res = rv;
+        restart = advance();
+        if (stop) {  // u kept the verified point
+          restart = false;
+          done = true;
+          break;
+        }
+        if (restart) break;
+        if (to_plain) {
+          plain = true;
+          st.sor = false;
+          break;
+        }
       }
     }
+    if (plain && !restart && !done) {
+      while (it < a.max_iter) {
+        float xo[NO];
+        sweep(xo);
+        bool stop;
+        const float rv = plain_update<B>(a, st, xo, ro, lro, uo, seeded, stop);
+        if (!seeded) res = rv;
+        restart = advance();
+        if (stop) {  // the step is taken; a stall check due now is moot
+          restart = false;
+          done = true;
+          break;
+        }
+        if (restart) break;
+      }
+    }
+    if (!restart) break;
+#pragma unroll
+    for (int m = 0; m < NI; ++m) u[m] = (slot_of<B, NI>(b, m) < q) ? (float)q : 0.f;
+#pragma unroll
+    for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
+    update_state_reset(st, a.omega);
+    fell_back = true;
+    if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
   }
 
   // Final half-step: v = c / (K^T u), then the distance
