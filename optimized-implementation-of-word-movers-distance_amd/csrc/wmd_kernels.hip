@@ -324,6 +324,98 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
   if (tid < 64 && t0 + tid < a.Vt) mq[t0 + tid] = __uint_as_float(smin[tid]);
 }
 
+// Raw mode (the shared distance stage): the block takes kRawChunks
+// consecutive pseudo-queries (256 consecutive distinct query words, whose
+// output columns are contiguous in Dd) against its 64 touched words.  Each
+// wave loads its A fragments once per k-step and applies them to the four
+// chunks' B fragments, so A is fetched a quarter as often, and no row
+// minima are formed (the epilogue kernel forms them per query).  Every
+// output sees the same MFMA sequence as in k_cdist_mfma, so the distances
+// are bitwise equal.
+constexpr int kRawChunks = 4;
+
+__global__ __launch_bounds__(256) void k_cdist_mfma_raw(MfmaCdistArgs a) {
+  const int nb = gridDim.x;
+  const int xcd = blockIdx.x & 7;
+  const int lin = xcd * (nb >> 3) + min(xcd, nb & 7) + (blockIdx.x >> 3);
+  const int sq = a.sup_q;
+  const int ng = a.nquery;  // groups of kRawChunks pseudo-queries
+  const int nsq = (ng + sq - 1) / sq;
+  const int sup = lin / (kCdSuper * sq), in = lin % (kCdSuper * sq);
+  const int g = (sup % nsq) * sq + in % sq;
+  const int t0 = ((sup / nsq) * kCdSuper + in / sq) * 64;
+  if (g >= ng || t0 >= a.Vt) return;  // uniform over the block
+  const int Q0 = g * kRawChunks;
+  const int qb = a.q_ptr[Q0];
+  const int q = a.q_ptr[min(Q0 + kRawChunks, a.Sq_groups_end)] - qb;
+  const int ks = a.k_stride[Q0];
+  float* Kq = a.K + a.k_off[Q0];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wy = w >> 1, wx = w & 1;
+
+  const int Dp = a.Dp;
+  const size_t tplane = (size_t)a.Vt * Dp;
+  const size_t qplane = (size_t)a.Sq * Dp;
+  const int ta = t0 + wy * 32 + (lane & 31);
+  const bool tav = ta < a.Vt;
+  const int kh = (lane >> 5) * 8;
+  const uint16_t* ap = a.Es + (size_t)(tav ? ta : 0) * Dp + kh;
+  const uint16_t* bp[kRawChunks];
+  bool bv[kRawChunks];
+#pragma unroll
+  for (int c = 0; c < kRawChunks; ++c) {
+    const int jb = c * 64 + wx * 32 + (lane & 31);
+    bv[c] = jb < q;
+    bp[c] = a.Qs + (size_t)(qb + (bv[c] ? jb : 0)) * Dp + kh;
+  }
+  f32x16 acc[kRawChunks];
+#pragma unroll
+  for (int c = 0; c < kRawChunks; ++c) acc[c] = f32x16{};
+  for (int k0 = 0; k0 < Dp; k0 += 16) {
+    const bf16x8 a0 = load_bf16x8(ap + k0, tav);
+    const bf16x8 a1 = load_bf16x8(ap + tplane + k0, tav);
+    const bf16x8 a2 = load_bf16x8(ap + 2 * tplane + k0, tav);
+#pragma unroll
+    for (int c = 0; c < kRawChunks; ++c) {
+      const bf16x8 b0 = load_bf16x8(bp[c] + k0, bv[c]);
+      const bf16x8 b1 = load_bf16x8(bp[c] + qplane + k0, bv[c]);
+      const bf16x8 b2 = load_bf16x8(bp[c] + 2 * qplane + k0, bv[c]);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b2, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc[c], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < kRawChunks; ++c) {
+    const int jo = c * 64 + wx * 32 + (lane & 31);  // output column of this lane
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int t = t0 + wy * 32 + i;
+      if (t < a.Vt && jo < q) {
+        const double na = a.nt[t], nb = a.nq[qb + jo];
+        double d2 = na + nb - 2.0 * (double)acc[c][r];
+        if (d2 < kNearTau * (na + nb)) {
+          const float* x = a.E + (int64_t)a.touched[t] * a.D;
+          const float* y = a.E + (int64_t)a.q_words[qb + jo] * a.D;
+          double s = 0.0;
+          for (int k = 0; k < a.D; ++k) {
+            const double df = (double)x[k] - (double)y[k];
+            s += df * df;
+          }
+          d2 = s;
+        }
+        Kq[(int64_t)t * ks + jo] = (float)sqrt(d2 > 0.0 ? d2 : 0.0);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Shared distance epilogue.  Block: a.rows consecutive touched words, whose
 // Dd rows (contiguous in memory) are staged once in LDS; wave w serves
@@ -595,6 +687,21 @@ hipError_t launch_split_rows(const SplitArgs& a, hipStream_t s) {
 
 hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
   if (a.Vt <= 0 || nq <= 0) return hipSuccess;
+  if (a.raw) {
+    // Groups of kRawChunks pseudo-queries; the last group ends at query nq.
+    const int ng = (nq + kRawChunks - 1) / kRawChunks;
+    const int sq = ng < kCdSuper ? ng : kCdSuper;
+    const int64_t nst = ((a.Vt + 63) / 64 + kCdSuper - 1) / kCdSuper;
+    const int64_t nsq = (ng + sq - 1) / sq;
+    const int64_t blocks = nst * nsq * kCdSuper * sq;
+    if (blocks > INT32_MAX) return hipErrorInvalidValue;
+    MfmaCdistArgs b = a;
+    b.nquery = ng;
+    b.sup_q = sq;
+    b.Sq_groups_end = nq;
+    hipLaunchKernelGGL(k_cdist_mfma_raw, dim3((unsigned)blocks), dim3(256), 0, s, b);
+    return hipGetLastError();
+  }
   const int sq = nq < kCdSuper ? nq : kCdSuper;
   const int64_t nst = ((a.Vt + 63) / 64 + kCdSuper - 1) / kCdSuper;
   const int64_t nsq = (nq + sq - 1) / sq;
