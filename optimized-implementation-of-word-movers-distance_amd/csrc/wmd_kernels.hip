@@ -689,8 +689,10 @@ hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
   if (a.Vt <= 0 || nq <= 0) return hipSuccess;
   if (a.raw) {
     // Groups of kRawChunks pseudo-queries; the last group ends at query nq.
+    // A super-tile spans kCdSuper pseudo-queries as in the per-query kernel,
+    // so its operand planes still fit one XCD's L2.
     const int ng = (nq + kRawChunks - 1) / kRawChunks;
-    const int sq = ng < kCdSuper ? ng : kCdSuper;
+    const int sq = ng < kCdSuper / kRawChunks ? ng : kCdSuper / kRawChunks;
     const int64_t nst = ((a.Vt + 63) / 64 + kCdSuper - 1) / kCdSuper;
     const int64_t nsq = (ng + sq - 1) / sq;
     const int64_t blocks = nst * nsq * kCdSuper * sq;
