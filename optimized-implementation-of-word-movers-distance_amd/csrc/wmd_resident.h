@@ -475,20 +475,35 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     bfly_sum_n<B, 8>(xo);
     if constexpr (W > 1) {
       // The waves of a pair own the same slots (ownership follows the row):
-      // add the other waves' sums through LDS, in wave order, so every wave
-      // holds bitwise-equal totals.  Double buffering: the next write to
-      // this half happens two iterations later, after every wave has passed
-      // the next barrier.
+      // add the waves' sums through LDS, in wave order, so every wave holds
+      // bitwise-equal totals.  Double buffering: the next write to this half
+      // happens two iterations later, after every wave has passed the next
+      // barrier.
       float* buf = xsh + (it & 1) * (W * NO * 64);
 #pragma unroll
       for (int k = 0; k < NO; ++k) buf[(wv * NO + k) * 64 + lane] = xo[k];
       __syncthreads();
+      if constexpr (W >= 4) {
+        // Wave wv totals slots wv, wv + W, ... into wave 0's entries (no
+        // other wave touches those slots in between), then every wave reads
+        // the totals: W - 1 adds per slot once instead of in every wave.
+        for (int k = wv; k < NO; k += W) {
+          float x = buf[k * 64 + lane];
 #pragma unroll
-      for (int k = 0; k < NO; ++k) {
-        float x = buf[k * 64 + lane];
+          for (int w = 1; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
+          buf[k * 64 + lane] = x;
+        }
+        __syncthreads();
 #pragma unroll
-        for (int w = 1; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
-        xo[k] = x;
+        for (int k = 0; k < NO; ++k) xo[k] = buf[k * 64 + lane];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NO; ++k) {
+          float x = buf[k * 64 + lane];
+#pragma unroll
+          for (int w = 1; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
+          xo[k] = x;
+        }
       }
     }
   };
