@@ -62,7 +62,7 @@ struct MfmaCdistArgs {
   int raw;                 // as in CdistArgs
   int nquery;              // queries of the launch (set by the launcher)
   int sup_q;               // queries per super-tile (set by the launcher)
-  int Sq_groups_end;       // raw mode: pseudo-queries of the launch (set by the launcher)
+  int raw_queries;       // raw mode: pseudo-queries of the launch (set by the launcher)
 };
 
 // Shared distance stage of a query batch: the raw distances of the batch's

@@ -3,7 +3,8 @@
 // Stages (reference: sinkhorn_wmd_fused.cpp:112-412, sinkhorn_wmd_nonfused.cpp:114-318):
 //   k_cdist          distances M = cdist(query words, touched words) in FP32
 //                    direct differences, row-min rescale, exp epilogue -> Kt, mrow
-//   k_cdist_mfma     the same distances on the matrix cores (three-way BF16 split)
+//   k_cdist_mfma     the same distances on the matrix cores (three-way BF16 split);
+//                    k_cdist_mfma_raw, its raw form for the shared distance stage
 //   k_resident<...>  one (query, doc) pair per wave/workgroup: the pair's q x n
 //                    tile of Kt is gathered once into registers and every
 //                    Sinkhorn iteration plus the final (K o M) pass runs on-chip
@@ -301,7 +302,7 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
       if ((lane & 31) == 0) atomicMin(&smin[wy * 32 + i], __float_as_uint(mn));
     }
   }
-  if (a.raw) return;  // uniform over the block
+  // (Raw-mode launches run k_cdist_mfma_raw instead.)
   __syncthreads();
   const float l2 = a.lamda * 1.4426950408889634f;
   for (int qc = 0; qc < ks; qc += 64) {
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(256) void k_cdist_mfma_raw(MfmaCdistArgs a) {
   if (g >= ng || t0 >= a.Vt) return;  // uniform over the block
   const int Q0 = g * kRawChunks;
   const int qb = a.q_ptr[Q0];
-  const int q = a.q_ptr[min(Q0 + kRawChunks, a.Sq_groups_end)] - qb;
+  const int q = a.q_ptr[min(Q0 + kRawChunks, a.raw_queries)] - qb;
   const int ks = a.k_stride[Q0];
   float* Kq = a.K + a.k_off[Q0];
   const int tid = threadIdx.x;
@@ -700,7 +701,7 @@ hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
     MfmaCdistArgs b = a;
     b.nquery = ng;
     b.sup_q = sq;
-    b.Sq_groups_end = nq;
+    b.raw_queries = nq;
     hipLaunchKernelGGL(k_cdist_mfma_raw, dim3((unsigned)blocks), dim3(256), 0, s, b);
     return hipGetLastError();
   }
