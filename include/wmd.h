@@ -88,7 +88,7 @@ enum wmd_accel {
   /* plain Sinkhorn steps (or over-relaxed ones, see omega) */
   WMD_ACCEL_NONE = 0,
   /* Anderson acceleration with a history of one step on the centred base-2
-   * potentials l = log2 u - mean(log2 u): with g the plain update of l
+   * potentials l = log2 u - mean(log2 u) (both domains): with g the plain update of l
    * (centred) and f = g - l, the next l is l + f - gamma (dl + df), gamma =
    * (df . f) / (df . df), dl and df the changes since the previous step.  A
    * step whose marginal residual rises (or is not finite) is rejected and
@@ -132,7 +132,7 @@ typedef struct wmd_options {
   int32_t warm_start;  /* enum wmd_warm */
   int32_t export_duals;/* 1: keep the converged log v of every (query, nonzero)
                           for wmd_context_duals / training (resident variant) */
-  int32_t accel;       /* enum wmd_accel (resident variant, linear domain, tol > 0) */
+  int32_t accel;       /* enum wmd_accel (resident variant, tol > 0; both domains) */
   double omega;        /* resident variant: over-relaxation of the u-update,
                           u <- u (r / (u K v))^omega, 1 = plain Sinkhorn, in
                           [1, 2).  The fixed point and the residual test are

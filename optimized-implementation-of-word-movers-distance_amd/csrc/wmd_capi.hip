@@ -1199,10 +1199,8 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     set_err("learned warm start and dual export run on the resident variant in the linear domain");
     return WMD_ERR_UNSUPPORTED;
   }
-  if (o.accel != WMD_ACCEL_NONE &&
-      (variant != WMD_VARIANT_RESIDENT || log_domain || !(o.tol > 0.0))) {
-    set_err("Anderson acceleration runs on the resident variant in the linear domain with a "
-            "tolerance (tol > 0)");
+  if (o.accel != WMD_ACCEL_NONE && (variant != WMD_VARIANT_RESIDENT || !(o.tol > 0.0))) {
+    set_err("Anderson acceleration runs on the resident variant with a tolerance (tol > 0)");
     return WMD_ERR_UNSUPPORTED;
   }
   if (log_domain && variant != WMD_VARIANT_RESIDENT) {

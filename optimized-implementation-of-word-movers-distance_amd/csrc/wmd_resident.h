@@ -697,6 +697,17 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
   int it = 0;
   float res = 0.f;
   float g[NE];
+  // Anderson acceleration on the base-2 potentials, as in accel_update: al
+  // the centred potentials, adl the last accepted step, apf that point's f;
+  // every lane holds all of its NI slots, so the sums over query words are
+  // sums over the B lanes of a group.
+  bool plain = !a.accel;
+  float al[NI], adl[NI], apf[NI];
+#pragma unroll
+  for (int m = 0; m < NI; ++m) al[m] = adl[m] = apf[m] = 0.f;
+  bool prev = false, check_due = false;
+  float res_best = INFINITY, res_mark = INFINITY;
+  const float inv_q = 1.f / (float)q;
   while (it < a.max_iter) {
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
@@ -761,6 +772,61 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
     for (int m = 0; m < NI; ++m)
       if (r[m] > 0.f) part += fabsf(exp2f(f[m] + lx[m]) - r[m]);
     res = group_sum<B>(part);
+    if (!plain) {
+      // The residual is the same on every lane of the pair, so every branch
+      // here is uniform.  A stop keeps the verified point.
+      const bool stop = use_tol && res < a.tol;
+      bool fire = false;
+      res_best = fminf(res_best, res);
+      if ((it & 63) == 63) check_due = true;
+      if (check_due && res < INFINITY) {
+        check_due = false;
+        fire = res_best >= res_mark;
+        res_mark = res_best;
+      }
+      ++it;
+      if (stop) break;
+      float gm[NI];
+      float sg = 0.f;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        gm[m] = (r[m] > 0.f) ? lr[m] - lx[m] : 0.f;
+        sg += gm[m];
+      }
+      if (fire) {  // progress guard: plain steps from this point's plain update
+        plain = true;
+#pragma unroll
+        for (int m = 0; m < NI; ++m) f[m] = (r[m] > 0.f) ? gm[m] : kLogZero;
+        continue;
+      }
+      const bool ok = !prev || (res <= res_prev && res < INFINITY);
+      const float mean = group_sum<B>(sg) * inv_q;
+      float fv[NI], df[NI];
+      float d0 = 0.f, d1 = 0.f;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        fv[m] = (r[m] > 0.f) ? gm[m] - mean - al[m] : 0.f;
+        df[m] = fv[m] - apf[m];
+        d0 = fmaf(df[m], df[m], d0);
+        d1 = fmaf(df[m], fv[m], d1);
+      }
+      d0 = group_sum<B>(d0);
+      d1 = group_sum<B>(d1);
+      const float gam = (ok && prev) ? fdiv(d1, fmaf(d0, 1.f + 1e-6f, 1e-30f)) : 0.f;
+      const uint32_t keep = opaque_mask(ok);
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        const float step = fv[m] - gam * (adl[m] + df[m]);
+        const float back = (al[m] - adl[m]) + apf[m];
+        al[m] = mask_select(keep, al[m] + step, back);
+        adl[m] = mask_select(keep, step, adl[m]);
+        apf[m] = mask_select(keep, fv[m], apf[m]);
+        f[m] = (r[m] > 0.f) ? al[m] : kLogZero;
+      }
+      prev = ok;
+      res_prev = ok ? res : res_prev;
+      continue;
+    }
     if (sor) {
       if (res > res_prev) sor = false;
       res_prev = res;

@@ -95,6 +95,52 @@ def test_log_domain_every_tile_shape(P, dg, oracle):
         assert err <= 2e-5, q
 
 
+def test_log_domain_anderson_precision_workload(precision, oracle):
+    # the accelerated rule on the base-2 potentials of the log-domain kernel:
+    # same fixed point as plain log-domain steps, judged against converged
+    # FP64 on the first 300 documents, in fewer iterations
+    corpus, qs, p, e = precision
+    say("[case] log domain, Anderson, lambda=50, residual 1e-4, against converged FP64")
+    e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"], log_domain=True)
+    wp, itp, _ = e.result()
+    e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"], log_domain=True, accel=True)
+    wa, ita, ra = e.result()
+    e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"], log_domain=True, accel=True)
+    assert np.array_equal(e.result()[0], wa)  # deterministic
+    n = 300
+    dp = corpus.doc_ptr[:n + 1]
+    truth, _, _ = oracle.perdoc(*qs.query(0), corpus.vecs.astype(np.float64), dp,
+                                corpus.doc_words[:dp[-1]], corpus.doc_vals[:dp[-1]], p["lamda"],
+                                100000, tol=1e-10, log_domain=True)
+    ep, ea = rel_err(wp[0, :n], truth), rel_err(wa[0, :n], truth)
+    say(f"  iterations plain {itp.sum()} (max {itp.max()}), Anderson {ita.sum()} (max {ita.max()}); "
+        f"error vs converged FP64: plain {ep:.2e}, Anderson {ea:.2e}")
+    assert np.all(ra < p["tol"])
+    assert ea <= 2e-4
+    assert ita.sum() < itp.sum()
+
+
+def test_log_domain_anderson_every_tile_shape(P, dg, oracle):
+    say("[case] log domain, Anderson, across tile shapes, lambda=30")
+    corpus, _, _ = dg.workload("batched", seed=1, N=400, n_queries=1)
+    rng = np.random.default_rng(12)
+    lengths = [3, 8, 20, 40, 64, 100, 200, 300]
+    qs = dg.make_queries(rng, corpus.V, lengths)
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-30.0, max_iter=20000, tol=1e-4, log_domain=True, accel=True)
+        w, it, res = e.result()
+    wo, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, corpus.vecs.astype(np.float64),
+                                   corpus.doc_ptr, corpus.doc_words, corpus.doc_vals, -30.0, 100000,
+                                   tol=1e-10, log_domain=True)
+    assert np.all(res < 1e-4)
+    for k, q in enumerate(lengths):
+        err = rel_err(w[k], wo[k])
+        say(f"  q={q:3d} err {err:.2e} iters mean {it[k].mean():.1f} max {it[k].max()}")
+        assert err <= 2e-4, q
+
+
 def test_log_domain_needs_resident_variant(P, dg):
     corpus, qs, p = dg.workload("tiny", seed=0)
     with P.WMDEngine(0) as e:
