@@ -88,9 +88,10 @@ enum wmd_accel {
   /* plain Sinkhorn steps (or over-relaxed ones, see omega) */
   WMD_ACCEL_NONE = 0,
   /* Anderson acceleration with a history of one step on the centred base-2
-   * potentials l = log2 u - mean(log2 u) (both domains): with g the plain update of l
-   * (centred) and f = g - l, the next l is l + f - gamma (dl + df), gamma =
-   * (df . f) / (df . df), dl and df the changes since the previous step.  A
+   * potentials l = log2 u - mean(log2 u), in either domain: with g the plain
+   * update of l (centred) and f = g - l, the next l is l + f - gamma (dl +
+   * df), gamma = (df . f) / (df . df), dl and df the changes since the
+   * previous step.  A
    * step whose marginal residual rises (or is not finite) is rejected and
    * replaced by the previous point's plain update.  A pair whose best
    * residual does not improve over a 64-iteration window continues with

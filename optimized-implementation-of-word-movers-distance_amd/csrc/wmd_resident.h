@@ -795,6 +795,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
       }
       if (fire) {  // progress guard: plain steps from this point's plain update
         plain = true;
+        sor = false;
 #pragma unroll
         for (int m = 0; m < NI; ++m) f[m] = (r[m] > 0.f) ? gm[m] : kLogZero;
         continue;
