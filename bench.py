@@ -236,6 +236,32 @@ def plain_shape(P, dg, reps: int = 5):
     return out
 
 
+def precision_shape(P, dg, reps: int = 5):
+    """The precision form: lambda = 50 with 10% near-duplicate query words
+    (one 21-word query x 5,000 documents), log domain, each pair to marginal
+    residual 1e-4, corpus resident; device time of the distance stage and the
+    solve with plain and with Anderson-accelerated steps."""
+    corpus, qs, prm = dg.workload("precision", seed=0)
+    q = int(qs.lengths()[0])
+    out = {"workload": f"precision: 1 query (q={q}, near-duplicate words) x {corpus.N} docs, "
+                       f"lambda {abs(prm['lamda']):g}, log domain, residual {prm['tol']:g}"}
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        for name, accel in (("plain", False), ("anderson", True)):
+            kw = dict(max_iter=prm["max_iter"], tol=prm["tol"], log_domain=True, accel=accel)
+            e.run(prm["lamda"], **kw)  # warm
+            cd = sv = 0.0
+            for _ in range(reps):
+                st = e.run(prm["lamda"], **kw)
+                cd += st["ms_cdist"]
+                sv += st["ms_solve"]
+            cd, sv = cd / reps, sv / reps
+            out[name] = {"ms_cdist": cd, "ms_solve": sv, "pairs_per_s": corpus.N / ((cd + sv) * 1e-3),
+                         "iters_mean": float(st["iters_mean"]), "iters_max": int(st["iters_max"])}
+    return out
+
+
 def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") -> int:
     """Per-step wall span of one kernel family in a rocprofv3 kernel_trace.csv.
 
@@ -436,7 +462,7 @@ def main() -> int:
         log(f"plain GPU method: {plain['value']:.4g} pairs/s on {nq} queries")
     eng.close()
     traffic = measured_traffic()
-    drop_in = plain_cfg = None
+    drop_in = plain_cfg = prec_cfg = None
     if not args.no_plain:
         drop_in = drop_in_rate(P, dg)
         log(f"drop-in entry (host buffers, PCIe-inclusive): {drop_in['value']:.4g} pairs/s, "
@@ -445,6 +471,10 @@ def main() -> int:
         log("reference shape, corpus resident: " + ", ".join(
             f"{k} {plain_cfg[k]['ms_solve']:.3f} ms solve ({plain_cfg[k]['alg_gbs']:.0f} GB/s alg.)"
             for k in ("resident", "fused", "nonfused")))
+        prec_cfg = precision_shape(P, dg)
+        log("precision shape (lambda 50, log domain): " + ", ".join(
+            f"{k} {prec_cfg[k]['ms_solve']:.3f} ms solve, {prec_cfg[k]['iters_mean']:.1f} iterations"
+            for k in ("plain", "anderson")))
 
     cpu = cpu_port = None
     if not args.no_cpu_baseline:
@@ -503,6 +533,7 @@ def main() -> int:
         "plain_gpu": plain,
         "drop_in": drop_in,
         "reference_shape": plain_cfg,
+        "precision_shape": prec_cfg,
     }
     print(json.dumps(line), flush=True)
     return 0
