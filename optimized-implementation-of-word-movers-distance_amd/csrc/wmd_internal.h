@@ -137,8 +137,8 @@ struct SolveArgs {
 // a query-word group stays inside a 16-lane row and every cross-row step is
 // a reduce-scatter; multi-wave linear shapes put more document-word lanes on
 // long queries instead of wider query-word groups.
-constexpr int kNumDocClasses = 10;
-constexpr int kDocClassWidth = 16;
+constexpr int kNumDocClasses = 20;
+constexpr int kDocClassWidth = 8;
 struct TileShape {
   int W, B, NI, lin, log;
   constexpr int cap_q() const { return B * NI; }

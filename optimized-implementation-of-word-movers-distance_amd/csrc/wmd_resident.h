@@ -910,7 +910,7 @@ template <int W, int B, int NI, bool LIN, bool LOG>
 hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
   constexpr int A = 64 * W / B;
   constexpr int C = kDocClassWidth;
-  static_assert(kNumDocClasses == 10, "one case per document-length class");
+  static_assert(kNumDocClasses == 20, "one case per document-length class");
   switch (cls) {
     case 0: return launch_res<W, B, NI, (1 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 1: return launch_res<W, B, NI, (2 * C + A - 1) / A, LIN, LOG>(lg, a, s);
@@ -922,6 +922,16 @@ hipError_t launch_shape(int cls, bool lg, const SolveArgs& a, hipStream_t s) {
     case 7: return launch_res<W, B, NI, (8 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 8: return launch_res<W, B, NI, (9 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     case 9: return launch_res<W, B, NI, (10 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 10: return launch_res<W, B, NI, (11 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 11: return launch_res<W, B, NI, (12 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 12: return launch_res<W, B, NI, (13 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 13: return launch_res<W, B, NI, (14 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 14: return launch_res<W, B, NI, (15 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 15: return launch_res<W, B, NI, (16 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 16: return launch_res<W, B, NI, (17 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 17: return launch_res<W, B, NI, (18 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 18: return launch_res<W, B, NI, (19 * C + A - 1) / A, LIN, LOG>(lg, a, s);
+    case 19: return launch_res<W, B, NI, (20 * C + A - 1) / A, LIN, LOG>(lg, a, s);
     default: return hipErrorInvalidValue;
   }
 }

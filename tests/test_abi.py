@@ -60,7 +60,7 @@ def test_host_only_entry_points():
     for (w, cap, _), (_, prev, _) in zip(shapes[1:], shapes[:-1]):
         assert cap - prev <= 64 * w // 8 or cap - prev <= 16
     n_cls, max_q = pkg().api.tuning_dims()
-    assert n_cls == 10 and max_q == caps[-1]
+    assert n_cls == 20 and max_q == caps[-1]
     assert L.wmd_context_autotune(None, -1.0, None, 8, None) == nat.WMD_ERR_ARG
     assert L.wmd_context_get_tuning(None, 0, None) == nat.WMD_ERR_ARG
     assert L.wmd_context_set_tuning(None, 0, None) == nat.WMD_ERR_ARG

@@ -166,7 +166,7 @@ def test_tile_class_boundaries_and_maximum_sizes(P, dg, oracle):
     V, D = 20_000, 300
     vecs = dg.make_embeddings(rng, V, D, norm_lognormal=(0.7, 0.3))
     cdf = dg.zipf_cdf(V)
-    lens = [1, 2] + [n for k in range(1, 11) for n in (16 * k - 1, 16 * k, 16 * k + 1)
+    lens = [1, 2] + [n for k in range(1, 21) for n in (8 * k - 1, 8 * k, 8 * k + 1)
                      if n <= 160]
     lens = lens * 2
     ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
