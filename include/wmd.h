@@ -293,16 +293,21 @@ int wmd_shape_domains(int shape);
  * the options' domain.  It runs the distance stage first and leaves no valid
  * result (run again before wmd_context_result).  ms_out (may be NULL)
  * receives the tuning time.  Tables are [n_classes][max_q + 1] shape ids
- * (-1: smallest that fits) per domain, document class c holding documents of
- * at most 32 (c + 1) words. */
+ * (-1: smallest that fits) per domain, with n_classes and max_q from
+ * wmd_tuning_dims; document class c holds the documents of 8 c + 1 to
+ * 8 (c + 1) words (class 0 also the empty ones).  n_entries is the table's
+ * length in int32 entries and must equal n_classes * (max_q + 1): a table
+ * saved by a build with other dimensions is refused (WMD_ERR_ARG). */
 int wmd_context_autotune(wmd_context* ctx, double lamda, const wmd_options* opt, int32_t iters,
                          double* ms_out);
 int wmd_tuning_dims(int32_t* n_classes, int32_t* max_q);
-int wmd_context_get_tuning(wmd_context* ctx, int32_t domain, int32_t* table);
+int wmd_context_get_tuning(wmd_context* ctx, int32_t domain, int32_t* table,
+                           uint64_t n_entries);
 /* Load a table (e.g. saved from an earlier autotune); NULL restores the
- * default.  Entries must hold their q (and have a log-domain form for the
- * log domain). */
-int wmd_context_set_tuning(wmd_context* ctx, int32_t domain, const int32_t* table);
+ * default (n_entries is then ignored).  Entries must hold their q (and have
+ * a log-domain form for the log domain). */
+int wmd_context_set_tuning(wmd_context* ctx, int32_t domain, const int32_t* table,
+                           uint64_t n_entries);
 
 /* ---- The reference's file formats (host only, no device), as read and
  * written by its programs' main (sinkhorn_wmd_fused.cpp:415-624):

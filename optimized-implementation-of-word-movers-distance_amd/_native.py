@@ -138,8 +138,8 @@ def load():
     L.wmd_context_autotune.argtypes = [P, C.c_double, C.POINTER(Options), C.c_int32,
                                        C.POINTER(C.c_double)]
     L.wmd_tuning_dims.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
-    L.wmd_context_get_tuning.argtypes = [P, C.c_int32, P]
-    L.wmd_context_set_tuning.argtypes = [P, C.c_int32, P]
+    L.wmd_context_get_tuning.argtypes = [P, C.c_int32, P, U64]
+    L.wmd_context_set_tuning.argtypes = [P, C.c_int32, P, U64]
     S = C.c_char_p
     L.wmd_mtx_header.argtypes = [S, P, P, P]
     L.wmd_read_mtx.argtypes = [S, U64, U64, U64, P, P, P]

@@ -175,7 +175,7 @@ class WMDEngine:
         ncls, maxq = tuning_dims()
         t = np.empty((ncls, maxq + 1), dtype=np.int32)
         nat.check(self._L.wmd_context_get_tuning(
-            self._h, nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR, t.ctypes.data),
+            self._h, nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR, t.ctypes.data, t.size),
             "wmd_context_get_tuning")
         return t
 
@@ -183,13 +183,14 @@ class WMDEngine:
         """Load a table from get_tuning (None restores the default)."""
         dom = nat.DOMAIN_LOG if log_domain else nat.DOMAIN_LINEAR
         if table is None:
-            nat.check(self._L.wmd_context_set_tuning(self._h, dom, None), "wmd_context_set_tuning")
+            nat.check(self._L.wmd_context_set_tuning(self._h, dom, None, 0),
+                      "wmd_context_set_tuning")
             return
         ncls, maxq = tuning_dims()
         t = np.ascontiguousarray(table, dtype=np.int32)
         if t.shape != (ncls, maxq + 1):
             raise ValueError(f"tuning table must be {ncls} x {maxq + 1}")
-        nat.check(self._L.wmd_context_set_tuning(self._h, dom, t.ctypes.data),
+        nat.check(self._L.wmd_context_set_tuning(self._h, dom, t.ctypes.data, t.size),
                   "wmd_context_set_tuning")
 
     # ---- learned warm start ----

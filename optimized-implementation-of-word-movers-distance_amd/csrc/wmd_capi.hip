@@ -1379,6 +1379,20 @@ int wmd_tuning_dims(int32_t* n_classes, int32_t* max_q) {
 
 namespace {
 
+// A tuning table is [kNumDocClasses][kMaxQ + 1]; a caller's buffer of any
+// other length (e.g. saved by a build with other class or shape limits) is
+// refused before it is read or written.
+int check_tuning_len(uint64_t n_entries) {
+  const uint64_t want = (uint64_t)wmd::kNumDocClasses * kTuneStride;
+  if (n_entries != want) {
+    set_err("tuning table has %llu entries, this build's is %d classes x %d = %llu "
+            "(wmd_tuning_dims)", (unsigned long long)n_entries, wmd::kNumDocClasses,
+            kTuneStride, (unsigned long long)want);
+    return WMD_ERR_ARG;
+  }
+  return WMD_OK;
+}
+
 std::vector<int32_t> default_tuning(bool log_domain) {
   std::vector<int32_t> t((size_t)wmd::kNumDocClasses * kTuneStride, -1);
   for (int cls = 0; cls < wmd::kNumDocClasses; ++cls)
@@ -1486,20 +1500,23 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
   return WMD_OK;
 }
 
-int wmd_context_get_tuning(wmd_context* c, int32_t domain, int32_t* table) {
+int wmd_context_get_tuning(wmd_context* c, int32_t domain, int32_t* table, uint64_t n_entries) {
   int rc = check_ctx(c);
   if (rc != WMD_OK) return rc;
   if (!table || (domain != WMD_DOMAIN_LINEAR && domain != WMD_DOMAIN_LOG)) {
     set_err("null table or unknown domain");
     return WMD_ERR_ARG;
   }
+  rc = check_tuning_len(n_entries);
+  if (rc != WMD_OK) return rc;
   const bool lg = domain == WMD_DOMAIN_LOG;
   const std::vector<int32_t> t = c->tune[lg ? 1 : 0].empty() ? default_tuning(lg) : c->tune[lg ? 1 : 0];
   std::memcpy(table, t.data(), t.size() * sizeof(int32_t));
   return WMD_OK;
 }
 
-int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table) {
+int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table,
+                           uint64_t n_entries) {
   int rc = check_ctx(c);
   if (rc != WMD_OK) return rc;
   if (domain != WMD_DOMAIN_LINEAR && domain != WMD_DOMAIN_LOG) {
@@ -1512,6 +1529,8 @@ int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table)
     ++c->plan_epoch;
     return WMD_OK;
   }
+  rc = check_tuning_len(n_entries);
+  if (rc != WMD_OK) return rc;
   for (int cls = 0; cls < wmd::kNumDocClasses; ++cls)
     for (int q = 1; q <= wmd::kMaxQ; ++q) {
       const int s = table[(size_t)cls * kTuneStride + q];

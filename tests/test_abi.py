@@ -62,8 +62,8 @@ def test_host_only_entry_points():
     n_cls, max_q = pkg().api.tuning_dims()
     assert n_cls == 20 and max_q == caps[-1]
     assert L.wmd_context_autotune(None, -1.0, None, 8, None) == nat.WMD_ERR_ARG
-    assert L.wmd_context_get_tuning(None, 0, None) == nat.WMD_ERR_ARG
-    assert L.wmd_context_set_tuning(None, 0, None) == nat.WMD_ERR_ARG
+    assert L.wmd_context_get_tuning(None, 0, None, 0) == nat.WMD_ERR_ARG
+    assert L.wmd_context_set_tuning(None, 0, None, 0) == nat.WMD_ERR_ARG
 
 
 def test_null_arguments_rejected_before_device_use():

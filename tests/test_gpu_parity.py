@@ -341,6 +341,14 @@ def test_autotuned_shapes_agree(P, dg):
         bad[0, caps[0] + 1] = 0  # shape 0 cannot hold cap + 1 words
         with pytest.raises(P.WMDError):
             e.set_tuning(bad)
+        # a table of another build's dimensions (fewer classes) is refused by
+        # length through the C ABI, before any entry is read or written
+        L, nat = e._L, P._native
+        short = np.ascontiguousarray(default[:10])
+        assert L.wmd_context_set_tuning(e._h, 0, short.ctypes.data, short.size) == nat.WMD_ERR_ARG
+        assert b"tuning table has" in L.wmd_last_error()
+        assert L.wmd_context_get_tuning(e._h, 0, short.ctypes.data, short.size) == nat.WMD_ERR_ARG
+        assert np.array_equal(short, default[:10])
         e.set_tuning(None)
         assert np.array_equal(e.get_tuning(), default)
 
