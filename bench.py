@@ -1,15 +1,20 @@
 #!/usr/bin/env python3
 """Throughput of the Sinkhorn-Knopp WMD engine on one MI355X.
 
-A step is one solve of the batched workload: 256 queries of 5-300 distinct
-words against 50,000 documents (V=100,000, D=300 FP32, lambda=10), every
-(query, document) pair iterated until its marginal residual is below 1e-4.
-Corpus and queries are resident in HBM before timing starts; a step runs the
-distance stage and the solver and ends when the results are on the device.
+A step is one solve of the batched workload with the learned warm start:
+256 queries of 5-300 distinct words against 50,000 documents (V=100,000,
+D=300 FP32, lambda=10), every (query, document) pair iterated until its
+marginal residual is below 1e-4.  Corpus, queries and the trained warm-start
+network are resident in HBM before timing starts; a step runs the distance
+stage, the warm-start inference and the solver, and ends when the results
+are on the device.  The network is trained once beforehand (on other seeded
+pairs over the same embeddings), like the tile-shape autotuning.
 
-Prints ONE JSON line (metric, value in pairs/s, roofline of the solver
-stage, the CPU baseline timed on this host, and the plain GPU method).
-Progress goes to stderr.
+Prints ONE JSON line: the metric in pairs/s, the solver stage's roofline
+(FP32 vector ALU, and its HBM fraction by the algorithmic-bytes measure), the
+error against converged FP64 of every reported result, the reference's own
+CPU path timed on this host, and the plain GPU method.  Progress goes to
+stderr.
 """
 from __future__ import annotations
 
@@ -17,6 +22,7 @@ import argparse
 import importlib
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -27,8 +33,11 @@ sys.path.insert(0, REPO)
 PKG = "optimized-implementation-of-word-movers-distance_amd"
 
 METRIC = "query-doc WMD pairs/sec at 1e-4 marginal residual, learned+batched vs plain"
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 peak
+# MI355X published peaks.  Packed FP32 is disabled for all device code, so the
+# FP32 vector ALU's FMA issue ceiling is half the 157.3 TFLOP/s FP32 figure.
+VALU_FP32_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+ERR_DOCS = 300  # documents per sampled query in the error checks
 
 
 def log(*a):
@@ -51,97 +60,88 @@ def sinkhorn_bytes(V, nnz, vt, q_len, N, iters_mean_per_query):
     return float(np.sum(per_q * iters_mean_per_query))
 
 
-def _doc_subset_csr(corpus, n_sub):
-    """Word-major V x n_sub CSR of the first n_sub documents."""
-    b = int(corpus.doc_ptr[n_sub])
-    words = corpus.doc_words[:b]
-    docs = np.repeat(np.arange(n_sub, dtype=np.int64), np.diff(corpus.doc_ptr[:n_sub + 1]))
-    order = np.lexsort((docs, words))
-    row_ptr = np.zeros(corpus.V + 1, dtype=np.uint64)
-    row_ptr[1:] = np.cumsum(np.bincount(words, minlength=corpus.V))
-    return (row_ptr, docs[order].astype(np.uint64), corpus.doc_vals[:b][order], n_sub)
-
-
-def cpu_baseline(corpus, qs, iters, lamda, budget_s, threads):
-    """The reference's fused sinkhorn_wmd (built from its sources into
-    oracle/_ref) on one query against a document subset sized to the time
-    budget.  The reference has no residual test, so it runs the fixed number
-    of iterations the slowest pair of the sample needed on the GPU."""
-    from oracle import pyoracle
-    use_ref = pyoracle.ref_available()
-    fn = pyoracle.ref_fused if use_ref else pyoracle.fused
-    vecs = corpus.vecs.astype(np.float64)
-    k = 0
-    r = qs.dense_r(k, corpus.V)
-    n_sub = min(corpus.N, 2000)
-    csr = _doc_subset_csr(corpus, n_sub)
-    log(f"cpu baseline: calibrating on {n_sub} docs ({threads} threads)")
-    t0 = time.perf_counter()
-    fn(r, vecs, csr, lamda, 2, nthreads=threads)
-    t2 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    fn(r, vecs, csr, lamda, 6, nthreads=threads)
-    t6 = time.perf_counter() - t0
-    per_it = max((t6 - t2) / 4.0, 1e-6)
-    setup = max(t2 - 2.0 * per_it, 0.0)
-    need = int(iters[k, :n_sub].max())
-    while setup + need * per_it > budget_s and n_sub > 50:
-        n_sub //= 2
-        per_it /= 2.0
-        need = int(iters[k, :n_sub].max())
-    csr = _doc_subset_csr(corpus, n_sub)
-    log(f"cpu baseline: query {k} (q={qs.lengths()[k]}) x {n_sub} docs, {need} iterations, "
-        f"projected {setup + need * per_it:.1f}s")
-    t0 = time.perf_counter()
-    fn(r, vecs, csr, lamda, need, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {
-        "value": n_sub / dt, "unit": "pairs/s", "cores": threads,
-        "kind": "reference" if use_ref else "port",
-        "sample": (f"query {k} (q={int(qs.lengths()[k])}) against the first {n_sub} of "
-                   f"{corpus.N} documents, {need} fixed iterations (the most any pair of the "
-                   f"sample needed on the GPU to reach residual 1e-4; the reference has no "
-                   f"residual test), FP64, {dt:.2f}s"),
-        "seconds": round(dt, 3),
-    }
-
-
-def cpu_port_per_pair(corpus, qs, lamda, tol, max_iter, threads, n_docs):
-    """The FP64 oracle's per-document form with the same per-pair stop rule
-    as the GPU, on one query against n_docs documents."""
-    from oracle import pyoracle
-    k = 0
-    wk, vk = qs.query(k)
-    ptr = corpus.doc_ptr[:n_docs + 1]
-    b = int(ptr[-1])
-    log(f"cpu port (per-pair stop): query {k} x {n_docs} docs")
-    t0 = time.perf_counter()
-    pyoracle.perdoc(wk, vk, corpus.vecs.astype(np.float64), ptr, corpus.doc_words[:b],
-                    corpus.doc_vals[:b], lamda, max_iter, tol=tol, nthreads=threads)
-    dt = time.perf_counter() - t0
-    return {"value": n_docs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"query {k} x first {n_docs} docs, per-pair stop at {tol:g}, FP64, "
-                      f"{dt:.2f}s"}
+def sample_queries(qs):
+    """The queries the error checks sample: shortest, median and longest."""
+    L = qs.lengths()
+    o = np.argsort(L, kind="stable")
+    return [int(o[0]), int(o[len(o) // 2]), int(o[-1])]
 
 
 def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=6):
-    """Train the warm-start MLP on converged duals of n_queries x n_docs
+    """Train the warm-start network on converged duals of n_queries x n_docs
     seeded synthetic pairs over the same embeddings (not part of a step)."""
     tc, tq = dg.training_set(vecs, n_queries=n_queries, n_docs=n_docs, seed=101)
     t0 = time.perf_counter()
     with P.WMDEngine(0) as e:
         e.set_corpus(tc.vecs, tc.word_major())
         e.set_queries(tq.q_ptr, tq.words, tq.weights)
-        e.run(lamda, max_iter=20000, tol=1e-6, export_duals=True, accel=True)
-        e.mlp_set(P.mlp_init(tc.D, seed=0))
-        losses = e.mlp_train(epochs=epochs, pairs_per_batch=32, lr=1e-3, seed=0)
+        e.run(lamda, max_iter=20000, tol=1e-6, export_duals=True, accel=True, summarize=False)
+        t1 = time.perf_counter()
+        e.mlp_set(P.mlp_init(seed=0))
+        losses = e.mlp_train(epochs=epochs, seed=0)
         params = e.mlp_get()
-    dt = time.perf_counter() - t0
-    log(f"warm-start MLP trained on {tq.n * tc.N} pairs in {dt:.1f}s, epoch losses "
-        f"{np.array2string(losses, precision=3)}")
+    dt, dtt = time.perf_counter() - t0, time.perf_counter() - t1
+    log(f"warm-start network trained on {tq.n * tc.N} pairs in {dtt:.2f}s ({dt:.1f}s with the "
+        f"target solve), epoch losses {np.array2string(losses, precision=3)}")
     return {"params": params, "pairs": int(tq.n * tc.N), "epochs": epochs,
-            "epoch_loss": [float(x) for x in losses], "seconds": round(dt, 2)}
+            "epoch_loss": [float(x) for x in losses], "train_seconds": round(dtt, 3),
+            "seconds_with_targets": round(dt, 2)}
 
+
+# ---- CPU leg: the reference on this host's cores, and the FP64 checker ----
+
+def cpu_reference_timing(threads):
+    """BASELINE's CPU baseline: the reference's own fused and nonfused
+    sinkhorn_wmd (oracle/_ref) on the small and the reference-shape
+    workloads, 15 fixed iterations, median of 5 after a warm-up, `threads`
+    OpenMP threads bound close to cores -- in a subprocess so the OpenMP
+    settings are in place before its runtime starts."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    res = subprocess.run([sys.executable, os.path.join(REPO, "oracle", "time_reference.py"),
+                          str(threads), "5"], capture_output=True, text=True, env=env,
+                         timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError(f"reference timing failed: {res.stderr[-2000:]}")
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def cpu_port_per_pair(corpus, qs, lamda, tol, max_iter, threads, n_docs):
+    """The FP64 oracle's per-document form with the same per-pair stop rule
+    as the GPU, on the first query against n_docs documents."""
+    from oracle import pyoracle
+    k = 0
+    wk, vk = qs.query(k)
+    ptr = corpus.doc_ptr[:n_docs + 1]
+    b = int(ptr[-1])
+    t0 = time.perf_counter()
+    pyoracle.perdoc(wk, vk, corpus.vecs.astype(np.float64), ptr, corpus.doc_words[:b],
+                    corpus.doc_vals[:b], lamda, max_iter, tol=tol, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n_docs / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"query {k} (q={int(qs.lengths()[k])}) x first {n_docs} docs of the "
+                      f"batched workload, per-pair stop at residual {tol:g}, FP64, {dt:.2f}s"}
+
+
+def fp64_truth(corpus, qs, queries, lamda, threads, log_domain=False, n_docs=ERR_DOCS):
+    """Converged FP64 distances (tol 1e-10) of the sampled queries against the
+    first n_docs documents: the checker for err_vs_fp64."""
+    from oracle import pyoracle
+    b = int(corpus.doc_ptr[n_docs])
+    vecs = corpus.vecs.astype(np.float64)
+    out = {}
+    for k in queries:
+        out[k] = pyoracle.perdoc(*qs.query(k), vecs, corpus.doc_ptr[:n_docs + 1],
+                                 corpus.doc_words[:b], corpus.doc_vals[:b], lamda, 200000,
+                                 tol=1e-10, log_domain=log_domain, nthreads=threads)[0]
+    return out
+
+
+def max_rel(samples, truth):
+    return float(max(np.max(np.abs(samples[k] - truth[k]) / np.abs(truth[k])) for k in truth))
+
+
+# ---- GPU measurements besides the headline ----
 
 def hbm_copy(mib: int, reps: int = 5) -> int:
     """Device-to-device copies of a known size: the scale for the FETCH_SIZE and
@@ -178,14 +178,14 @@ def measured_traffic():
                 k, v = line.split(":", 1)
                 vals[k.strip()] = v.strip()
     try:
-        return {"bytes_per_stage": float(vals["bytes_per_stage"]), "source": TRAFFIC_FILE[len(REPO) + 1:],
-                "config": vals.get("config", "")}
+        return {"bytes_per_stage": float(vals["bytes_per_stage"]),
+                "source": TRAFFIC_FILE[len(REPO) + 1:], "config": vals.get("config", "")}
     except (KeyError, ValueError):
         return None
 
 
 def drop_in_rate(P, dg, reps: int = 3):
-    """The one-shot C entry (wmd_sinkhorn) on the plain workload: one 19-word
+    """The one-shot C entry (wmd_sinkhorn) on the reference shape: one 19-word
     query x 5,000 documents, 15 fixed iterations, host buffers in and out,
     device corpus rebuilt per call (PCIe-inclusive)."""
     corpus, qs, prm = dg.workload("plain", seed=0)
@@ -195,14 +195,14 @@ def drop_in_rate(P, dg, reps: int = 3):
     P.sinkhorn_wmd_fused(r, vecs, wm, lamda=prm["lamda"], max_iter=prm["max_iter"])  # warm
     t0 = time.perf_counter()
     for _ in range(reps):
-        P.sinkhorn_wmd_fused(r, vecs, wm, lamda=prm["lamda"], max_iter=prm["max_iter"])
+        w = P.sinkhorn_wmd_fused(r, vecs, wm, lamda=prm["lamda"], max_iter=prm["max_iter"])
     dt = (time.perf_counter() - t0) / reps
     return {"value": corpus.N / dt, "unit": "pairs/s", "ms_per_call": dt * 1e3,
-            "workload": f"plain: 1 query (q={int(qs.lengths()[0])}) x {corpus.N} docs, V={corpus.V}, "
-                        f"15 fixed iterations, FP64 host arrays in/out"}
+            "workload": f"plain: 1 query (q={int(qs.lengths()[0])}) x {corpus.N} docs, "
+                        f"V={corpus.V}, 15 fixed iterations, FP64 host arrays in/out"}, w
 
 
-def plain_shape(P, dg, reps: int = 5):
+def reference_shape(P, dg, reps: int = 5):
     """The reference's own shape (one 19-word query x 5,000 documents,
     V = 100,000, 15 fixed iterations) with the corpus resident: device time of
     the distance stage and of the solve per variant, and the sparse stage's
@@ -217,23 +217,26 @@ def plain_shape(P, dg, reps: int = 5):
     b_iter = 4.0 * (corpus.V + 1) + 8.0 * corpus.nnz + 4.0 * vt * q + 8.0 * corpus.N * q
     out = {"workload": f"plain: 1 query (q={q}) x {corpus.N} docs, V={corpus.V}, nnz {corpus.nnz}, "
                        f"15 fixed iterations, corpus resident",
-           "alg_bytes_per_iter": b_iter, "hbm_peak_gbs": 8000.0}
+           "alg_bytes_per_iter": b_iter, "hbm_peak_gbs": HBM_PEAK_GBS}
+    results = {}
     with P.WMDEngine(0) as e:
         e.set_corpus(corpus.vecs, corpus.word_major())
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
         for name, variant in (("resident", P.VARIANT_RESIDENT), ("fused", P.VARIANT_FUSED),
                               ("nonfused", P.VARIANT_NONFUSED)):
             e.run(prm["lamda"], max_iter=15, variant=variant, summarize=False)  # warm
-            cd = sv = 0.0
+            cd, sv = [], []
             for _ in range(reps):
                 st = e.run(prm["lamda"], max_iter=15, variant=variant, summarize=False)
-                cd += st["ms_cdist"]
-                sv += st["ms_solve"]
-            cd, sv = cd / reps, sv / reps
-            gbs = 15 * b_iter / (sv * 1e-3) / 1e9
-            out[name] = {"ms_cdist": cd, "ms_solve": sv, "pairs_per_s": corpus.N / ((cd + sv) * 1e-3),
-                         "alg_gbs": gbs, "hbm_frac": gbs / 8000.0}
-    return out
+                cd.append(st["ms_cdist"])
+                sv.append(st["ms_solve"])
+            cdm, svm = float(np.median(cd)), float(np.median(sv))
+            gbs = 15 * b_iter / (svm * 1e-3) / 1e9
+            out[name] = {"ms_cdist": cdm, "ms_solve": svm,
+                         "pairs_per_s": corpus.N / ((cdm + svm) * 1e-3),
+                         "alg_gbs": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+            results[name] = e.result()[0][0].copy()
+    return out, results
 
 
 def precision_shape(P, dg, reps: int = 5):
@@ -245,21 +248,50 @@ def precision_shape(P, dg, reps: int = 5):
     q = int(qs.lengths()[0])
     out = {"workload": f"precision: 1 query (q={q}, near-duplicate words) x {corpus.N} docs, "
                        f"lambda {abs(prm['lamda']):g}, log domain, residual {prm['tol']:g}"}
+    results = {}
     with P.WMDEngine(0) as e:
         e.set_corpus(corpus.vecs, corpus.word_major())
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
         for name, accel in (("plain", False), ("anderson", True)):
             kw = dict(max_iter=prm["max_iter"], tol=prm["tol"], log_domain=True, accel=accel)
             e.run(prm["lamda"], **kw)  # warm
-            cd = sv = 0.0
+            cd, sv = [], []
             for _ in range(reps):
                 st = e.run(prm["lamda"], **kw)
-                cd += st["ms_cdist"]
-                sv += st["ms_solve"]
-            cd, sv = cd / reps, sv / reps
-            out[name] = {"ms_cdist": cd, "ms_solve": sv, "pairs_per_s": corpus.N / ((cd + sv) * 1e-3),
-                         "iters_mean": float(st["iters_mean"]), "iters_max": int(st["iters_max"])}
-    return out
+                cd.append(st["ms_cdist"])
+                sv.append(st["ms_solve"])
+            cdm, svm = float(np.median(cd)), float(np.median(sv))
+            w, it, res = e.result()
+            assert np.all(res < prm["tol"]), f"precision {name}: a pair did not reach the residual"
+            out[name] = {"ms_cdist": cdm, "ms_solve": svm,
+                         "pairs_per_s": corpus.N / ((cdm + svm) * 1e-3),
+                         "iters_mean": float(it.mean()), "iters_max": int(it.max())}
+            results[name] = w[0, :ERR_DOCS].copy()
+    return out, results, (corpus, qs, prm)
+
+
+def plain_gpu(P, eng, qs, lamda, max_iter, tol, nq):
+    """The plain GPU method: the reference's fused SDDMM+SpMM iteration, one
+    launch per iteration from a captured graph, uniform start, no
+    acceleration, one query per call, the first nq queries of the batch."""
+    tp = 0.0
+    pairs = 0
+    for k in range(nq):
+        wk, vk = qs.query(k)
+        eng.set_queries(np.array([0, wk.shape[0]]), wk, vk)
+        kw = dict(max_iter=max_iter, tol=tol, variant=P.VARIANT_FUSED, check_every=4,
+                  summarize=False)
+        eng.run(lamda, **kw)  # graphs built, caches warm
+        t1 = time.perf_counter()
+        eng.run(lamda, **kw)
+        tp += time.perf_counter() - t1
+        pairs += eng.N
+        res = eng.result()[2]
+        assert np.all(res < tol), "plain GPU method: a pair did not reach the residual"
+    return {"value": pairs / tp, "unit": "pairs/s", "queries": nq, "seconds": round(tp, 2),
+            "method": "one query per call, fused SDDMM+SpMM launch per iteration from a "
+                      "captured graph, uniform start, plain steps, convergence checked every 4 "
+                      "iterations"}
 
 
 def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") -> int:
@@ -296,23 +328,21 @@ def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") ->
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--queries", type=int, default=256)
     ap.add_argument("--docs", type=int, default=50000)
     ap.add_argument("--tol", type=float, default=1e-4)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--plain-queries", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true",
+                    help="skip the CPU leg (reference timing, FP64 port, error checks)")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU leg (one GPU's share of the box)")
+    ap.add_argument("--plain-queries", type=int, default=256)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--cdist", choices=("mfma", "fp32"), default="mfma")
-    ap.add_argument("--omega", type=float, default=1.5,
-                    help="over-relaxation of the u-update (1 = plain Sinkhorn)")
-    ap.add_argument("--warm", choices=("uniform", "learned"), default="uniform")
+    ap.add_argument("--warm", choices=("uniform", "learned"), default="learned")
     ap.add_argument("--accel", choices=("none", "anderson"), default="anderson",
-                    help="u-update acceleration (anderson overrides --omega)")
-    # 20k training pairs over many distinct queries (the network has to
-    # generalise across queries more than across documents)
+                    help="u-update acceleration")
     ap.add_argument("--train-queries", type=int, default=1000)
     ap.add_argument("--train-docs", type=int, default=20)
     ap.add_argument("--no-variants", action="store_true")
@@ -345,10 +375,11 @@ def main() -> int:
     lamda, max_iter = prm["lamda"], prm["max_iter"]
     log(f"workload: {qs.n} queries (sum q {int(qs.lengths().sum())}) x {corpus.N} docs, "
         f"nnz {corpus.nnz}, generated in {time.perf_counter() - t0:.1f}s")
+    err_q = sample_queries(qs)
+    n_err = min(ERR_DOCS, corpus.N)
 
     learned = None
-    need_mlp = args.warm == "learned" or not args.no_variants
-    if need_mlp:
+    if args.warm == "learned" or not args.no_variants:
         learned = train_warm_start(P, dg, corpus.vecs, lamda, args.train_queries, args.train_docs)
 
     eng = P.WMDEngine(0)
@@ -356,25 +387,23 @@ def main() -> int:
     eng.set_queries(qs.q_ptr, qs.words, qs.weights)
     if learned is not None:
         eng.mlp_set(learned["params"])
-    log("corpus and queries resident on the GPU")
+    log("corpus, queries and network resident on the GPU")
     tune_ms = tune_changed = None
     if not args.no_autotune:
         default_table = eng.get_tuning()
-        tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma,
-                               accel=args.accel == "anderson", omega=args.omega)
+        tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma, accel=args.accel == "anderson")
         tune_changed = int((eng.get_tuning() != default_table).sum())
         log(f"tile shapes autotuned in {tune_ms:.0f} ms ({tune_changed} table entries differ "
             f"from the smallest-fitting shape)")
-    kw = dict(max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=args.omega,
+    kw = dict(max_iter=max_iter, tol=args.tol, mfma_cdist=mfma,
               warm_start=args.warm == "learned", accel=args.accel == "anderson",
               graph=not args.no_graph)
 
-    warm = None
     for w in range(max(1, args.warmup)):
-        warm = eng.run(lamda, summarize=True, **kw)
-        log(f"warmup {w}: {warm['ms_total']:.2f} ms (cdist {warm['ms_cdist']:.2f}, warm start "
-            f"{warm['ms_warm']:.2f}, solve {warm['ms_solve']:.2f}), iters mean "
-            f"{warm['iters_mean']:.2f} max {warm['iters_max']}")
+        st = eng.run(lamda, summarize=True, **kw)
+        log(f"warmup {w}: {st['ms_total']:.2f} ms (cdist {st['ms_cdist']:.2f}, warm start "
+            f"{st['ms_warm']:.2f}, solve {st['ms_solve']:.2f}), iters mean "
+            f"{st['iters_mean']:.2f} max {st['iters_max']}, fallbacks {st['fallbacks']}")
     cd = sv = wm = 0.0
     t_start = time.perf_counter()
     for s in range(args.steps):
@@ -391,18 +420,20 @@ def main() -> int:
 
     w_out, iters, resid = eng.result()
     assert np.all(resid < args.tol), "a pair did not reach the residual"
+    samples = {"headline": {k: w_out[k, :n_err].copy() for k in err_q}}
     flops = solver_flops(qs.lengths(), corpus.doc_lengths(), iters)
     solve_ms = sv / args.steps
     achieved = flops / (solve_ms * 1e-3) / 1e12
     vt = int(np.unique(corpus.doc_words).shape[0])
-    it_q = iters.mean(axis=1)
-    bytes_alg = sinkhorn_bytes(corpus.V, corpus.nnz, vt, qs.lengths(), corpus.N, it_q)
+    bytes_alg = sinkhorn_bytes(corpus.V, corpus.nnz, vt, qs.lengths(), corpus.N,
+                               iters.mean(axis=1))
+    alg_gbs = bytes_alg / (solve_ms * 1e-3) / 1e9
 
     cd_ms = cd / args.steps
     dp = (corpus.D + 15) // 16 * 16
     q_len = qs.lengths()
-    shared = bool(warm and warm.get("shared"))
-    n_cols = int(warm["distinct"]) if shared else int(q_len.sum())
+    shared = bool(st.get("shared"))
+    n_cols = int(st["distinct"]) if shared else int(q_len.sum())
     # one FMA per component; the shared form computes each distinct query word once
     cdist_alg = 2.0 * vt * n_cols * corpus.D
     tiles = (-(-n_cols // 64) * 64) if shared else float((np.ceil(q_len / 64.0) * 64).sum())
@@ -420,110 +451,142 @@ def main() -> int:
                                   "per-query rescale included)"})
     variants = {}
     if not args.no_variants:
-        # Each configuration: one untimed run, one timed run (device events).
+        # Each configuration: one untimed run, then the median of three
+        # event-timed runs; every pair must reach the residual.
         for name, extra in (
-                ("batched, uniform start, omega 1", dict(omega=1.0, warm_start=False, accel=False)),
-                ("batched, uniform start, omega 1.5", dict(omega=1.5, warm_start=False, accel=False)),
+                ("batched, uniform start, plain steps", dict(warm_start=False, accel=False)),
                 ("batched, uniform start, Anderson", dict(warm_start=False, accel=True)),
-                ("batched, learned start, omega 1", dict(omega=1.0, warm_start=True, accel=False)),
-                ("batched, learned start, omega 1.5", dict(omega=1.5, warm_start=True, accel=False)),
+                ("batched, learned start, plain steps", dict(warm_start=True, accel=False)),
                 ("batched, learned start, Anderson", dict(warm_start=True, accel=True))):
+            if extra["warm_start"] and learned is None:
+                continue
             k2 = dict(kw, **extra)
             eng.run(lamda, summarize=False, **k2)
-            t1 = time.perf_counter()
-            stv = eng.run(lamda, summarize=True, **k2)
-            dt = time.perf_counter() - t1
+            runs = [eng.run(lamda, summarize=True, **k2) for _ in range(3)]
+            stv = sorted(runs, key=lambda r: r["ms_total"])[1]
+            wv, itv, rv = eng.result()
+            assert np.all(rv < args.tol), f"variant {name}: a pair did not reach the residual"
+            samples[name] = {k: wv[k, :n_err].copy() for k in err_q}
             variants[name] = {"pairs_per_s": pairs / stv["ms_total"] * 1e3,
-                              "ms_total": stv["ms_total"], "ms_warm_start": stv["ms_warm"],
-                              "ms_solve": stv["ms_solve"], "iters_mean": stv["iters_mean"],
-                              "iters_max": stv["iters_max"], "fallbacks": stv["fallbacks"]}
-            log(f"variant {name}: {variants[name]['pairs_per_s']:.4g} pairs/s, iters mean "
+                              "ms_total": stv["ms_total"], "ms_cdist": stv["ms_cdist"],
+                              "ms_warm_start": stv["ms_warm"], "ms_solve": stv["ms_solve"],
+                              "iters_mean": stv["iters_mean"], "iters_max": stv["iters_max"],
+                              "fallbacks": stv["fallbacks"], "resid_max": float(rv.max())}
+            log(f"variant {name}: {variants[name]['pairs_per_s']:.4g} pairs/s "
+                f"(warm {stv['ms_warm']:.1f} + solve {stv['ms_solve']:.1f} ms), iters mean "
                 f"{stv['iters_mean']:.2f} max {stv['iters_max']}, fallbacks {stv['fallbacks']}")
         eng.run(lamda, summarize=False, **kw)  # leave the timed configuration's state
 
     plain = None
     if not args.no_plain and args.plain_queries > 0:
         nq = min(args.plain_queries, qs.n)
-        tp = 0.0
-        for k in range(nq):
-            wk, vk = qs.query(k)
-            eng.set_queries(np.array([0, wk.shape[0]]), wk, vk)
-            eng.run(lamda, max_iter=max_iter, tol=args.tol, variant=P.VARIANT_FUSED,
-                    check_every=4, summarize=False, mfma_cdist=mfma)  # graphs built, caches warm
-            t1 = time.perf_counter()
-            eng.run(lamda, max_iter=max_iter, tol=args.tol, variant=P.VARIANT_FUSED,
-                    check_every=4, summarize=False, mfma_cdist=mfma)
-            tp += time.perf_counter() - t1
-        plain = {"value": nq * corpus.N / tp, "unit": "pairs/s",
-                 "method": "one query per call, fused SDDMM+SpMM launch per iteration from a "
-                           "captured graph, uniform start, convergence checked every 4 iterations",
-                 "queries": nq}
+        plain = plain_gpu(P, eng, qs, lamda, max_iter, args.tol, nq)
         plain["speedup_of_value"] = value / plain["value"]
-        log(f"plain GPU method: {plain['value']:.4g} pairs/s on {nq} queries")
+        log(f"plain GPU method: {plain['value']:.4g} pairs/s on {nq} queries "
+            f"({plain['seconds']:.1f}s); headline is {plain['speedup_of_value']:.1f}x")
     eng.close()
     traffic = measured_traffic()
-    drop_in = plain_cfg = prec_cfg = None
+    drop_in = ref_cfg = prec_cfg = None
+    ref_results = prec_results = prec_data = drop_w = None
     if not args.no_plain:
-        drop_in = drop_in_rate(P, dg)
+        drop_in, drop_w = drop_in_rate(P, dg)
         log(f"drop-in entry (host buffers, PCIe-inclusive): {drop_in['value']:.4g} pairs/s, "
             f"{drop_in['ms_per_call']:.1f} ms per call")
-        plain_cfg = plain_shape(P, dg)
+        ref_cfg, ref_results = reference_shape(P, dg)
         log("reference shape, corpus resident: " + ", ".join(
-            f"{k} {plain_cfg[k]['ms_solve']:.3f} ms solve ({plain_cfg[k]['alg_gbs']:.0f} GB/s alg.)"
+            f"{k} {ref_cfg[k]['ms_solve']:.3f} ms solve ({ref_cfg[k]['alg_gbs']:.0f} GB/s alg.)"
             for k in ("resident", "fused", "nonfused")))
-        prec_cfg = precision_shape(P, dg)
+        prec_cfg, prec_results, prec_data = precision_shape(P, dg)
         log("precision shape (lambda 50, log domain): " + ", ".join(
             f"{k} {prec_cfg[k]['ms_solve']:.3f} ms solve, {prec_cfg[k]['iters_mean']:.1f} iterations"
             for k in ("plain", "anderson")))
 
-    cpu = cpu_port = None
+    cpu = cpu_port = err_headline = None
     if not args.no_cpu_baseline:
-        # The reference iterates plain Sinkhorn: size its fixed iteration count
-        # from a uniform-start, omega = 1 solve of the same pairs.
-        eng2 = P.WMDEngine(0)
-        eng2.set_corpus(corpus.vecs, corpus.word_major())
-        wq, vq = qs.query(0)
-        eng2.set_queries(np.array([0, wq.shape[0]]), wq, vq)
-        eng2.run(lamda, max_iter=max_iter, tol=args.tol, mfma_cdist=mfma, omega=1.0)
-        iters_plain = eng2.result()[1]
-        eng2.close()
-        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-        cpu = cpu_baseline(corpus, qs, iters_plain, lamda, args.cpu_seconds, threads)
-        log(f"cpu baseline ({cpu['kind']}, {cpu['cores']} threads): {cpu['value']:.4g} pairs/s")
+        threads = args.cpu_threads
+        t0 = time.perf_counter()
+        ref = cpu_reference_timing(threads)
+        c1 = ref["cases"]["plain/fused"]
+        cpu = {"value": c1["pairs_per_s"], "unit": "pairs/s", "cores": threads,
+               "kind": ref["kind"],
+               "sample": (f"the reference's fused sinkhorn_wmd on its own shape: 1 query "
+                          f"(q={c1['q']}) x {c1['pairs']} docs, V={c1['V']}, 15 fixed "
+                          f"iterations (the reference has no residual test), FP64, median of "
+                          f"{ref['runs']} after a warm-up, {threads} threads "
+                          f"OMP_PROC_BIND=close OMP_PLACES=cores, {ref['cpu']}"),
+               "median_s": c1["median_s"], "cases": ref["cases"]}
+        log(f"cpu baseline ({cpu['kind']}, {threads} threads): fused {c1['pairs_per_s']:.4g} "
+            f"pairs/s ({c1['median_s'] * 1e3:.1f} ms), nonfused "
+            f"{ref['cases']['plain/nonfused']['pairs_per_s']:.4g} pairs/s")
         cpu_port = cpu_port_per_pair(corpus, qs, lamda, args.tol, max_iter, threads,
                                      min(corpus.N, 5000))
-        log(f"cpu port per-pair stop: {cpu_port['value']:.4g} pairs/s")
+        log(f"cpu FP64 port with the per-pair stop: {cpu_port['value']:.4g} pairs/s")
+        # The checker: converged FP64 distances of the sampled pairs.
+        truth = fp64_truth(corpus, qs, err_q, lamda, threads, n_docs=n_err)
+        desc = (f"queries {err_q} (q = {[int(q_len[k]) for k in err_q]}) x first {n_err} docs, "
+                f"max relative error vs converged FP64 (tol 1e-10)")
+        err_headline = {"max_rel": max_rel(samples["headline"], truth), "sample": desc}
+        for name in variants:
+            variants[name]["err_vs_fp64"] = max_rel(samples[name], truth)
+        if prec_cfg is not None:
+            pc, pq, pp = prec_data
+            ptruth = fp64_truth(pc, pq, [0], pp["lamda"], threads, log_domain=True)
+            for name in ("plain", "anderson"):
+                prec_cfg[name]["err_vs_fp64"] = max_rel({0: prec_results[name]}, ptruth)
+            prec_cfg["err_sample"] = (f"first {ERR_DOCS} docs vs converged FP64 log-domain "
+                                      f"(tol 1e-10)")
+        if ref_cfg is not None:
+            # 15 fixed iterations: the reference's own FP64 output is the truth.
+            pc, pq, pp = dg.workload("plain", seed=0)
+            from oracle import pyoracle
+            rf = (pyoracle.ref_fused if pyoracle.ref_available() else pyoracle.fused)(
+                pq.dense_r(0, pc.V), pc.vecs.astype(np.float64), pc.word_major(), pp["lamda"], 15,
+                nthreads=threads)
+            for name in ("resident", "fused", "nonfused"):
+                ref_cfg[name]["err_vs_fp64"] = float(np.max(np.abs(ref_results[name] - rf) /
+                                                            np.abs(rf)))
+            drop_in["err_vs_fp64"] = float(np.max(np.abs(drop_w - rf) / np.abs(rf)))
+            ref_cfg["err_sample"] = ("all 5000 docs vs the reference's fused sinkhorn_wmd in FP64 "
+                                     "(15 iterations)")
+        log(f"error checks done in {time.perf_counter() - t0:.1f}s: headline "
+            f"{err_headline['max_rel']:.2e}")
 
     line = {
         "metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {
-            "workload": (f"batched: {qs.n} queries of 5-300 distinct Zipf(1.1) words "
-                         f"(lengths lognormal(3.5,0.8)) x {corpus.N} docs of 40-150 words, "
-                         f"V={corpus.V}, D={corpus.D} FP32, lambda={-lamda:g}, per-pair stop at "
-                         f"marginal residual {args.tol:g}"),
+            "workload": (f"batched, {args.warm} start: {qs.n} queries of 5-300 distinct "
+                         f"Zipf(1.1) words (lengths lognormal(3.5,0.8)) x {corpus.N} docs of "
+                         f"40-150 words, V={corpus.V}, D={corpus.D} FP32, lambda={-lamda:g}, "
+                         f"per-pair stop at marginal residual {args.tol:g}"),
             "queries": qs.n, "docs": corpus.N, "nnz": corpus.nnz,
-            "sum_query_words": int(qs.lengths().sum()), "warm_start": args.warm,
-            "omega": args.omega, "accel": args.accel, "variant": "resident",
-            "autotune_ms": tune_ms,
-            "autotune_entries_changed": tune_changed,
-            "iters_mean": float(iters.mean()),
-            "iters_max": int(iters.max()),
+            "sum_query_words": int(q_len.sum()), "warm_start": args.warm,
+            "accel": args.accel, "variant": "resident",
+            "autotune_ms": tune_ms, "autotune_entries_changed": tune_changed,
+            "iters_mean": float(iters.mean()), "iters_max": int(iters.max()),
         },
+        "err_vs_fp64": err_headline,
+        "speedup_vs_plain_gpu": None if plain is None else plain["speedup_of_value"],
         "roofline": {
-            "bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
+            "bound": "valu", "achieved": achieved, "peak": VALU_FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / VALU_FP32_PEAK_TFLOPS,
             "traffic": None if traffic is None else traffic["bytes_per_stage"],
             "traffic_source": traffic,
             "kernel": "k_resident (solver stage: every k_resident launch of a step, concurrent "
                       "on side streams, device time from HIP events around the fork/join)",
-            "engine": "FP32 VALU FMA (packed FP32 disabled: 78.6 TFLOP/s issue ceiling)",
+            "engine": "FP32 VALU FMA; packed FP32 disabled for all device code, so the issue "
+                      "ceiling is 78.6 TFLOP/s (half the 157.3 TFLOP/s FP32 figure)",
             "flops_per_step": flops, "ms_per_step": solve_ms,
-            "sinkhorn_alg_bytes_per_step": bytes_alg,
-            "sinkhorn_alg_gbs": bytes_alg / (solve_ms * 1e-3) / 1e9,
         },
-        "stages_ms": {"cdist": cd / args.steps, "warm_start": wm / args.steps, "solve": solve_ms},
+        "roofline_hbm": {
+            "bound": "hbm", "achieved": alg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": alg_gbs / HBM_PEAK_GBS, "bytes_per_step": bytes_alg,
+            "formula": "sum over queries of (4(V+1) + 8 nnz + 4 Vt q + 8 N q) x the query's mean "
+                       "iterations, over the solver stage's time (the sparse stages' HBM bound "
+                       "if every iteration streamed the kernel rows)",
+        },
+        "stages_ms": {"cdist": cd_ms, "warm_start": wm / args.steps, "solve": solve_ms},
         "variants": variants,
         "learned_training": None if learned is None else
         {k: v for k, v in learned.items() if k != "params"},
@@ -532,7 +595,7 @@ def main() -> int:
         "cpu_port_per_pair_stop": cpu_port,
         "plain_gpu": plain,
         "drop_in": drop_in,
-        "reference_shape": plain_cfg,
+        "reference_shape": ref_cfg,
         "precision_shape": prec_cfg,
     }
     print(json.dumps(line), flush=True)

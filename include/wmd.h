@@ -231,36 +231,55 @@ const double* wmd_context_device_result(const wmd_context* ctx);
  * without a distance (NaN) after every other.  1 <= k <= N. */
 int wmd_context_topk(wmd_context* ctx, int32_t k, int32_t* doc_ids, double* wmd);
 
-/* ---- Learned warm start (no reference counterpart).  A 3D -> 256 -> 256 -> 1
- * MLP with tanh-form GELU hidden layers predicts, for every (query, document
- * word), the converged log v_e from [word embedding; document mean embedding;
- * query mean embedding] (means weighted by c and r).  Parameters are one flat
- * FP32 array: W1 [256][3D] row-major, b1 [256], W2 [256][256], b2 [256],
- * w3 [256], b3 [1].  The targets are centred per (query, document) pair: the
- * scaling pair (a u, v / a) gives the same plan, so only differences of log v
- * within a document matter. */
-#define WMD_MLP_HIDDEN 256
+/* ---- Learned warm start (no reference counterpart; it replaces the
+ * reference's uniform start x = 1/q, sinkhorn_wmd_fused.cpp:188-191).  A
+ * 15 -> 16 -> 16 -> 1 MLP with tanh-form GELU hidden layers predicts, for
+ * every (query, document word), the converged log v_e (row-rescaled kernel
+ * convention) from norm features of the word's embedding, a document summary
+ * and a query summary (feature k):
+ *   0, 1    the word's embedding norm n and n^2
+ *   2..6    document: c-weighted mean, std, min, max of its words' norms, log n_j
+ *   7..11   query: r-weighted mean, std, min, max of its words' norms, log q
+ *   12      the word's c-weighted mid-rank by norm within its document (cdf)
+ *   13      T: the query's r-weighted norm quantile at cdf (numpy.interp over
+ *           the query's sorted norms and their mid-ranks)
+ *   14      |lamda| (sqrt(T^2 + n^2) - min_i M[word, i])
+ * Parameters are one flat FP32 array: xm[15], xs[15] (inputs enter as
+ * (x - xm) * xs), W1 [16][15] row-major, b1 [16], W2 [16][16], b2 [16],
+ * w3 [16], b3 [1].  Targets are centred per (query, document) pair: the
+ * scaling pair (a u, v / a) gives the same plan, so only differences of
+ * log v within a document matter. */
+#define WMD_MLP_HIDDEN 16
+#define WMD_MLP_FEATURES 15
 
-uint64_t wmd_mlp_param_count(uint64_t D);
+uint64_t wmd_mlp_param_count(void);
 int wmd_context_mlp_set(wmd_context* ctx, const float* params, uint64_t n_params);
 int wmd_context_mlp_get(wmd_context* ctx, float* params, uint64_t n_params);
 
 typedef struct wmd_train_options {
   int32_t epochs;          /* passes over every (query, document) pair */
-  int32_t pairs_per_batch; /* pairs per Adam step */
+  int32_t pairs_per_batch; /* pairs per Adam step (at most 4096) */
   double lr;               /* Adam learning rate (beta 0.9 / 0.999, eps 1e-8) */
   double weight_decay;     /* L2 coefficient added to the gradient */
   uint64_t seed;           /* pair shuffling */
-  int32_t reserved[8];
+  int32_t standardize;     /* 1: set xm / xs to the training samples' per-feature
+                              mean and 1 / std before the first step */
+  int32_t reserved[7];
 } wmd_train_options;
 
 void wmd_train_options_default(wmd_train_options* opt);
 
 /* Train the context's MLP on the GPU against the duals of the last run that
  * exported them (same corpus and queries): every (query, document) pair of
- * the current batch is one training group.  epoch_loss[epochs] (may be NULL)
- * receives the mean centred squared error of each epoch. */
+ * the current batch is one training group; one fused forward/backward kernel
+ * and one Adam kernel per step.  epoch_loss[epochs] (may be NULL) receives
+ * the mean centred squared error of each epoch. */
 int wmd_context_mlp_train(wmd_context* ctx, const wmd_train_options* opt, double* epoch_loss);
+
+/* The 15 features above for every (query, nonzero) of the current batch, as
+ * the last run's row minima give them: out[n_queries][nnz][15], nonzeros in
+ * the document-major order of wmd_context_duals. */
+int wmd_context_warm_features(wmd_context* ctx, float* out);
 
 /* Caller-supplied starting log v (n_queries x nnz, the layout of
  * wmd_context_duals) for runs with warm_start = WMD_WARM_GIVEN. */
@@ -270,12 +289,14 @@ int wmd_context_set_warm(wmd_context* ctx, const float* log_v);
  * document-major nonzero order (documents ascending, words ascending). */
 int wmd_context_duals(wmd_context* ctx, float* out);
 
-/* The training step's loss and gradient on the GPU for n samples X [n][3D]
- * with targets y [n] in n_groups contiguous groups (group_ptr [n_groups + 1]);
- * params and grad use the flat layout above.  Stateless (device `device`). */
-int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const float* y,
-                      uint64_t n_groups, const int32_t* group_ptr, const float* params,
-                      double* loss, float* grad);
+/* The training step's loss and gradient on the GPU for n samples of raw
+ * features X [n][15] with targets y [n] in n_groups contiguous groups
+ * (group_ptr [n_groups + 1]): loss = sum over groups of the squared
+ * deviations of (prediction - y) from their group mean, over n; params and
+ * grad use the flat layout above (xm / xs get zero gradient).  Stateless
+ * (device `device`). */
+int wmd_mlp_loss_grad(int device, uint64_t n, const float* X, const float* y, uint64_t n_groups,
+                      const int32_t* group_ptr, const float* params, double* loss, float* grad);
 
 /* Number of resident tile shapes and their limits (for tuners/tests). */
 int wmd_num_shapes(void);

@@ -38,7 +38,8 @@ ACCEL_ANDERSON = 1
 SHARE_AUTO = 0
 SHARE_OFF = 1
 SHARE_ON = 2
-MLP_HIDDEN = 256
+MLP_HIDDEN = 16
+MLP_FEATURES = 15
 
 # Every symbol include/wmd.h declares.
 EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
@@ -48,7 +49,8 @@ EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_
            "wmd_last_error",
            "wmd_mlp_param_count", "wmd_context_mlp_set", "wmd_context_mlp_get",
            "wmd_train_options_default", "wmd_context_mlp_train", "wmd_context_duals",
-           "wmd_mlp_loss_grad", "wmd_context_set_warm", "wmd_context_autotune",
+           "wmd_mlp_loss_grad", "wmd_context_set_warm", "wmd_context_warm_features",
+           "wmd_context_autotune",
            "wmd_tuning_dims", "wmd_context_get_tuning", "wmd_context_set_tuning",
            "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims", "wmd_read_vecs", "wmd_read_r",
            "wmd_write_scores", "wmd_write_mtx", "wmd_write_vecs", "wmd_write_r",
@@ -81,7 +83,8 @@ class Stats(C.Structure):
 
 class TrainOptions(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("pairs_per_batch", C.c_int32), ("lr", C.c_double),
-                ("weight_decay", C.c_double), ("seed", C.c_uint64), ("reserved", C.c_int32 * 8)]
+                ("weight_decay", C.c_double), ("seed", C.c_uint64), ("standardize", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
 
 
 class WMDError(RuntimeError):
@@ -125,7 +128,7 @@ def load():
                                  C.POINTER(C.c_int32)]
     L.wmd_last_error.argtypes = []
     L.wmd_last_error.restype = C.c_char_p
-    L.wmd_mlp_param_count.argtypes = [U64]
+    L.wmd_mlp_param_count.argtypes = []
     L.wmd_mlp_param_count.restype = C.c_uint64
     L.wmd_context_mlp_set.argtypes = [P, P, U64]
     L.wmd_context_mlp_get.argtypes = [P, P, U64]
@@ -134,7 +137,8 @@ def load():
     L.wmd_context_mlp_train.argtypes = [P, C.POINTER(TrainOptions), P]
     L.wmd_context_duals.argtypes = [P, P]
     L.wmd_context_set_warm.argtypes = [P, P]
-    L.wmd_mlp_loss_grad.argtypes = [C.c_int, U64, U64, P, P, U64, P, P, P, P]
+    L.wmd_mlp_loss_grad.argtypes = [C.c_int, U64, P, P, U64, P, P, P, P]
+    L.wmd_context_warm_features.argtypes = [P, P]
     L.wmd_context_autotune.argtypes = [P, C.c_double, C.POINTER(Options), C.c_int32,
                                        C.POINTER(C.c_double)]
     L.wmd_tuning_dims.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -165,6 +169,7 @@ def load():
                  "wmd_shape_domains",
                  "wmd_context_mlp_set", "wmd_context_mlp_get", "wmd_context_mlp_train",
                  "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm",
+                 "wmd_context_warm_features",
                  "wmd_context_autotune", "wmd_tuning_dims", "wmd_context_get_tuning",
                  "wmd_context_set_tuning", "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims",
                  "wmd_read_vecs", "wmd_read_r", "wmd_write_scores", "wmd_write_mtx",

@@ -200,22 +200,32 @@ class WMDEngine:
                   "wmd_context_mlp_set")
 
     def mlp_get(self):
-        n = mlp_param_count(self.D)
+        n = mlp_param_count()
         out = np.empty(n, dtype=np.float32)
         nat.check(self._L.wmd_context_mlp_get(self._h, out.ctypes.data, n), "wmd_context_mlp_get")
         return out
 
-    def mlp_train(self, epochs=4, pairs_per_batch=32, lr=1e-3, weight_decay=0.0, seed=0):
+    def mlp_train(self, epochs=6, pairs_per_batch=64, lr=3e-3, weight_decay=0.0, seed=0,
+                  standardize=True):
         """Train on the duals of the last run with export_duals; returns the
         per-epoch mean centred squared error."""
         o = nat.TrainOptions()
         self._L.wmd_train_options_default(C.byref(o))
         o.epochs, o.pairs_per_batch, o.lr = int(epochs), int(pairs_per_batch), float(lr)
         o.weight_decay, o.seed = float(weight_decay), int(seed)
+        o.standardize = int(bool(standardize))
         losses = np.zeros(max(int(epochs), 1), dtype=np.float64)
         nat.check(self._L.wmd_context_mlp_train(self._h, C.byref(o), losses.ctypes.data),
                   "wmd_context_mlp_train")
         return losses[:int(epochs)]
+
+    def warm_features(self, nnz):
+        """[n_queries, nnz, 15] warm-start features of the current batch (the
+        last run's row minima)."""
+        out = np.empty((self.n_queries, int(nnz), nat.MLP_FEATURES), dtype=np.float32)
+        nat.check(self._L.wmd_context_warm_features(self._h, out.ctypes.data),
+                  "wmd_context_warm_features")
+        return out
 
     def set_warm(self, log_v):
         log_v = np.ascontiguousarray(log_v, dtype=np.float32)
@@ -239,14 +249,16 @@ class WMDEngine:
         return w.reshape(shp), it.reshape(shp), rs.reshape(shp)
 
 
-def mlp_param_count(D):
-    return int(nat.load().wmd_mlp_param_count(int(D)))
+def mlp_param_count():
+    return int(nat.load().wmd_mlp_param_count())
 
 
-def mlp_layout(D):
-    """Offsets of W1 [256, 3D], b1, W2 [256, 256], b2, w3, b3 in the flat parameters."""
-    H, F = nat.MLP_HIDDEN, 3 * int(D)
-    off = {"W1": 0, "b1": H * F}
+def mlp_layout():
+    """Offsets of xm [15], xs [15], W1 [16, 15], b1, W2 [16, 16], b2, w3, b3 in
+    the flat parameters."""
+    H, F = nat.MLP_HIDDEN, nat.MLP_FEATURES
+    off = {"xm": 0, "xs": F, "W1": 2 * F}
+    off["b1"] = off["W1"] + H * F
     off["W2"] = off["b1"] + H
     off["b2"] = off["W2"] + H * H
     off["w3"] = off["b2"] + H
@@ -255,12 +267,14 @@ def mlp_layout(D):
     return off
 
 
-def mlp_init(D, seed=0):
-    """He-normal weights, zero biases, small output layer."""
+def mlp_init(seed=0):
+    """Identity standardisation, He-normal weights, zero biases, small output
+    layer."""
     rng = np.random.default_rng(seed)
-    H, F = nat.MLP_HIDDEN, 3 * int(D)
-    L = mlp_layout(D)
+    H, F = nat.MLP_HIDDEN, nat.MLP_FEATURES
+    L = mlp_layout()
     p = np.zeros(L["total"], dtype=np.float32)
+    p[L["xs"]:L["W1"]] = 1.0
     p[L["W1"]:L["b1"]] = rng.normal(0, np.sqrt(2.0 / F), H * F)
     p[L["W2"]:L["b2"]] = rng.normal(0, np.sqrt(2.0 / H), H * H)
     p[L["w3"]:L["b3"]] = rng.normal(0, np.sqrt(1.0 / H), H)
@@ -269,15 +283,17 @@ def mlp_init(D, seed=0):
 
 def mlp_loss_grad(X, y, group_ptr, params, device=0):
     """Centred squared-error loss and its gradient, computed by the GPU
-    training kernels (stateless)."""
+    training kernel (stateless); X is [n, 15] raw features."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     group_ptr = np.ascontiguousarray(group_ptr, dtype=np.int32)
     params = np.ascontiguousarray(params, dtype=np.float32)
     n, F = X.shape
+    if F != nat.MLP_FEATURES:
+        raise ValueError(f"X must have {nat.MLP_FEATURES} columns")
     grad = np.empty_like(params)
     loss = C.c_double()
-    nat.check(nat.load().wmd_mlp_loss_grad(int(device), n, F // 3, X.ctypes.data, y.ctypes.data,
+    nat.check(nat.load().wmd_mlp_loss_grad(int(device), n, X.ctypes.data, y.ctypes.data,
                                            group_ptr.shape[0] - 1, group_ptr.ctypes.data,
                                            params.ctypes.data, C.byref(loss), grad.ctypes.data),
               "wmd_mlp_loss_grad")

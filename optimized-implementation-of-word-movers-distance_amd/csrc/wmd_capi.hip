@@ -207,14 +207,14 @@ struct wmd_context {
   int n_cu = 0;
   std::vector<int32_t> h_doc_ptr, h_doc_t, h_touched;  // host copies for training batches
   DevBuf<int32_t> doc_of;               // nnz: document of each nonzero
-  DevBuf<float> mlp_p, mlp_g, mlp_m, mlp_v;  // parameters, gradients, Adam moments
-  int64_t mlp_D = 0;                    // embedding size the parameters are for
+  DevBuf<float> mlp_p, mlp_m, mlp_v;    // parameters, Adam moments
   int adam_step = 0;
   bool mlp_ready = false;
-  float mlp_b3 = 0.f;                   // host copy of the output bias
-  bool factors_ready = false;           // At / Bd / dbar for the current corpus and weights
-  DevBuf<uint16_t> w2b;
-  DevBuf<float> dbar, qbar, At, Bd, Cq, pred;
+  bool feat_corpus_ready = false;       // wnorm / dstat / dcdf for the current corpus
+  DevBuf<float> wnorm, dstat, dcdf;     // corpus features
+  DevBuf<float> qstat, qc, qn;          // query features of the current batch
+  DevBuf<float> Pd, Pq, pred;           // folded layer-1 parts, predictions
+  double last_lamda = 0.0;              // lamda of the last run (its mrow)
   DevBuf<float> duals;
   bool duals_valid = false;
   DevBuf<uint32_t> fallbacks;
@@ -612,147 +612,104 @@ int one_shot(const double* r, const double* vecs, double* WMD, double lamda, uin
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Learned warm start: parameter layout, training step, inference stage.
+// Learned warm start: features, inference stage.
 namespace {
 
-struct MlpLayout {
-  int64_t F;  // input features 3D
-  int64_t W1, b1, W2, b2, w3, b3, total;
-  explicit MlpLayout(int64_t D) {
-    const int64_t H = wmd::kMlpHidden;
-    F = 3 * D;
-    W1 = 0;
-    b1 = H * F;
-    W2 = b1 + H;
-    b2 = W2 + H * H;
-    w3 = b2 + H;
-    b3 = w3 + H;
-    total = b3 + 1;
-  }
-};
-
-struct MlpScratch {
-  DevBuf<float> X, y, Z1, H1, Z2, H2, yhat, dy, G1, G2;
-  hipError_t ensure(int64_t nb, int64_t F) {
-    const int64_t H = wmd::kMlpHidden;
-    hipError_t e = X.ensure((size_t)(nb * F));
-    if (e == hipSuccess) e = y.ensure((size_t)nb);
-    if (e == hipSuccess) e = Z1.ensure((size_t)(nb * H));
-    if (e == hipSuccess) e = H1.ensure((size_t)(nb * H));
-    if (e == hipSuccess) e = Z2.ensure((size_t)(nb * H));
-    if (e == hipSuccess) e = H2.ensure((size_t)(nb * H));
-    if (e == hipSuccess) e = yhat.ensure((size_t)nb);
-    if (e == hipSuccess) e = dy.ensure((size_t)nb);
-    if (e == hipSuccess) e = G1.ensure((size_t)(nb * H));
-    if (e == hipSuccess) e = G2.ensure((size_t)(nb * H));
-    return e;
-  }
-  void release() {
-    X.release(); y.release(); Z1.release(); H1.release(); Z2.release(); H2.release();
-    yhat.release(); dy.release(); G1.release(); G2.release();
-  }
-};
-
-wmd::GemmArgs gemm(int M, int N, int K, const float* A, int64_t sam, int64_t sak, const float* B,
-                   int64_t sbk, int64_t sbn, float* C, int64_t ldc, const int32_t* arow = nullptr) {
-  wmd::GemmArgs g{};
-  g.M = M; g.N = N; g.K = K;
-  g.A = A; g.sam = sam; g.sak = sak; g.arow = arow;
-  g.B = B; g.sbk = sbk; g.sbn = sbn;
-  g.C = C; g.ldc = ldc; g.beta = 0.f;
-  return g;
-}
-
-// Loss (added into *d_loss) and gradient of the centred squared error for nb
-// samples already in S.X / S.y, groups given by batch-relative offsets gptr.
-hipError_t mlp_loss_grad(hipStream_t st, int64_t nb, int64_t D, const int32_t* gptr, int groups,
-                         const float* p, float* g, double* d_loss, MlpScratch& S) {
-  const MlpLayout L(D);
-  const int H = wmd::kMlpHidden;
-  const int F = (int)L.F;
-  const int n = (int)nb;
-  hipError_t e;
-#define MLP_TRY(x) do { e = (x); if (e != hipSuccess) return e; } while (0)
-  MLP_TRY(wmd::launch_gemm(gemm(n, H, F, S.X.p, F, 1, p + L.W1, 1, F, S.Z1.p, H), st));
-  MLP_TRY(wmd::launch_bias_gelu(S.Z1.p, S.H1.p, p + L.b1, nb, H, st));
-  MLP_TRY(wmd::launch_gemm(gemm(n, H, H, S.H1.p, H, 1, p + L.W2, 1, H, S.Z2.p, H), st));
-  MLP_TRY(wmd::launch_bias_gelu(S.Z2.p, S.H2.p, p + L.b2, nb, H, st));
-  MLP_TRY(wmd::launch_gemm(gemm(n, 1, H, S.H2.p, H, 1, p + L.w3, 1, 0, S.yhat.p, 1), st));
-  MLP_TRY(wmd::launch_centred_loss(S.yhat.p, p + L.b3, S.y.p, gptr, groups, nb, S.dy.p, d_loss, st));
-  MLP_TRY(wmd::launch_gemm(gemm(H, 1, n, S.H2.p, 1, H, S.dy.p, 1, 0, g + L.w3, 1), st));
-  MLP_TRY(wmd::launch_sum_to(S.dy.p, nb, g + L.b3, st));
-  MLP_TRY(wmd::launch_outer(S.G2.p, S.dy.p, p + L.w3, nb, H, st));
-  MLP_TRY(wmd::launch_gelu_back(S.G2.p, S.Z2.p, nb * H, st));
-  MLP_TRY(wmd::launch_gemm(gemm(H, H, n, S.G2.p, 1, H, S.H1.p, H, 1, g + L.W2, H), st));
-  MLP_TRY(wmd::launch_colsum(S.G2.p, g + L.b2, nb, H, st));
-  MLP_TRY(wmd::launch_gemm(gemm(n, H, H, S.G2.p, H, 1, p + L.W2, H, 1, S.G1.p, H), st));
-  MLP_TRY(wmd::launch_gelu_back(S.G1.p, S.Z1.p, nb * H, st));
-  MLP_TRY(wmd::launch_gemm(gemm(H, F, n, S.G1.p, 1, H, S.X.p, F, 1, g + L.W1, F), st));
-  MLP_TRY(wmd::launch_colsum(S.G1.p, g + L.b1, nb, H, st));
-#undef MLP_TRY
-  return hipSuccess;
-}
-
-// Mean embeddings of the corpus documents (weights c) into c->dbar.
-int mlp_doc_means(wmd_context* c) {
-  HIP_TRY(c->dbar.ensure((size_t)c->N * c->D));
-  HIP_TRY(wmd::launch_mean_rows(c->E.p, (int)c->D, c->doc_ptr.p, c->doc_t.p, c->touched.p,
-                                c->doc_val.p, (int)c->N, c->dbar.p, c->stream));
+// Word norms, document statistics and mid-ranks: once per corpus, outside the
+// timed stages (like the matrix-core planes).
+int warm_corpus_features(wmd_context* c) {
+  if (c->feat_corpus_ready) return WMD_OK;
+  HIP_TRY(c->wnorm.ensure((size_t)c->Vt));
+  HIP_TRY(c->dstat.ensure((size_t)c->N * 5));
+  HIP_TRY(c->dcdf.ensure((size_t)c->nnz));
+  HIP_TRY(wmd::launch_word_norms(c->E.p, c->touched.p, (int)c->Vt, (int)c->D, c->wnorm.p,
+                                 c->stream));
+  HIP_TRY(wmd::launch_doc_feats(c->doc_ptr.p, c->doc_t.p, c->doc_val.p, c->wnorm.p, (int)c->N,
+                                c->dstat.p, c->dcdf.p, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->feat_corpus_ready = true;
   return WMD_OK;
 }
 
-int mlp_query_means(wmd_context* c) {
-  HIP_TRY(c->qbar.ensure((size_t)c->nq * c->D));
-  HIP_TRY(wmd::launch_mean_rows(c->E.p, (int)c->D, c->q_ptr.p, c->q_words.p, nullptr, c->q_r.p,
-                                c->nq, c->qbar.p, c->stream));
+// Query statistics and sorted norm tables of the current batch (every run:
+// they belong to the queries, so a serving loop pays them per batch).
+int warm_query_features(wmd_context* c, int* launches) {
+  if (c->qmax > wmd::kWarmQueryMax) {
+    set_err("the learned warm start takes queries of at most %d words (longest %d)",
+            wmd::kWarmQueryMax, c->qmax);
+    return WMD_ERR_UNSUPPORTED;
+  }
+  const size_t sq = (size_t)c->h_q_ptr[c->nq];
+  HIP_TRY(c->qstat.ensure((size_t)c->nq * 5));
+  HIP_TRY(c->qc.ensure(sq));
+  HIP_TRY(c->qn.ensure(sq));
+  HIP_TRY(wmd::launch_query_feats(c->q_ptr.p, c->q_words.p, c->q_r.p, c->E.p, (int)c->D, c->nq,
+                                  c->qstat.p, c->qc.p, c->qn.p, c->stream));
+  ++*launches;
   return WMD_OK;
 }
 
-// Inference stage: per-query factor, then every (query, nonzero) sample.
-// Corpus factors (per touched word, per document) are built once per
-// corpus and weight set.
-int mlp_predict(wmd_context* c, int* launches) {
-  const MlpLayout L(c->D);
-  const int H = wmd::kMlpHidden;
-  const int D = (int)c->D;
-  const int F = (int)L.F;
-  const float* p = c->mlp_p.p;
-  if (!c->factors_ready) {
-    int rc = mlp_doc_means(c);
-    if (rc != WMD_OK) return rc;
-    HIP_TRY(c->At.ensure((size_t)c->Vt * H));
-    HIP_TRY(c->Bd.ensure((size_t)c->N * H));
-    HIP_TRY(wmd::launch_gemm(gemm((int)c->Vt, H, D, c->E.p, D, 1, p + L.W1, 1, F, c->At.p, H,
-                                  c->touched.p), c->stream));
-    HIP_TRY(wmd::launch_gemm(gemm((int)c->N, H, D, c->dbar.p, D, 1, p + L.W1 + D, 1, F, c->Bd.p,
-                                  H), c->stream));
-    HIP_TRY(wmd::launch_add_rows(c->Bd.p, p + L.b1, c->N, H, c->stream));
-    HIP_TRY(c->w2b.ensure((size_t)H * H));
-    HIP_TRY(wmd::launch_to_bf16(p + L.W2, (int64_t)H * H, c->w2b.p, c->stream));
-    c->factors_ready = true;
-  }
-  int rc = mlp_query_means(c);
+wmd::WarmInferArgs warm_args(wmd_context* c, double lamda) {
+  wmd::WarmInferArgs a{};
+  a.c.wnorm = c->wnorm.p;
+  a.c.dstat = c->dstat.p;
+  a.c.dcdf = c->dcdf.p;
+  a.c.doc_t = c->doc_t.p;
+  a.c.doc_of = c->doc_of.p;
+  a.c.nnz = c->nnz;
+  a.q.qstat = c->qstat.p;
+  a.q.qc = c->qc.p;
+  a.q.qn = c->qn.p;
+  a.q.q_ptr = c->q_ptr.p;
+  a.q.nq = c->nq;
+  a.mrow = c->mrow.p;
+  a.Vt = (int)c->Vt;
+  a.lam_abs = (float)(-lamda);
+  a.params = c->mlp_p.p;
+  return a;
+}
+
+// Inference stage (after the distance stage, which leaves mrow): query
+// features, layer 1's folded document and query parts, then every
+// (query, nonzero) sample into c->pred.
+int mlp_predict(wmd_context* c, double lamda, int* launches) {
+  int rc = warm_query_features(c, launches);
   if (rc != WMD_OK) return rc;
-  HIP_TRY(c->Cq.ensure((size_t)c->nq * H));
-  HIP_TRY(wmd::launch_gemm(gemm(c->nq, H, D, c->qbar.p, D, 1, p + L.W1 + 2 * D, 1, F, c->Cq.p, H),
-                           c->stream));
+  HIP_TRY(c->Pd.ensure((size_t)c->N * wmd::kMlpHidden));
+  HIP_TRY(c->Pq.ensure((size_t)c->nq * wmd::kMlpHidden));
   HIP_TRY(c->pred.ensure((size_t)c->nq * (size_t)c->nnz));
-  const float b3 = c->mlp_b3;
-  wmd::MlpInferArgs ia{};
-  ia.At = c->At.p;
-  ia.Bd = c->Bd.p;
-  ia.Cq = c->Cq.p;
-  ia.W2b = c->w2b.p;
-  ia.b2 = p + L.b2;
-  ia.w3 = p + L.w3;
-  ia.b3 = b3;
-  ia.doc_t = c->doc_t.p;
-  ia.doc_of = c->doc_of.p;
-  ia.nnz = c->nnz;
-  ia.nq = c->nq;
-  ia.pred = c->pred.p;
-  HIP_TRY(wmd::launch_mlp_infer(ia, c->n_cu, c->stream));
-  *launches += 4;
+  HIP_TRY(wmd::launch_warm_fold(c->mlp_p.p, c->dstat.p, (int)c->N, c->qstat.p, c->nq, c->Pd.p,
+                                c->Pq.p, c->stream));
+  wmd::WarmInferArgs a = warm_args(c, lamda);
+  a.Pd = c->Pd.p;
+  a.Pq = c->Pq.p;
+  a.pred = c->pred.p;
+  HIP_TRY(wmd::launch_warm_infer(a, c->stream));
+  *launches += 2;
+  return WMD_OK;
+}
+
+// Host-side checks of a training batch before its launch: every sample
+// names a query and a nonzero of the context, targets index the dual buffer,
+// and the group offsets run from 0 to the batch size without decreasing.
+int check_train_batch(const wmd_context* c, const int32_t* sq, const int32_t* spos,
+                      const int64_t* yidx, int64_t nb, const int32_t* rel, int groups) {
+  if (rel[0] != 0 || rel[groups] != nb) {
+    set_err("training batch offsets do not cover the batch");
+    return WMD_ERR_STATE;
+  }
+  for (int g = 0; g < groups; ++g)
+    if (rel[g + 1] < rel[g]) {
+      set_err("training batch offsets decrease");
+      return WMD_ERR_STATE;
+    }
+  const int64_t ndual = (int64_t)c->nq * c->nnz;
+  for (int64_t s = 0; s < nb; ++s)
+    if (sq[s] < 0 || sq[s] >= c->nq || spos[s] < 0 || spos[s] >= c->nnz || yidx[s] < 0 ||
+        yidx[s] >= ndual) {
+      set_err("training sample %lld out of range", (long long)s);
+      return WMD_ERR_STATE;
+    }
   return WMD_OK;
 }
 
@@ -846,9 +803,10 @@ void wmd_context_destroy(wmd_context* c) {
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
   c->doc_perm.release();
   c->Es.release(); c->nt.release(); c->Qs.release(); c->nqw.release();
-  c->doc_of.release(); c->mlp_p.release(); c->mlp_g.release(); c->mlp_m.release();
-  c->mlp_v.release(); c->w2b.release(); c->dbar.release(); c->qbar.release();
-  c->At.release(); c->Bd.release(); c->Cq.release(); c->pred.release(); c->duals.release();
+  c->doc_of.release(); c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release();
+  c->wnorm.release(); c->dstat.release(); c->dcdf.release(); c->qstat.release();
+  c->qc.release(); c->qn.release(); c->Pd.release(); c->Pq.release(); c->pred.release();
+  c->duals.release();
   c->fallbacks.release(); c->warm_vals.release();
   c->status.release();
   c->plan_bucket.release();
@@ -962,9 +920,8 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   c->h_doc_ptr = dptr;
   c->h_doc_t = dt;
   c->h_touched = touched;
-  c->factors_ready = false;
+  c->feat_corpus_ready = false;
   c->duals_valid = false;
-  if (c->mlp_D != (int64_t)D) c->mlp_ready = false;
   c->V = (int64_t)V;
   c->D = (int64_t)D;
   c->N = (int64_t)N;
@@ -1155,9 +1112,13 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     return WMD_ERR_STATE;
   }
   if (learned && !c->mlp_ready) {
-    set_err("learned warm start needs MLP parameters for D=%lld (wmd_context_mlp_set / train)",
-            (long long)c->D);
+    set_err("learned warm start needs MLP parameters (wmd_context_mlp_set / train)");
     return WMD_ERR_STATE;
+  }
+  if (learned && c->qmax > wmd::kWarmQueryMax) {
+    set_err("the learned warm start takes queries of at most %d words (longest %d)",
+            wmd::kWarmQueryMax, c->qmax);
+    return WMD_ERR_UNSUPPORTED;
   }
   const int Dp = ((int)c->D + 15) & ~15;
   if (mfma && !c->split_ready) {
@@ -1176,6 +1137,10 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     HIP_TRY(wmd::launch_split_rows(sa, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->split_ready = true;
+  }
+  if (learned) {
+    rc = warm_corpus_features(c);  // once per corpus, outside the timed stages
+    if (rc != WMD_OK) return rc;
   }
   if (variant == WMD_VARIANT_RESIDENT) {
     bool fits = true;
@@ -1308,7 +1273,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   HIP_TRY(hipEventRecord(c->ev[1], c->stream));
   wmd::SolveArgs a = base_args(c, lamda, o);
   if (learned) {
-    rc = mlp_predict(c, &launches);
+    rc = mlp_predict(c, lamda, &launches);
     if (rc != WMD_OK) return rc;
     a.warm = c->pred.p;
   } else if (given) {
@@ -1330,6 +1295,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   HIP_TRY(hipEventRecord(c->ev[2], c->stream));
   HIP_TRY(hipEventSynchronize(c->ev[2]));
   c->ran = true;
+  c->last_lamda = lamda;
   c->duals_valid = o.export_duals != 0;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -1657,28 +1623,24 @@ int wmd_sinkhorn_nonfused(const double* r, const double* vecs, double* WMD, doub
 }
 
 
-uint64_t wmd_mlp_param_count(uint64_t D) { return (uint64_t)MlpLayout((int64_t)D).total; }
+uint64_t wmd_mlp_param_count(void) { return (uint64_t)wmd::kWarmParams; }
 
 void wmd_train_options_default(wmd_train_options* opt) {
   if (!opt) return;
   std::memset(opt, 0, sizeof(*opt));
-  opt->epochs = 4;
-  opt->pairs_per_batch = 32;
-  opt->lr = 1e-3;
+  opt->epochs = 6;
+  opt->pairs_per_batch = 64;
+  opt->lr = 3e-3;
   opt->weight_decay = 0.0;
   opt->seed = 0;
+  opt->standardize = 1;
 }
 
 int wmd_context_mlp_set(wmd_context* c, const float* params, uint64_t n_params) {
   int rc = check_ctx(c);
   if (rc != WMD_OK) return rc;
-  if (!c->has_corpus) {
-    set_err("set_corpus must come first (the input size is 3 D)");
-    return WMD_ERR_STATE;
-  }
-  const MlpLayout L(c->D);
-  if (!params || n_params != (uint64_t)L.total) {
-    set_err("expected %lld MLP parameters for D=%lld", (long long)L.total, (long long)c->D);
+  if (!params || n_params != (uint64_t)wmd::kWarmParams) {
+    set_err("expected %d MLP parameters", wmd::kWarmParams);
     return WMD_ERR_ARG;
   }
   for (uint64_t k = 0; k < n_params; ++k)
@@ -1687,17 +1649,13 @@ int wmd_context_mlp_set(wmd_context* c, const float* params, uint64_t n_params) 
       return WMD_ERR_ARG;
     }
   HIP_TRY(c->mlp_p.upload(params, n_params, c->stream));
-  HIP_TRY(c->mlp_g.ensure(n_params));
   HIP_TRY(c->mlp_m.ensure(n_params));
   HIP_TRY(c->mlp_v.ensure(n_params));
   HIP_TRY(hipMemsetAsync(c->mlp_m.p, 0, n_params * sizeof(float), c->stream));
   HIP_TRY(hipMemsetAsync(c->mlp_v.p, 0, n_params * sizeof(float), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  c->mlp_D = c->D;
-  c->mlp_b3 = params[L.b3];
   c->adam_step = 0;
   c->mlp_ready = true;
-  c->factors_ready = false;
   return WMD_OK;
 }
 
@@ -1708,9 +1666,8 @@ int wmd_context_mlp_get(wmd_context* c, float* params, uint64_t n_params) {
     set_err("no MLP parameters in this context");
     return WMD_ERR_STATE;
   }
-  const MlpLayout L(c->mlp_D);
-  if (!params || n_params != (uint64_t)L.total) {
-    set_err("expected %lld MLP parameters", (long long)L.total);
+  if (!params || n_params != (uint64_t)wmd::kWarmParams) {
+    set_err("expected %d MLP parameters", wmd::kWarmParams);
     return WMD_ERR_ARG;
   }
   HIP_TRY(hipMemcpy(params, c->mlp_p.p, n_params * sizeof(float), hipMemcpyDeviceToHost));
@@ -1756,11 +1713,44 @@ int wmd_context_duals(wmd_context* c, float* out) {
   return WMD_OK;
 }
 
+int wmd_context_warm_features(wmd_context* c, float* out) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->ran) {
+    set_err("run must complete before the warm-start features (they use its row minima)");
+    return WMD_ERR_STATE;
+  }
+  if (!out) {
+    set_err("null output");
+    return WMD_ERR_ARG;
+  }
+  rc = warm_corpus_features(c);
+  if (rc != WMD_OK) return rc;
+  int launches = 0;
+  rc = warm_query_features(c, &launches);
+  if (rc != WMD_OK) return rc;
+  const size_t n = (size_t)c->nq * (size_t)c->nnz * wmd::kWarmFeatures;
+  DevBuf<float> feats;
+  HIP_TRY(feats.ensure(n));
+  wmd::WarmInferArgs a = warm_args(c, c->last_lamda);
+  a.feats = feats.p;
+  hipError_t e = wmd::launch_warm_infer(a, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, feats.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                                          c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  feats.release();
+  if (e != hipSuccess) {
+    set_err("warm-start features failed: %s", hipGetErrorString(e));
+    return WMD_ERR_HIP;
+  }
+  return WMD_OK;
+}
+
 int wmd_context_mlp_train(wmd_context* c, const wmd_train_options* opt_in, double* epoch_loss) {
   int rc = check_ctx(c);
   if (rc != WMD_OK) return rc;
-  if (!c->mlp_ready || c->mlp_D != c->D) {
-    set_err("set MLP parameters for this corpus first (wmd_context_mlp_set)");
+  if (!c->mlp_ready) {
+    set_err("set MLP parameters first (wmd_context_mlp_set)");
     return WMD_ERR_STATE;
   }
   if (!c->duals_valid) {
@@ -1769,32 +1759,65 @@ int wmd_context_mlp_train(wmd_context* c, const wmd_train_options* opt_in, doubl
   }
   wmd_train_options o;
   if (opt_in) o = *opt_in; else wmd_train_options_default(&o);
-  if (o.epochs < 0 || o.pairs_per_batch <= 0 || !(o.lr > 0.0) || !(o.weight_decay >= 0.0)) {
+  if (o.epochs < 0 || o.pairs_per_batch <= 0 || o.pairs_per_batch > 4096 || !(o.lr > 0.0) ||
+      !(o.weight_decay >= 0.0)) {
     set_err("bad training options");
     return WMD_ERR_ARG;
   }
-  const MlpLayout L(c->D);
-  const int D = (int)c->D;
+  rc = warm_corpus_features(c);
+  if (rc != WMD_OK) return rc;
+  int launches = 0;
+  rc = warm_query_features(c, &launches);
+  if (rc != WMD_OK) return rc;
+  wmd::WarmInferArgs a = warm_args(c, c->last_lamda);
   const int64_t n_pairs = (int64_t)c->nq * c->N;
-  rc = mlp_doc_means(c);
-  if (rc != WMD_OK) return rc;
-  rc = mlp_query_means(c);
-  if (rc != WMD_OK) return rc;
-  // Sample layout per epoch: pairs in shuffled order, each pair's words contiguous.
+  const int64_t total = (int64_t)c->nq * c->nnz;
+  constexpr int F = wmd::kWarmFeatures;
+  if (o.standardize) {
+    // z = (x - mean) / std per feature over every training sample.
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (total + 255) / 256));
+    DevBuf<double> part;
+    HIP_TRY(part.ensure((size_t)blocks * 2 * F));
+    HIP_TRY(wmd::launch_warm_moments(a, part.p, blocks, c->stream));
+    std::vector<double> hp((size_t)blocks * 2 * F);
+    HIP_TRY(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    part.release();
+    float xm[F], xs[F];
+    for (int k = 0; k < F; ++k) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int b = 0; b < blocks; ++b) {
+        s1 += hp[(size_t)b * 2 * F + k];
+        s2 += hp[(size_t)b * 2 * F + F + k];
+      }
+      const double mean = total ? s1 / (double)total : 0.0;
+      const double var = total ? std::max(0.0, s2 / (double)total - mean * mean) : 0.0;
+      xm[k] = (float)mean;
+      xs[k] = var > 1e-12 ? (float)(1.0 / std::sqrt(var)) : 1.f;
+    }
+    HIP_TRY(hipMemcpy(c->mlp_p.p + wmd::kWarmXm, xm, sizeof(xm), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->mlp_p.p + wmd::kWarmXs, xs, sizeof(xs), hipMemcpyHostToDevice));
+  }
+  // Sample layout per epoch: pairs in shuffled order, each pair's nonzeros
+  // contiguous; sample = (query, nonzero), target = that nonzero's dual.
   std::vector<int64_t> order(n_pairs);
   for (int64_t k = 0; k < n_pairs; ++k) order[k] = k;
-  const int64_t total = (int64_t)c->nq * c->nnz;
-  std::vector<int32_t> s_word(total), s_doc(total), s_query(total);
-  std::vector<int64_t> s_tidx(total);
+  std::vector<int32_t> s_q(total), s_pos(total);
+  std::vector<int64_t> s_y(total);
   const int P = o.pairs_per_batch;
   const int64_t n_batches = (n_pairs + P - 1) / P;
   std::vector<int32_t> rel((size_t)n_batches * (P + 1));
-  std::vector<int64_t> b_first(n_batches + 1), b_groups(n_batches);
-  int64_t nb_max = 0;
-  DevBuf<int32_t> d_word, d_doc, d_query, d_rel;
-  DevBuf<int64_t> d_tidx;
-  DevBuf<double> d_loss;
-  MlpScratch S;
+  std::vector<int64_t> b_first(n_batches + 1);
+  std::vector<int> b_groups(n_batches);
+  DevBuf<int32_t> d_q, d_pos, d_rel;
+  DevBuf<int64_t> d_y;
+  DevBuf<float> gpart;
+  DevBuf<double> lpart, eloss;
+  HIP_TRY(gpart.ensure((size_t)P * wmd::kWarmParams));
+  HIP_TRY(lpart.ensure((size_t)P));
+  HIP_TRY(eloss.ensure((size_t)std::max(1, o.epochs)));
+  HIP_TRY(hipMemsetAsync(eloss.p, 0, sizeof(double) * (size_t)std::max(1, o.epochs), c->stream));
   std::mt19937_64 rng(o.seed);
   int status = WMD_OK;
   for (int ep = 0; ep < o.epochs && status == WMD_OK; ++ep) {
@@ -1803,72 +1826,69 @@ int wmd_context_mlp_train(wmd_context* c, const wmd_train_options* opt_in, doubl
     for (int64_t b = 0; b < n_batches; ++b) {
       b_first[b] = at;
       const int64_t p0 = b * P, p1 = std::min<int64_t>(n_pairs, p0 + P);
-      b_groups[b] = p1 - p0;
+      b_groups[b] = (int)(p1 - p0);
       int32_t* rb = rel.data() + (size_t)b * (P + 1);
       rb[0] = 0;
       for (int64_t k = p0; k < p1; ++k) {
         const int64_t pr = order[k];
         const int Q = (int)(pr / c->N), j = (int)(pr % c->N);
         for (int32_t pos = c->h_doc_ptr[j]; pos < c->h_doc_ptr[j + 1]; ++pos) {
-          s_word[at] = c->h_touched[c->h_doc_t[pos]];
-          s_doc[at] = j;
-          s_query[at] = Q;
-          s_tidx[at] = (int64_t)Q * c->nnz + pos;
+          s_q[at] = Q;
+          s_pos[at] = pos;
+          s_y[at] = (int64_t)Q * c->nnz + pos;
           ++at;
         }
         rb[k - p0 + 1] = (int32_t)(at - b_first[b]);
       }
-      nb_max = std::max(nb_max, at - b_first[b]);
+      rc = check_train_batch(c, s_q.data() + b_first[b], s_pos.data() + b_first[b],
+                             s_y.data() + b_first[b], at - b_first[b], rb, b_groups[b]);
+      if (rc != WMD_OK) return rc;
     }
     b_first[n_batches] = at;
-    if (ep == 0) {
-      HIP_TRY(S.ensure(std::max<int64_t>(nb_max, 1), L.F));
-      HIP_TRY(d_loss.ensure((size_t)n_batches));
-    }
-    HIP_TRY(d_word.upload(s_word.data(), (size_t)at, c->stream));
-    HIP_TRY(d_doc.upload(s_doc.data(), (size_t)at, c->stream));
-    HIP_TRY(d_query.upload(s_query.data(), (size_t)at, c->stream));
-    HIP_TRY(d_tidx.upload(s_tidx.data(), (size_t)at, c->stream));
+    HIP_TRY(d_q.upload(s_q.data(), (size_t)at, c->stream));
+    HIP_TRY(d_pos.upload(s_pos.data(), (size_t)at, c->stream));
+    HIP_TRY(d_y.upload(s_y.data(), (size_t)at, c->stream));
     HIP_TRY(d_rel.upload(rel.data(), rel.size(), c->stream));
-    HIP_TRY(hipMemsetAsync(d_loss.p, 0, sizeof(double) * (size_t)n_batches, c->stream));
     for (int64_t b = 0; b < n_batches; ++b) {
       const int64_t s0 = b_first[b], nb = b_first[b + 1] - s0;
       if (nb <= 0) continue;
-      HIP_TRY(wmd::launch_gather_x(c->E.p, D, d_word.p + s0, d_doc.p + s0, d_query.p + s0,
-                                   c->dbar.p, c->qbar.p, nb, S.X.p, c->stream));
-      HIP_TRY(wmd::launch_gather_f(c->duals.p, d_tidx.p + s0, nb, S.y.p, c->stream));
-      HIP_TRY(mlp_loss_grad(c->stream, nb, c->D, d_rel.p + (size_t)b * (P + 1), (int)b_groups[b],
-                            c->mlp_p.p, c->mlp_g.p, d_loss.p + b, S));
+      wmd::WarmTrainArgs t{};
+      t.ctx = a;
+      t.s_q = d_q.p + s0;
+      t.s_pos = d_pos.p + s0;
+      t.ysrc = c->duals.p;
+      t.yidx = d_y.p + s0;
+      t.gptr = d_rel.p + (size_t)b * (P + 1);
+      t.groups = b_groups[b];
+      t.n_total = nb;
+      t.params = c->mlp_p.p;
+      t.gpart = gpart.p;
+      t.lpart = lpart.p;
+      HIP_TRY(wmd::launch_warm_train(t, c->stream));
       ++c->adam_step;
-      HIP_TRY(wmd::launch_adam(c->mlp_p.p, c->mlp_g.p, c->mlp_m.p, c->mlp_v.p, L.total,
-                               (float)o.lr, 0.9f, 0.999f, 1e-8f, c->adam_step,
-                               (float)o.weight_decay, c->stream));
+      HIP_TRY(wmd::launch_warm_adam(c->mlp_p.p, gpart.p, t.groups, c->mlp_m.p, c->mlp_v.p,
+                                    (float)o.lr, 0.9f, 0.999f, 1e-8f, c->adam_step,
+                                    (float)o.weight_decay, lpart.p, eloss.p + ep, c->stream));
     }
-    std::vector<double> bl(n_batches);
-    HIP_TRY(hipMemcpyAsync(bl.data(), d_loss.p, sizeof(double) * (size_t)n_batches,
-                           hipMemcpyDeviceToHost, c->stream));
+    double el = 0.0;
+    HIP_TRY(hipMemcpyAsync(&el, eloss.p + ep, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    double num = 0.0;
-    for (int64_t b = 0; b < n_batches; ++b) num += bl[b] * (double)(b_first[b + 1] - b_first[b]);
-    if (epoch_loss) epoch_loss[ep] = at ? num / (double)at : 0.0;
-    if (!std::isfinite(num)) {
+    el = at ? el / (double)at : 0.0;
+    if (epoch_loss) epoch_loss[ep] = el;
+    if (!std::isfinite(el)) {
       set_err("training diverged in epoch %d", ep);
       status = WMD_ERR_NONFINITE;
     }
   }
-  HIP_TRY(hipMemcpy(&c->mlp_b3, c->mlp_p.p + L.b3, sizeof(float), hipMemcpyDeviceToHost));
-  d_word.release(); d_doc.release(); d_query.release(); d_rel.release(); d_tidx.release();
-  d_loss.release();
-  S.release();
-  c->factors_ready = false;
+  d_q.release(); d_pos.release(); d_y.release(); d_rel.release();
+  gpart.release(); lpart.release(); eloss.release();
   return status;
 }
 
-int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const float* y,
-                      uint64_t n_groups, const int32_t* group_ptr, const float* params,
-                      double* loss, float* grad) {
-  if (!X || !y || !group_ptr || !params || !loss || !grad || n == 0 || D == 0 || n_groups == 0 ||
-      n > (1u << 24) || D > 65536) {
+int wmd_mlp_loss_grad(int device, uint64_t n, const float* X, const float* y, uint64_t n_groups,
+                      const int32_t* group_ptr, const float* params, double* loss, float* grad) {
+  if (!X || !y || !group_ptr || !params || !loss || !grad || n == 0 || n_groups == 0 ||
+      n > (1u << 24) || n_groups > (1u << 20)) {
     set_err("bad arguments");
     return WMD_ERR_ARG;
   }
@@ -1888,33 +1908,46 @@ int wmd_mlp_loss_grad(int device, uint64_t n, uint64_t D, const float* X, const 
     return WMD_ERR_ARG;
   }
   HIP_TRY(hipSetDevice(device));
-  const MlpLayout L((int64_t)D);
-  MlpScratch S;
-  DevBuf<float> dp, dg;
-  DevBuf<int32_t> dgp;
-  DevBuf<double> dl;
+  constexpr int F = wmd::kWarmFeatures;
+  DevBuf<float> dX, dy, dp, dgp, dg;
+  DevBuf<int32_t> dgr;
+  DevBuf<double> dlp, dl;
   hipStream_t st = nullptr;
   HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  hipError_t e = S.ensure((int64_t)n, L.F);
-  if (e == hipSuccess) e = S.X.upload(X, n * (size_t)L.F, st);
-  if (e == hipSuccess) e = S.y.upload(y, n, st);
-  if (e == hipSuccess) e = dp.upload(params, (size_t)L.total, st);
-  if (e == hipSuccess) e = dg.ensure((size_t)L.total);
-  if (e == hipSuccess) e = dgp.upload(group_ptr, n_groups + 1, st);
+  hipError_t e = dX.upload(X, n * F, st);
+  if (e == hipSuccess) e = dy.upload(y, n, st);
+  if (e == hipSuccess) e = dp.upload(params, wmd::kWarmParams, st);
+  if (e == hipSuccess) e = dgr.upload(group_ptr, n_groups + 1, st);
+  if (e == hipSuccess) e = dgp.ensure((size_t)n_groups * wmd::kWarmParams);
+  if (e == hipSuccess) e = dg.ensure(wmd::kWarmParams);
+  if (e == hipSuccess) e = dlp.ensure(n_groups);
   if (e == hipSuccess) e = dl.ensure(1);
-  if (e == hipSuccess) e = hipMemsetAsync(dl.p, 0, sizeof(double), st);
-  if (e == hipSuccess) e = mlp_loss_grad(st, (int64_t)n, (int64_t)D, dgp.p, (int)n_groups, dp.p,
-                                         dg.p, dl.p, S);
-  if (e == hipSuccess) e = hipMemcpyAsync(grad, dg.p, sizeof(float) * (size_t)L.total,
+  if (e == hipSuccess) {
+    wmd::WarmTrainArgs t{};
+    t.X = dX.p;
+    t.ysrc = dy.p;
+    t.gptr = dgr.p;
+    t.groups = (int)n_groups;
+    t.n_total = (int64_t)n;
+    t.params = dp.p;
+    t.gpart = dgp.p;
+    t.lpart = dlp.p;
+    e = wmd::launch_warm_train(t, st);
+  }
+  if (e == hipSuccess) e = wmd::launch_warm_reduce(dgp.p, (int)n_groups, dlp.p, dg.p, dl.p, st);
+  double lsum = 0.0;
+  if (e == hipSuccess) e = hipMemcpyAsync(grad, dg.p, sizeof(float) * wmd::kWarmParams,
                                           hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(loss, dl.p, sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&lsum, dl.p, sizeof(double), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  S.release(); dp.release(); dg.release(); dgp.release(); dl.release();
+  dX.release(); dy.release(); dp.release(); dgr.release(); dgp.release(); dg.release();
+  dlp.release(); dl.release();
   (void)hipStreamDestroy(st);
   if (e != hipSuccess) {
     set_err("MLP loss/gradient failed: %s", hipGetErrorString(e));
     return WMD_ERR_HIP;
   }
+  *loss = lsum / (double)n;
   return WMD_OK;
 }
 

@@ -199,62 +199,100 @@ constexpr int kMaxQ = kShapes[kNumShapes - 1].B * kShapes[kNumShapes - 1].NI;
   X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39)
 
 // ---- learned warm start (wmd_mlp.hip) ----
-constexpr int kMlpHidden = 256;
+// A 15 -> 16 -> 16 -> 1 network (GELU hidden layers) predicts the converged
+// log v_e of every (query, document word) from norm features of the word,
+// the document and the query (feature index k):
+//   0 |e_w|, 1 |e_w|^2 (the word's embedding norm n)
+//   2..6  document: c-weighted mean, std, min, max of its words' norms, log n_j
+//   7..11 query: r-weighted mean, std, min, max of its words' norms, log q
+//   12 cdf: the word's c-weighted mid-rank among its document's words by norm
+//   13 T: the query's r-weighted norm quantile function at cdf
+//   14 |lamda| (sqrt(T^2 + n^2) - mrow): the kernel-row rescale gap at the
+//      norm-matched distance
+// Parameters (flat FP32): xm[F], xs[F] (inputs enter as z = (x - xm) xs; set
+// from the training data, not trained), W1[H][F], b1[H], W2[H][H], b2[H],
+// w3[H], b3.
+constexpr int kWarmFeatures = 15;
+constexpr int kMlpHidden = 16;
+constexpr int kWarmXm = 0;
+constexpr int kWarmXs = kWarmXm + kWarmFeatures;
+constexpr int kWarmW1 = kWarmXs + kWarmFeatures;
+constexpr int kWarmB1 = kWarmW1 + kMlpHidden * kWarmFeatures;
+constexpr int kWarmW2 = kWarmB1 + kMlpHidden;
+constexpr int kWarmB2 = kWarmW2 + kMlpHidden * kMlpHidden;
+constexpr int kWarmW3 = kWarmB2 + kMlpHidden;
+constexpr int kWarmB3 = kWarmW3 + kMlpHidden;
+constexpr int kWarmParams = kWarmB3 + 1;
+constexpr int kWarmQueryMax = 1024;  // longest query the feature kernels take
 
-// C[m, n] = beta C[m, n] + sum_k A(m, k) B(k, n) in FP32;
-// A(m, k) = A[r(m) sam + k sak] with r(m) = arow ? arow[m] : m, B(k, n) = B[k sbk + n sbn].
-struct GemmArgs {
-  int M, N, K;
-  const float* A;
-  int64_t sam, sak;
-  const int32_t* arow;
-  const float* B;
-  int64_t sbk, sbn;
-  float* C;
-  int64_t ldc;
-  float beta;
-};
-
-struct MlpInferArgs {
-  const float* At;         // Vt x H   W1e E[touched t]
-  const float* Bd;         // N x H    W1d dbar_j + b1
-  const float* Cq;         // nq x H   W1q qbar_Q
-  const uint16_t* W2b;     // H x H    BF16 bits, row = output unit
-  const float* b2;
-  const float* w3;
-  float b3;
+// Corpus features: word norms (per touched word), document statistics and
+// each nonzero's mid-rank in its document.
+struct WarmCorpus {
+  const float* wnorm;      // Vt
+  const float* dstat;      // N x 5
+  const float* dcdf;       // nnz
   const int32_t* doc_t;    // nnz
-  const int32_t* doc_of;   // nnz: document of each nonzero
+  const int32_t* doc_of;   // nnz
   int64_t nnz;
+};
+// Query features: statistics and the sorted norm table (per query Q, entries
+// q_ptr[Q] .. q_ptr[Q+1]-1: mid-rank cdf ascending, norms ascending).
+struct WarmQueries {
+  const float* qstat;      // nq x 5
+  const float* qc;         // sum q
+  const float* qn;         // sum q
+  const int32_t* q_ptr;    // nq + 1
   int nq;
-  float* pred;             // nq x nnz
+};
+struct WarmInferArgs {
+  WarmCorpus c;
+  WarmQueries q;
+  const float* mrow;       // nq x Vt row minima of the distance stage
+  int Vt;
+  float lam_abs;           // |lamda|
+  const float* params;     // kWarmParams
+  const float* Pd;         // N x H folded document part of layer 1
+  const float* Pq;         // nq x H folded query part of layer 1 (with b1)
+  float* pred;             // nq x nnz out
+  float* feats;            // export mode: nq x nnz x F raw features (pred unused)
+};
+// One Adam step's loss and gradient: groups (pairs) of samples, group g =
+// samples gptr[g] .. gptr[g+1]-1 of the batch; sample s is either row s of X
+// (X given) or (query s_q[s], nonzero s_pos[s]) of a context (X null, its
+// features computed like inference), with target y[s] (ysrc[yidx[s]] when
+// yidx is given).
+struct WarmTrainArgs {
+  const float* X;          // n x F raw features, or null
+  WarmInferArgs ctx;       // feature source when X is null
+  const int32_t* s_q;
+  const int32_t* s_pos;
+  const float* ysrc;
+  const int64_t* yidx;     // or null: y = ysrc[s]
+  const int32_t* gptr;     // groups + 1, batch-relative
+  int groups;
+  int64_t n_total;         // samples of the batch
+  const float* params;
+  float* gpart;            // groups x kWarmParams gradient partials
+  double* lpart;           // groups: sum of centred squared errors
 };
 
-hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
-hipError_t launch_bias_gelu(float* Z, float* Hout, const float* b, int64_t rows, int cols,
-                            hipStream_t s);
-hipError_t launch_gelu_back(float* dH, const float* Z, int64_t n, hipStream_t s);
-hipError_t launch_outer(float* dH, const float* dy, const float* w, int64_t rows, int cols,
-                        hipStream_t s);
-hipError_t launch_colsum(const float* X, float* out, int64_t rows, int cols, hipStream_t s);
-hipError_t launch_centred_loss(const float* yhat, const float* b3, const float* y,
-                               const int32_t* gptr, int groups, int64_t n_total, float* dy,
-                               double* loss, hipStream_t s);
-hipError_t launch_sum_to(const float* x, int64_t n, float* out, hipStream_t s);
-hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
-                       float b2, float eps, int step, float wd, hipStream_t s);
-hipError_t launch_mean_rows(const float* E, int D, const int32_t* ptr, const int32_t* ids,
-                            const int32_t* touched, const float* wts, int rows, float* out,
-                            hipStream_t s);
-hipError_t launch_gather_x(const float* E, int D, const int32_t* s_word, const int32_t* s_doc,
-                           const int32_t* s_query, const float* dbar, const float* qbar,
-                           int64_t n, float* X, hipStream_t s);
-hipError_t launch_gather_f(const float* src, const int64_t* idx, int64_t n, float* dst,
-                           hipStream_t s);
-hipError_t launch_add_rows(float* X, const float* b, int64_t rows, int cols, hipStream_t s);
-hipError_t launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t s);
-hipError_t launch_mlp_infer(const MlpInferArgs& a, int n_cu, hipStream_t s);
-size_t mlp_infer_lds_bytes();
+hipError_t launch_word_norms(const float* E, const int32_t* ids, int rows, int D, float* out,
+                             hipStream_t s);
+hipError_t launch_doc_feats(const int32_t* doc_ptr, const int32_t* doc_t, const float* doc_val,
+                            const float* wnorm, int N, float* dstat, float* dcdf, hipStream_t s);
+hipError_t launch_query_feats(const int32_t* q_ptr, const int32_t* q_words, const float* q_r,
+                              const float* E, int D, int nq, float* qstat, float* qc, float* qn,
+                              hipStream_t s);
+hipError_t launch_warm_fold(const float* params, const float* dstat, int N, const float* qstat,
+                            int nq, float* Pd, float* Pq, hipStream_t s);
+hipError_t launch_warm_infer(const WarmInferArgs& a, hipStream_t s);
+hipError_t launch_warm_moments(const WarmInferArgs& a, double* part, int blocks, hipStream_t s);
+hipError_t launch_warm_train(const WarmTrainArgs& t, hipStream_t s);
+hipError_t launch_warm_adam(float* p, const float* gpart, int groups, float* m, float* v,
+                            float lr, float b1, float b2, float eps, int step, float wd,
+                            const double* lpart, double* loss_acc, hipStream_t s);
+hipError_t launch_warm_reduce(const float* gpart, int groups, const double* lpart, float* grad,
+                              double* loss, hipStream_t s);
 
 // Nearest documents per query (wmd_topk.hip).
 struct TopkArgs {

@@ -1,190 +1,510 @@
-// Learned warm start: a 3D -> 256 -> 256 -> 1 MLP (GELU hidden layers) that
-// predicts, for every (query, document word) nonzero, the converged log of
-// the column scaling v_e (in the row-rescaled kernel convention of the
-// solver), from the word's embedding, the document's mean embedding and the
-// query's mean embedding.  Because (u, v) -> (a u, v / a) leaves the plan
-// unchanged, targets and loss are centred per (query, document) pair.
+// Learned warm start: a 15 -> 16 -> 16 -> 1 network (tanh-form GELU hidden
+// layers) predicts, for every (query, document word) nonzero, the converged
+// log of the column scaling v_e (in the solver's row-rescaled kernel
+// convention) from norm features of the word, its document and the query
+// (feature list in wmd_internal.h).  Because (u, v) -> (a u, v / a) leaves
+// the plan unchanged, targets and loss are centred per (query, document)
+// pair.
 //
-// Training (FP32): tiled GEMM k_gemm for the linear layers and their
-// gradients, GELU forward/backward, centred squared-error loss, Adam.
-// Inference: layer 1 is linear in the concatenated input, so W1 [e; d; q]
-// = W1e e + W1d d + W1q q is precomputed per touched word, per document and
-// per query; each sample then runs GELU, the 256 x 256 layer on the matrix
-// cores (BF16 inputs, FP32 accumulation; W2 resident in LDS), GELU and the
-// output dot product (k_mlp_infer).
+// Why norm features: the embeddings' norms set most of every distance
+// (|a - b|^2 = |a|^2 + |b|^2 - 2 a.b with a small a.b), and that part of the
+// cost, sqrt(|a|^2 + |b|^2), has a negative mixed derivative, so the
+// transport plan pairs the document's words with the query's words in order
+// of norm.  The c-weighted mid-rank of a word's norm in its document, mapped
+// through the query's r-weighted norm quantile function, is the norm it is
+// matched to; the network learns the dual from it.
+//
+// Kernels: the corpus features (once per corpus), the query features (per
+// run), the folding of the document and query parts of layer 1, inference
+// (one thread per sample), and training: one fused kernel per Adam step
+// (forward, centred loss and backward for one pair per block, gradient
+// partials through LDS) plus the Adam update, which sums the partials.
 #include "wmd_internal.h"
 #include "wmd_lanes.h"
 
 namespace wmd {
 namespace {
 
+constexpr int F = kWarmFeatures;
 constexpr int H = kMlpHidden;
+// The network's parameters are read through the constant address space: no
+// kernel that reads them writes them, so every lane's same-address load
+// becomes a scalar load (s_load through the scalar cache) instead of a
+// per-lane vector load.
+typedef __attribute__((address_space(4))) const float cfloat;
+__device__ __forceinline__ const cfloat* as_const(const float* p) {
+  return (const cfloat*)(p);
+}
+// Feature slots taken per sample (the rest are per document and per query).
+constexpr int kSampleFeat[5] = {0, 1, 12, 13, 14};
 
-// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))).
+// GELU, tanh form, 0.5 x (1 + tanh(z)) with z = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as x * sigmoid(2z) = x / (1 + 2^(x (c0 + c1 x^2))) with
+// c0 = -2 sqrt(2/pi) log2(e), c1 = 0.044715 c0.
 constexpr float kGeluC = 0.7978845608028654f;
 constexpr float kGeluA = 0.044715f;
+constexpr float kGeluE0 = -2.f * kGeluC * 1.4426950408889634f;
+constexpr float kGeluE1 = kGeluE0 * kGeluA;
 
-__device__ __forceinline__ float gelu(float x) {
-  const float z = kGeluC * fmaf(kGeluA * x * x, x, x);
-  return 0.5f * x * (1.f + tanhf(z));
-}
-
-__device__ __forceinline__ float gelu_grad(float x) {
-  const float z = kGeluC * fmaf(kGeluA * x * x, x, x);
-  const float t = tanhf(z);
-  return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * kGeluC * fmaf(3.f * kGeluA, x * x, 1.f);
-}
-
-// Same function through exp: x * sigmoid(2z) = x / (1 + exp(-2z)) =
-// x / (1 + 2^(x (c0 + c1 x^2))) with c0 = -2 sqrt(2/pi) log2(e) and
-// c1 = c0 * 0.044715; used on the inference path (one v_exp_f32, one
-// v_rcp_f32 and five other instructions).
-constexpr float kGeluE0 = -2.f * 0.7978845608028654f * 1.4426950408889634f;
-constexpr float kGeluE1 = kGeluE0 * 0.044715f;
-__device__ __forceinline__ float gelu_fast(float x) {
+__device__ __forceinline__ float sig2z(float x) {
   const float p = fmaf(kGeluE1, x * x, kGeluE0);
-  return x * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * p));
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * p));
+}
+__device__ __forceinline__ float gelu(float x) { return x * sig2z(x); }
+// d/dx x s(x), s = sigmoid(2z): s + x s (1 - s) 2 sqrt(2/pi) (1 + 3 a x^2)
+__device__ __forceinline__ float gelu_grad(float x, float s) {
+  return fmaf(x * s * (1.f - s), 2.f * kGeluC * fmaf(3.f * kGeluA, x * x, 1.f), s);
+}
+
+// numpy.interp of x over ascending knots xc[0..q) with values yv: clamped
+// outside the knots, linear between the bracketing pair.
+__device__ __forceinline__ float interp_knots(const float* xc, const float* yv, int q, float x) {
+  if (x <= xc[0]) return yv[0];
+  if (x >= xc[q - 1]) return yv[q - 1];
+  int lo = 0, hi = q - 1;  // xc[lo] <= x < xc[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (xc[mid] <= x) lo = mid; else hi = mid;
+  }
+  return yv[lo] + (x - xc[lo]) * (yv[hi] - yv[lo]) / (xc[hi] - xc[lo]);
+}
+
+// The five per-sample features (slots 0, 1, 12, 13, 14) of nonzero pos
+// (touched word t) under query Q, whose sorted norm table is qc / qn.
+__device__ __forceinline__ void sample_feats(const WarmInferArgs& a, const float* qc,
+                                             const float* qn, int q, int Q, int64_t pos, int t,
+                                             float (&x)[5]) {
+  const float n = a.c.wnorm[t];
+  const float cdf = a.c.dcdf[pos];
+  const float T = interp_knots(qc, qn, q, cdf);
+  const float mr = a.mrow[(int64_t)Q * a.Vt + t];
+  x[0] = n;
+  x[1] = n * n;
+  x[2] = cdf;
+  x[3] = T;
+  x[4] = a.lam_abs * (sqrtf(fmaf(T, T, n * n)) - mr);
+}
+
+// All 15 raw features of a sample.
+__device__ __forceinline__ void all_feats(const WarmInferArgs& a, const float* qc, const float* qn,
+                                          int q, int Q, int64_t pos, float (&x)[F]) {
+  const int t = a.c.doc_t[pos];
+  const int j = a.c.doc_of[pos];
+  float s5[5];
+  sample_feats(a, qc, qn, q, Q, pos, t, s5);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[kSampleFeat[k]] = s5[k];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[2 + k] = a.c.dstat[(int64_t)j * 5 + k];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) x[7 + k] = a.q.qstat[(int64_t)Q * 5 + k];
 }
 
 // ---------------------------------------------------------------------------
-// C[m, n] = beta C[m, n] + sum_k A(m, k) B(k, n), 64 x 64 tiles, k in chunks of
-// 16 through LDS, 4 x 4 outputs per thread.  A(m, k) = A[r(m) sam + k sak]
-// with r(m) = arow ? arow[m] : m; B(k, n) = B[k sbk + n sbn].
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-  __shared__ float As[16][65];
-  __shared__ float Bs[16][65];
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-  float acc[4][4] = {};
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int idx = tid + 256 * r;
-      const int mm = idx >> 4, kk = idx & 15;
-      const int m = m0 + mm, k = k0 + kk;
-      float v = 0.f;
-      if (m < g.M && k < g.K) {
-        const int64_t row = g.arow ? (int64_t)g.arow[m] : (int64_t)m;
-        v = g.A[row * g.sam + (int64_t)k * g.sak];
-      }
-      As[kk][mm] = v;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int idx = tid + 256 * r;
-      const int kk = idx >> 6, nn = idx & 63;
-      const int k = k0 + kk, n = n0 + nn;
-      Bs[kk][nn] = (k < g.K && n < g.N) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      float a[4], b[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
-    }
-    __syncthreads();
+// Corpus and query features.
+
+// |E[ids[r]]| in FP64 accumulation, one wave per row.
+__global__ __launch_bounds__(256) void k_word_norms(const float* E, const int32_t* ids, int rows,
+                                                    int D, float* out) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (r >= rows) return;
+  const int lane = threadIdx.x & 63;
+  const float* e = E + (int64_t)ids[r] * D;
+  double s = 0.0;
+  for (int k = lane; k < D; k += 64) s = fma((double)e[k], (double)e[k], s);
+  s = wave_sum_d(s);
+  if (lane == 0) out[r] = (float)sqrt(s);
+}
+
+// One wave per document: c-weighted mean, std, min, max of its words' norms
+// and log(n) into dstat[j], and each word's c-weighted mid-rank by norm
+// (ties by position) into dcdf.  An empty document gets zeros.
+__global__ __launch_bounds__(64) void k_doc_feats(const int32_t* doc_ptr, const int32_t* doc_t,
+                                                  const float* doc_val, const float* wnorm,
+                                                  float* dstat, float* dcdf) {
+  const int j = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int b = doc_ptr[j], n = doc_ptr[j + 1] - b;
+  double w = 0.0, s1 = 0.0;
+  float mn = INFINITY, mx = -INFINITY;
+  for (int e = lane; e < n; e += 64) {
+    const double c = doc_val[b + e], x = wnorm[doc_t[b + e]];
+    w += c;
+    s1 = fma(c, x, s1);
+    mn = fminf(mn, (float)x);
+    mx = fmaxf(mx, (float)x);
   }
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int m = m0 + ty * 4 + i, n = n0 + tx * 4 + j;
-      if (m < g.M && n < g.N) {
-        float* c = g.C + (int64_t)m * g.ldc + n;
-        *c = (g.beta == 0.f) ? acc[i][j] : fmaf(g.beta, *c, acc[i][j]);
-      }
+  w = wave_sum_d(w);
+  s1 = wave_sum_d(s1);
+  mn = group_red<64, kMin>(mn);
+  mx = group_red<64, kMax>(mx);
+  const double mean = n > 0 ? s1 / w : 0.0;
+  double s2 = 0.0;
+  for (int e = lane; e < n; e += 64) {
+    const double d = (double)wnorm[doc_t[b + e]] - mean;
+    s2 = fma((double)doc_val[b + e] * d, d, s2);
+  }
+  s2 = wave_sum_d(s2);
+  if (lane == 0) {
+    float* o = dstat + (int64_t)j * 5;
+    o[0] = (float)mean;
+    o[1] = n > 0 ? (float)sqrt(s2 / w) : 0.f;
+    o[2] = n > 0 ? mn : 0.f;
+    o[3] = n > 0 ? mx : 0.f;
+    o[4] = n > 0 ? logf((float)n) : 0.f;
+  }
+  for (int e = lane; e < n; e += 64) {
+    const float xe = wnorm[doc_t[b + e]];
+    double below = 0.0;
+    for (int f = 0; f < n; ++f) {
+      const float xf = wnorm[doc_t[b + f]];
+      if (xf < xe || (xf == xe && f < e)) below += doc_val[b + f];
     }
+    dcdf[b + e] = (float)((below + 0.5 * (double)doc_val[b + e]) / w);
+  }
 }
 
-// H = gelu(Z + bias) with Z overwritten by Z + bias (kept for the backward).
-__global__ void k_bias_gelu(float* Z, float* Hout, const float* bias, int64_t rows, int cols) {
+// One block per query: the norms of its words (FP64 accumulation), sorted
+// ascending (ties by position) with their r-weighted mid-ranks into qn / qc,
+// and r-weighted mean, std, min, max and log(q) into qstat[Q].
+__global__ __launch_bounds__(256) void k_query_feats(const int32_t* q_ptr, const int32_t* q_words,
+                                                     const float* q_r, const float* E, int D,
+                                                     float* qstat, float* qc, float* qn) {
+  __shared__ float nrm[kWarmQueryMax], sn[kWarmQueryMax], sr[kWarmQueryMax];
+  const int Q = blockIdx.x;
+  const int qb = q_ptr[Q], q = q_ptr[Q + 1] - qb;
+  if (q > kWarmQueryMax) return;  // refused on the host; never index past the tables
+  for (int i = threadIdx.x; i < q; i += blockDim.x) {
+    const float* e = E + (int64_t)q_words[qb + i] * D;
+    double s = 0.0;
+    for (int k = 0; k < D; ++k) s = fma((double)e[k], (double)e[k], s);
+    nrm[i] = (float)sqrt(s);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < q; i += blockDim.x) {
+    int rank = 0;
+    for (int f = 0; f < q; ++f) rank += (nrm[f] < nrm[i] || (nrm[f] == nrm[i] && f < i)) ? 1 : 0;
+    sn[rank] = nrm[i];
+    sr[rank] = q_r[qb + i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double w = 0.0, s1 = 0.0;
+    for (int k = 0; k < q; ++k) {
+      w += sr[k];
+      s1 = fma((double)sr[k], (double)sn[k], s1);
+    }
+    const double mean = s1 / w;
+    double s2 = 0.0, cum = 0.0;
+    for (int k = 0; k < q; ++k) {
+      const double d = (double)sn[k] - mean;
+      s2 = fma((double)sr[k] * d, d, s2);
+      qc[qb + k] = (float)((cum + 0.5 * (double)sr[k]) / w);
+      qn[qb + k] = sn[k];
+      cum += sr[k];
+    }
+    float* o = qstat + (int64_t)Q * 5;
+    o[0] = (float)mean;
+    o[1] = (float)sqrt(s2 / w);
+    o[2] = sn[0];
+    o[3] = sn[q - 1];
+    o[4] = logf((float)q);
+  }
+}
+
+// Layer 1's document and query parts: Pd[j][h] = sum_{k=2..6} W1[h][k] z_k,
+// Pq[Q][h] = b1[h] + sum_{k=7..11} W1[h][k] z_k.
+__global__ void k_warm_fold(const float* p, const float* dstat, int N, const float* qstat, int nq,
+                            float* Pd, float* Pq) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * cols) return;
-  const float z = Z[i] + bias[i % cols];
-  Z[i] = z;
-  Hout[i] = gelu(z);
+  const int64_t rows = (int64_t)N + nq;
+  if (i >= rows * H) return;
+  const int64_t row = i / H;
+  const int h = (int)(i % H);
+  const bool doc = row < N;
+  const int k0 = doc ? 2 : 7;
+  const float* st = doc ? dstat + row * 5 : qstat + (row - N) * 5;
+  float acc = doc ? 0.f : p[kWarmB1 + h];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)
+    acc = fmaf(p[kWarmW1 + h * F + k0 + k], (st[k] - p[kWarmXm + k0 + k]) * p[kWarmXs + k0 + k], acc);
+  if (doc) Pd[row * H + h] = acc;
+  else Pq[(row - N) * H + h] = acc;
 }
 
-// dZ = dH * gelu'(Z) (in place on dH).
-__global__ void k_gelu_back(float* dH, const float* Z, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dH[i] *= gelu_grad(Z[i]);
+// ---------------------------------------------------------------------------
+// Inference.  Blocks of 256 threads over kInferChunks x 256 consecutive
+// nonzeros of one query (blockIdx.y); the query's sorted norm table sits in
+// LDS for the quantile lookups.  With export set the kernel writes the 15
+// raw features instead of the prediction.
+constexpr int kInferChunks = 8;
+
+template <bool EXPORT>
+__global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
+  __shared__ float sqc[kWarmQueryMax], sqn[kWarmQueryMax];
+  const int Q = blockIdx.y;
+  const int qb = a.q.q_ptr[Q], q = a.q.q_ptr[Q + 1] - qb;
+  if (q > kWarmQueryMax) return;  // refused on the host; never index past the tables
+  for (int i = threadIdx.x; i < q; i += 256) {
+    sqc[i] = a.q.qc[qb + i];
+    sqn[i] = a.q.qn[qb + i];
+  }
+  __syncthreads();
+  const cfloat* p = as_const(a.params);
+  const int64_t base = (int64_t)blockIdx.x * (256 * kInferChunks);
+  for (int ch = 0; ch < kInferChunks; ++ch) {
+    const int64_t pos = base + ch * 256 + threadIdx.x;
+    if (pos >= a.c.nnz) break;
+    if constexpr (EXPORT) {
+      float x[F];
+      all_feats(a, sqc, sqn, q, Q, pos, x);
+      float* o = a.feats + ((int64_t)Q * a.c.nnz + pos) * F;
+#pragma unroll
+      for (int k = 0; k < F; ++k) o[k] = x[k];
+    } else {
+      const int t = a.c.doc_t[pos];
+      const int j = a.c.doc_of[pos];
+      float x[5];
+      sample_feats(a, sqc, sqn, q, Q, pos, t, x);
+      float z[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        z[k] = (x[k] - p[kWarmXm + kSampleFeat[k]]) * p[kWarmXs + kSampleFeat[k]];
+      const float4* pd = reinterpret_cast<const float4*>(a.Pd + (int64_t)j * H);
+      const float* pq = a.Pq + (int64_t)Q * H;
+      float h1[H];
+#pragma unroll
+      for (int v = 0; v < H / 4; ++v) {
+        const float4 d = pd[v];
+        h1[4 * v] = d.x;
+        h1[4 * v + 1] = d.y;
+        h1[4 * v + 2] = d.z;
+        h1[4 * v + 3] = d.w;
+      }
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        float acc = h1[h] + pq[h];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc = fmaf(p[kWarmW1 + h * F + kSampleFeat[k]], z[k], acc);
+        h1[h] = gelu(acc);
+      }
+      float y = p[kWarmB3];
+#pragma unroll
+      for (int o = 0; o < H; ++o) {
+        float acc = p[kWarmB2 + o];
+#pragma unroll
+        for (int h = 0; h < H; ++h) acc = fmaf(p[kWarmW2 + o * H + h], h1[h], acc);
+        y = fmaf(p[kWarmW3 + o], gelu(acc), y);
+      }
+      a.pred[(int64_t)Q * a.c.nnz + pos] = y;
+    }
+  }
 }
 
-// dH2[s, c] = dy[s] * w3[c]
-__global__ void k_outer(float* dH, const float* dy, const float* w, int64_t rows, int cols) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < rows * cols) dH[i] = dy[i / cols] * w[i % cols];
+// Per-feature sums of x and x^2 over every (query, nonzero) sample of a
+// context, FP64, one partial per block (part[block][2F]); the host adds the
+// partials (the standardisation of the training inputs).
+__global__ __launch_bounds__(256) void k_warm_moments(WarmInferArgs a, double* part) {
+  __shared__ double red[4][2 * F];
+  const int64_t total = (int64_t)a.q.nq * a.c.nnz;
+  double s[2 * F] = {};
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * 256) {
+    const int Q = (int)(i / a.c.nnz);
+    const int64_t pos = i % a.c.nnz;
+    const int qb = a.q.q_ptr[Q], q = a.q.q_ptr[Q + 1] - qb;
+    float x[F];
+    all_feats(a, a.q.qc + qb, a.q.qn + qb, q, Q, pos, x);
+#pragma unroll
+    for (int k = 0; k < F; ++k) {
+      s[k] += x[k];
+      s[F + k] = fma((double)x[k], (double)x[k], s[F + k]);
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 2 * F; ++k) {
+    const double v = wave_sum_d(s[k]);
+    if (lane == 0) red[wv][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * F) {
+    double v = 0.0;
+    for (int w = 0; w < 4; ++w) v += red[w][threadIdx.x];
+    part[(int64_t)blockIdx.x * 2 * F + threadIdx.x] = v;
+  }
 }
 
-// out[c] = sum_s X[s, c]  (bias gradients); one thread per column.
-__global__ void k_colsum(const float* X, float* out, int64_t rows, int cols) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  float s = 0.f;
-  for (int64_t r = 0; r < rows; ++r) s += X[r * cols + c];
-  out[c] = s;
+// ---------------------------------------------------------------------------
+// Training: one block of kTrainThreads per group (pair).
+constexpr int kTrainThreads = 128;
+// LDS row per sample of a chunk: z[F], h1[H], dz1[H], dz2[H], h2 dy[H], dy.
+constexpr int kRowZ = 0, kRowH1 = F, kRowDz1 = F + H, kRowDz2 = F + 2 * H, kRowH2 = F + 3 * H,
+              kRowDy = F + 4 * H, kRow = F + 4 * H + 1;
+
+struct Fwd {
+  float z[F], a1[H], s1[H], h1[H], a2[H], s2[H], h2[H];
+  float y;
+};
+
+__device__ __forceinline__ void forward(const cfloat* p, const float (&x)[F], Fwd& f) {
+#pragma unroll
+  for (int k = 0; k < F; ++k) f.z[k] = (x[k] - p[kWarmXm + k]) * p[kWarmXs + k];
+#pragma unroll
+  for (int h = 0; h < H; ++h) {
+    float acc = p[kWarmB1 + h];
+#pragma unroll
+    for (int k = 0; k < F; ++k) acc = fmaf(p[kWarmW1 + h * F + k], f.z[k], acc);
+    f.a1[h] = acc;
+    f.s1[h] = sig2z(acc);
+    f.h1[h] = acc * f.s1[h];
+  }
+  float y = p[kWarmB3];
+#pragma unroll
+  for (int o = 0; o < H; ++o) {
+    float acc = p[kWarmB2 + o];
+#pragma unroll
+    for (int h = 0; h < H; ++h) acc = fmaf(p[kWarmW2 + o * H + h], f.h1[h], acc);
+    f.a2[o] = acc;
+    f.s2[o] = sig2z(acc);
+    f.h2[o] = acc * f.s2[o];
+    y = fmaf(p[kWarmW3 + o], f.h2[o], y);
+  }
+  f.y = y;
 }
 
-// Centred loss per group (one block per group, groups are contiguous sample
-// ranges): delta = yhat - y; loss += sum (delta - mean delta)^2 / n_total;
-// dy = 2 (delta - mean delta) / n_total.
-__global__ __launch_bounds__(256) void k_centred_loss(const float* yhat, const float* b3,
-                                                      const float* y, const int32_t* gptr,
-                                                      int64_t n_total, float* dy, double* loss) {
+__device__ __forceinline__ void train_sample(const WarmTrainArgs& t, int64_t s, float (&x)[F],
+                                             float& y) {
+  if (t.X) {
+#pragma unroll
+    for (int k = 0; k < F; ++k) x[k] = t.X[s * F + k];
+  } else {
+    const int Q = t.s_q[s];
+    const int qb = t.ctx.q.q_ptr[Q], q = t.ctx.q.q_ptr[Q + 1] - qb;
+    all_feats(t.ctx, t.ctx.q.qc + qb, t.ctx.q.qn + qb, q, Q, t.s_pos[s], x);
+  }
+  y = t.ysrc[t.yidx ? t.yidx[s] : s];
+}
+
+// Block reduction of a double over kTrainThreads threads (two waves).
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+  v = wave_sum_d(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1];
+}
+
+__global__ __launch_bounds__(kTrainThreads) void k_warm_train(WarmTrainArgs t) {
+  __shared__ float rows[kTrainThreads * kRow];
+  __shared__ double red[2];
   const int g = blockIdx.x;
-  const int s0 = gptr[g], s1 = gptr[g + 1];
-  __shared__ double red[256];
-  double acc = 0.0;
-  for (int s = s0 + threadIdx.x; s < s1; s += 256) acc += (double)(yhat[s] + *b3 - y[s]);
-  red[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+  const int64_t s0 = t.gptr[g], s1 = t.gptr[g + 1];
+  const cfloat* p = as_const(t.params);
+  // Pass 1: the group's mean residual (the centring).
+  double sd = 0.0;
+  for (int64_t s = s0 + threadIdx.x; s < s1; s += kTrainThreads) {
+    float x[F], y;
+    train_sample(t, s, x, y);
+    Fwd f;
+    forward(p, x, f);
+    sd += (double)f.y - (double)y;
+  }
+  sd = block_sum_d(sd, red);
+  const double mean = s1 > s0 ? sd / (double)(s1 - s0) : 0.0;
+  // Pass 2: per chunk of samples, the per-sample backward vectors go to LDS
+  // and every thread sums its parameters' gradients over the chunk.
+  constexpr int kPer = (kWarmParams + kTrainThreads - 1) / kTrainThreads;
+  float acc[kPer] = {};
+  double loss = 0.0;
+  for (int64_t c0 = s0; c0 < s1; c0 += kTrainThreads) {
+    const int cnt = (s1 - c0 < kTrainThreads) ? (int)(s1 - c0) : kTrainThreads;
+    const int64_t s = c0 + threadIdx.x;
+    if (threadIdx.x < cnt) {
+      float x[F], y;
+      train_sample(t, s, x, y);
+      Fwd f;
+      forward(p, x, f);
+      const double d = ((double)f.y - (double)y) - mean;
+      loss = fma(d, d, loss);
+      const float dy = (float)(2.0 * d / (double)t.n_total);
+      float* r = rows + threadIdx.x * kRow;
+      float dz2[H];
+#pragma unroll
+      for (int o = 0; o < H; ++o) {
+        dz2[o] = dy * p[kWarmW3 + o] * gelu_grad(f.a2[o], f.s2[o]);
+        r[kRowDz2 + o] = dz2[o];
+        r[kRowH2 + o] = dy * f.h2[o];
+      }
+#pragma unroll
+      for (int h = 0; h < H; ++h) {
+        float bk = 0.f;
+#pragma unroll
+        for (int o = 0; o < H; ++o) bk = fmaf(p[kWarmW2 + o * H + h], dz2[o], bk);
+        r[kRowDz1 + h] = bk * gelu_grad(f.a1[h], f.s1[h]);
+        r[kRowH1 + h] = f.h1[h];
+      }
+#pragma unroll
+      for (int k = 0; k < F; ++k) r[kRowZ + k] = f.z[k];
+      r[kRowDy] = dy;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kPer; ++m) {
+      const int pi = threadIdx.x + m * kTrainThreads;
+      int ca = -1, cb = -1;  // gradient = sum over rows of row[ca] * (cb < 0 ? 1 : row[cb])
+      if (pi >= kWarmW1 && pi < kWarmB1) {
+        const int h = (pi - kWarmW1) / F, k = (pi - kWarmW1) % F;
+        ca = kRowDz1 + h; cb = kRowZ + k;
+      } else if (pi >= kWarmB1 && pi < kWarmW2) {
+        ca = kRowDz1 + (pi - kWarmB1);
+      } else if (pi >= kWarmW2 && pi < kWarmB2) {
+        const int o = (pi - kWarmW2) / H, h = (pi - kWarmW2) % H;
+        ca = kRowDz2 + o; cb = kRowH1 + h;
+      } else if (pi >= kWarmB2 && pi < kWarmW3) {
+        ca = kRowDz2 + (pi - kWarmB2);
+      } else if (pi >= kWarmW3 && pi < kWarmB3) {
+        ca = kRowH2 + (pi - kWarmW3);
+      } else if (pi == kWarmB3) {
+        ca = kRowDy;
+      }
+      if (ca >= 0) {
+        float v = acc[m];
+        for (int rr = 0; rr < cnt; ++rr) {
+          const float* r = rows + rr * kRow;
+          v = fmaf(r[ca], cb >= 0 ? r[cb] : 1.f, v);
+        }
+        acc[m] = v;
+      }
+    }
     __syncthreads();
   }
-  const double mean = (s1 > s0) ? red[0] / (double)(s1 - s0) : 0.0;
-  __syncthreads();
-  double l = 0.0;
-  for (int s = s0 + threadIdx.x; s < s1; s += 256) {
-    const double d = (double)(yhat[s] + *b3 - y[s]) - mean;
-    l += d * d;
-    dy[s] = (float)(2.0 * d / (double)n_total);
+  float* out = t.gpart + (int64_t)g * kWarmParams;
+#pragma unroll
+  for (int m = 0; m < kPer; ++m) {
+    const int pi = threadIdx.x + m * kTrainThreads;
+    if (pi < kWarmParams) out[pi] = acc[m];  // 0 for the standardisation slots
   }
-  red[threadIdx.x] = l;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) atomicAdd(loss, red[0] / (double)n_total);
+  loss = block_sum_d(loss, red);
+  if (threadIdx.x == 0) t.lpart[g] = loss;
 }
 
-__global__ void k_sum_to(const float* x, int64_t n, float* out) {
-  __shared__ float red[256];
-  float s = 0.f;
-  for (int64_t i = threadIdx.x; i < n; i += 256) s += x[i];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
+// Adam over the trainable parameters (the standardisation slots stay): the
+// gradient is the sum of the groups' partials in group order; block 0 also
+// adds the groups' losses to *loss_acc.
+__global__ void k_warm_adam(float* p, const float* gpart, int groups, float* m, float* v, float lr,
+                            float b1, float b2, float eps, float c1, float c2, float wd,
+                            const double* lpart, double* loss_acc) {
+  const int i = kWarmW1 + blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && loss_acc) {
+    double l = 0.0;
+    for (int gg = 0; gg < groups; ++gg) l += lpart[gg];
+    *loss_acc += l;
   }
-  if (threadIdx.x == 0) *out = red[0];
-}
-
-__global__ void k_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
-                       float b1, float b2, float eps, float c1, float c2, float wd) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float gi = g[i] + wd * p[i];
+  if (i >= kWarmParams) return;
+  float gi = 0.f;
+  for (int gg = 0; gg < groups; ++gg) gi += gpart[(int64_t)gg * kWarmParams + i];
+  gi = fmaf(wd, p[i], gi);
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
@@ -192,242 +512,90 @@ __global__ void k_adam(float* p, const float* g, float* m, float* v, int64_t n, 
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
-// Weighted mean embeddings: out[r] = sum_e w_e E[id_e] / sum_e w_e over the
-// CSR row r (documents: words with c; queries: words with r).  One block of
-// 64 threads per row, threads over components.
-__global__ void k_mean_rows(const float* E, int D, const int32_t* ptr, const int32_t* ids,
-                            const int32_t* touched, const float* wts, float* out) {
-  const int row = blockIdx.x;
-  const int b = ptr[row], e = ptr[row + 1];
-  float wsum = 0.f;
-  for (int k = b; k < e; ++k) wsum += wts[k];
-  const float inv = wsum > 0.f ? 1.f / wsum : 0.f;
-  for (int c = threadIdx.x; c < D; c += blockDim.x) {
-    float s = 0.f;
-    for (int k = b; k < e; ++k) {
-      const int id = touched ? touched[ids[k]] : ids[k];
-      s += wts[k] * E[(int64_t)id * D + c];
-    }
-    out[(int64_t)row * D + c] = s * inv;
+// grad = sum of the groups' partials, *loss = sum of their losses.
+__global__ void k_warm_reduce(const float* gpart, int groups, const double* lpart, float* grad,
+                              double* loss) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    double l = 0.0;
+    for (int gg = 0; gg < groups; ++gg) l += lpart[gg];
+    *loss = l;
   }
-}
-
-// Training batch: X[s] = [E[word]; dbar[doc]; qbar[query]], y[s] = target.
-__global__ void k_gather_x(const float* E, int D, const int32_t* s_word, const int32_t* s_doc,
-                           const int32_t* s_query, const float* dbar, const float* qbar,
-                           int64_t n, float* X) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t F = 3 * (int64_t)D;
-  if (i >= n * F) return;
-  const int64_t s = i / F;
-  const int f = (int)(i % F);
-  float v;
-  if (f < D) v = E[(int64_t)s_word[s] * D + f];
-  else if (f < 2 * D) v = dbar[(int64_t)s_doc[s] * D + (f - D)];
-  else v = qbar[(int64_t)s_query[s] * D + (f - 2 * D)];
-  X[i] = v;
-}
-
-__global__ void k_gather_f(const float* src, const int64_t* idx, int64_t n, float* dst) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[idx[i]];
-}
-
-__global__ void k_add_rows(float* X, const float* b, int64_t rows, int cols) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < rows * cols) X[i] += b[i % cols];
-}
-
-__device__ __forceinline__ uint16_t to_bf16(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-
-__global__ void k_to_bf16(const float* x, int64_t n, uint16_t* out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = to_bf16(x[i]);
-}
-
-// ---------------------------------------------------------------------------
-// Inference over every (query, nonzero) sample s = Q * nnz + pos:
-//   h1 = gelu(At[t] + Bd[j] + Cq[Q]), h2 = gelu(W2 h1 + b2), y = w3 . h2 + b3.
-// Persistent blocks of kInferWaves waves, one per CU; W2 (BF16, rows padded
-// by 8) lives in LDS, shared by the block's waves.
-// Wave tile = 32 samples: lane l computes h1 for sample l % 32, features
-// 16 s + 8 (l / 32) .. +7 of each 16-wide step (exactly the A-operand
-// fragments of v_mfma_f32_32x32x16_bf16), then 8 column tiles x 16 steps of
-// MFMA against W2 fragments from LDS.
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
-constexpr int kW2Pitch = H + 8;
-
-constexpr int kInferWaves = 12;  // waves per block, all sharing the block's LDS copy of W2
-
-__global__ __launch_bounds__(64 * kInferWaves) void k_mlp_infer(MlpInferArgs a) {
-  extern __shared__ uint16_t w2s[];  // H x kW2Pitch
-  for (int i = threadIdx.x; i < H * H; i += 64 * kInferWaves)
-    w2s[(i / H) * kW2Pitch + (i % H)] = a.W2b[i];
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  // Tile = 32 consecutive nonzeros x one query, queries fastest: the waves
-  // of a block work on the same 32 nonzeros for different queries, so their
-  // At / Bd rows come from the CU's cache.
-  const int64_t npb = (a.nnz + 31) / 32;
-  const int64_t ntiles = npb * a.nq;
-  const int half = lane >> 5;
-  for (int64_t tile = (int64_t)blockIdx.x * kInferWaves + wv; tile < ntiles;
-       tile += (int64_t)gridDim.x * kInferWaves) {
-    const int Q = (int)(tile % a.nq);
-    const int64_t pos0 = (tile / a.nq) * 32;
-    const int64_t pl = pos0 + (lane & 31);
-    const int pos = (int)(pl < a.nnz ? pl : a.nnz - 1);  // tail lanes: a valid row, not stored
-    const float* at = a.At + (int64_t)a.doc_t[pos] * H;
-    const float* bd = a.Bd + (int64_t)a.doc_of[pos] * H;
-    const float* cq = a.Cq + (int64_t)Q * H;
-    bf16x8 h1[H / 16];
-#pragma unroll
-    for (int st = 0; st < H / 16; ++st) {
-      const int k0 = 16 * st + 8 * half;
-      // (__bf16) rounds to nearest even; pairs compile to v_cvt_pk_bf16_f32.
-#pragma unroll
-      for (int e = 0; e < 8; ++e)
-        h1[st][e] = (__bf16)gelu_fast(at[k0 + e] + bd[k0 + e] + cq[k0 + e]);
-    }
-    float yp[16] = {};
-#pragma unroll 1
-    for (int nt = 0; nt < H / 32; ++nt) {
-      const int col = nt * 32 + (lane & 31);
-      const uint16_t* wrow = w2s + col * kW2Pitch + 8 * half;
-      f32x16 acc = {};
-#pragma unroll
-      for (int st = 0; st < H / 16; ++st) {
-        const bf16x8 bfrag = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(wrow + 16 * st));
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(h1[st], bfrag, acc, 0, 0, 0);
-      }
-      const float b2 = a.b2[col], w3 = a.w3[col];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) yp[r] = fmaf(w3, gelu_fast(acc[r] + b2), yp[r]);
-    }
-    // Column sums: the 32 lanes of each half hold the 32 columns of a row
-    // (butterflies over xor 1..16 stay inside the half).
-#pragma unroll
-    for (int r = 0; r < 16; ++r) yp[r] = bfly_sum<1, 16>(yp[r]);
-    if ((lane & 31) == 0) {
-      float* out = a.pred + (int64_t)Q * a.nnz;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int64_t po = pos0 + 8 * (r >> 2) + 4 * half + (r & 3);
-        if (po < a.nnz) out[po] = yp[r] + a.b3;
-      }
-    }
-  }
+  if (i >= kWarmParams) return;
+  float gi = 0.f;
+  for (int gg = 0; gg < groups; ++gg) gi += gpart[(int64_t)gg * kWarmParams + i];
+  grad[i] = gi;
 }
 
 }  // namespace
 
-static unsigned blocks_for(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
-
-hipError_t launch_gemm(const GemmArgs& g, hipStream_t s) {
-  if (g.M <= 0 || g.N <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gemm, dim3((g.N + 63) / 64, (g.M + 63) / 64), dim3(256), 0, s, g);
-  return hipGetLastError();
-}
-
-hipError_t launch_bias_gelu(float* Z, float* Hout, const float* b, int64_t rows, int cols,
-                            hipStream_t s) {
-  hipLaunchKernelGGL(k_bias_gelu, dim3(blocks_for(rows * cols)), dim3(256), 0, s, Z, Hout, b,
-                     rows, cols);
-  return hipGetLastError();
-}
-
-hipError_t launch_gelu_back(float* dH, const float* Z, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_gelu_back, dim3(blocks_for(n)), dim3(256), 0, s, dH, Z, n);
-  return hipGetLastError();
-}
-
-hipError_t launch_outer(float* dH, const float* dy, const float* w, int64_t rows, int cols,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_outer, dim3(blocks_for(rows * cols)), dim3(256), 0, s, dH, dy, w, rows,
-                     cols);
-  return hipGetLastError();
-}
-
-hipError_t launch_colsum(const float* X, float* out, int64_t rows, int cols, hipStream_t s) {
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(cols, 64)), dim3(64), 0, s, X, out, rows, cols);
-  return hipGetLastError();
-}
-
-hipError_t launch_centred_loss(const float* yhat, const float* b3, const float* y,
-                               const int32_t* gptr, int groups, int64_t n_total, float* dy,
-                               double* loss, hipStream_t s) {
-  if (groups <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_centred_loss, dim3(groups), dim3(256), 0, s, yhat, b3, y, gptr, n_total,
-                     dy, loss);
-  return hipGetLastError();
-}
-
-hipError_t launch_sum_to(const float* x, int64_t n, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, s, x, n, out);
-  return hipGetLastError();
-}
-
-hipError_t launch_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
-                       float b2, float eps, int step, float wd, hipStream_t s) {
-  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, s, p, g, m, v, n, lr, b1, b2,
-                     eps, c1, c2, wd);
-  return hipGetLastError();
-}
-
-hipError_t launch_mean_rows(const float* E, int D, const int32_t* ptr, const int32_t* ids,
-                            const int32_t* touched, const float* wts, int rows, float* out,
-                            hipStream_t s) {
+hipError_t launch_word_norms(const float* E, const int32_t* ids, int rows, int D, float* out,
+                             hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mean_rows, dim3(rows), dim3(64), 0, s, E, D, ptr, ids, touched, wts, out);
+  hipLaunchKernelGGL(k_word_norms, dim3((rows + 3) / 4), dim3(256), 0, s, E, ids, rows, D, out);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_x(const float* E, int D, const int32_t* s_word, const int32_t* s_doc,
-                           const int32_t* s_query, const float* dbar, const float* qbar,
-                           int64_t n, float* X, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_x, dim3(blocks_for(n * 3 * (int64_t)D)), dim3(256), 0, s, E, D,
-                     s_word, s_doc, s_query, dbar, qbar, n, X);
+hipError_t launch_doc_feats(const int32_t* doc_ptr, const int32_t* doc_t, const float* doc_val,
+                            const float* wnorm, int N, float* dstat, float* dcdf, hipStream_t s) {
+  if (N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_doc_feats, dim3(N), dim3(64), 0, s, doc_ptr, doc_t, doc_val, wnorm, dstat,
+                     dcdf);
   return hipGetLastError();
 }
 
-hipError_t launch_gather_f(const float* src, const int64_t* idx, int64_t n, float* dst,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_f, dim3(blocks_for(n)), dim3(256), 0, s, src, idx, n, dst);
+hipError_t launch_query_feats(const int32_t* q_ptr, const int32_t* q_words, const float* q_r,
+                              const float* E, int D, int nq, float* qstat, float* qc, float* qn,
+                              hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_query_feats, dim3(nq), dim3(256), 0, s, q_ptr, q_words, q_r, E, D, qstat,
+                     qc, qn);
   return hipGetLastError();
 }
 
-hipError_t launch_add_rows(float* X, const float* b, int64_t rows, int cols, hipStream_t s) {
-  hipLaunchKernelGGL(k_add_rows, dim3(blocks_for(rows * cols)), dim3(256), 0, s, X, b, rows, cols);
+hipError_t launch_warm_fold(const float* params, const float* dstat, int N, const float* qstat,
+                            int nq, float* Pd, float* Pq, hipStream_t s) {
+  const int64_t n = ((int64_t)N + nq) * H;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_warm_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, params,
+                     dstat, N, qstat, nq, Pd, Pq);
   return hipGetLastError();
 }
 
-hipError_t launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(n)), dim3(256), 0, s, x, n, out);
+hipError_t launch_warm_infer(const WarmInferArgs& a, hipStream_t s) {
+  if (a.q.nq <= 0 || a.c.nnz <= 0) return hipSuccess;
+  const int64_t per = 256 * kInferChunks;
+  const dim3 grid((unsigned)((a.c.nnz + per - 1) / per), (unsigned)a.q.nq);
+  if (a.feats) hipLaunchKernelGGL(k_warm_infer<true>, grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL(k_warm_infer<false>, grid, dim3(256), 0, s, a);
   return hipGetLastError();
 }
 
-size_t mlp_infer_lds_bytes() { return sizeof(uint16_t) * (size_t)H * kW2Pitch; }
+hipError_t launch_warm_moments(const WarmInferArgs& a, double* part, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_warm_moments, dim3(blocks), dim3(256), 0, s, a, part);
+  return hipGetLastError();
+}
 
-hipError_t launch_mlp_infer(const MlpInferArgs& a, int n_cu, hipStream_t s) {
-  const int64_t total = (int64_t)a.nq * a.nnz;
-  if (total <= 0) return hipSuccess;
-  const int64_t tiles = (total + 31) / 32;
-  int64_t blocks = (tiles + kInferWaves - 1) / kInferWaves;
-  const int64_t cap = (int64_t)n_cu;  // one block per CU: W2 fills most of the LDS
-  if (blocks > cap) blocks = cap;
-  const size_t lds = mlp_infer_lds_bytes();
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k_mlp_infer),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_mlp_infer, dim3((unsigned)blocks), dim3(64 * kInferWaves), lds, s, a);
+hipError_t launch_warm_train(const WarmTrainArgs& t, hipStream_t s) {
+  if (t.groups <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_warm_train, dim3(t.groups), dim3(kTrainThreads), 0, s, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_warm_adam(float* p, const float* gpart, int groups, float* m, float* v,
+                            float lr, float b1, float b2, float eps, int step, float wd,
+                            const double* lpart, double* loss_acc, hipStream_t s) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  const int n = kWarmParams - kWarmW1;
+  hipLaunchKernelGGL(k_warm_adam, dim3((n + 255) / 256), dim3(256), 0, s, p, gpart, groups, m, v,
+                     lr, b1, b2, eps, c1, c2, wd, lpart, loss_acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_warm_reduce(const float* gpart, int groups, const double* lpart, float* grad,
+                              double* loss, hipStream_t s) {
+  hipLaunchKernelGGL(k_warm_reduce, dim3((kWarmParams + 255) / 256), dim3(256), 0, s, gpart,
+                     groups, lpart, grad, loss);
   return hipGetLastError();
 }
 

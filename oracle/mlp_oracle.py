@@ -1,28 +1,42 @@
-"""FP64 restatement of the learned warm start's training step -- test
-infrastructure only (the checker for the GPU forward/backward kernels).
+"""FP64 restatement of the learned warm start's features and training step --
+test infrastructure only (the checker for the GPU feature, forward and
+backward kernels).
 
-Model: x = [word embedding; document mean; query mean] (3D features),
-h1 = gelu(W1 x + b1), h2 = gelu(W2 h1 + b2), y = w3 . h2 + b3, with the tanh
-form gelu(z) = 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
+Features of a (query, document word) sample (15 per sample):
+  0 n = |e_w|, 1 n^2
+  2..6  document: c-weighted mean, std (population), min, max of its words'
+        norms, log n_j
+  7..11 query: r-weighted mean, std, min, max of its words' norms, log q
+  12 cdf = (sum of c over the document's words with a smaller norm, ties by
+     position, + c_e / 2) / sum c
+  13 T = numpy.interp(cdf, query mid-ranks, query norms sorted ascending),
+     mid-rank = (sum of r below + r_i / 2) / sum r
+  14 |lamda| (sqrt(T^2 + n^2) - mrow), mrow = min_i |e_w - e_i| over the
+     query's words
+Model: z = (x - xm) * xs, h1 = gelu(W1 z + b1), h2 = gelu(W2 h1 + b2),
+y = w3 . h2 + b3, gelu(a) = 0.5 a (1 + tanh(sqrt(2/pi) (a + 0.044715 a^3))).
 Loss: for contiguous groups g (one (query, document) pair each),
 L = sum_g sum_{s in g} (d_s - mean_g d)^2 / n, d = yhat - y.
-Parameters: one flat array W1 [256, 3D], b1, W2 [256, 256], b2, w3, b3.
+Parameters: one flat array xm [15], xs [15], W1 [16, 15], b1, W2 [16, 16],
+b2, w3, b3 (xm and xs are not trained: zero gradient).
 
-This function has no counterpart in the reference.  Pinned by central
-finite differences of the loss (test_mlp_oracle.py).
+These functions have no counterpart in the reference.  The loss gradient is
+pinned by central finite differences (tests/test_mlp_oracle.py); the
+features by a hand-worked example and by numpy / scipy routines there.
 """
 from __future__ import annotations
 
 import numpy as np
 
-H = 256
+H = 16
+F = 15
 C = np.sqrt(2.0 / np.pi)
 A = 0.044715
 
 
-def layout(D):
-    F = 3 * D
-    off = {"W1": 0, "b1": H * F}
+def layout():
+    off = {"xm": 0, "xs": F, "W1": 2 * F}
+    off["b1"] = off["W1"] + H * F
     off["W2"] = off["b1"] + H
     off["b2"] = off["W2"] + H * H
     off["w3"] = off["b2"] + H
@@ -31,13 +45,12 @@ def layout(D):
     return off
 
 
-def unpack(p, D):
-    L = layout(D)
-    F = 3 * D
+def unpack(p):
+    L = layout()
     p = np.asarray(p, dtype=np.float64)
-    return (p[L["W1"]:L["b1"]].reshape(H, F), p[L["b1"]:L["W2"]],
-            p[L["W2"]:L["b2"]].reshape(H, H), p[L["b2"]:L["w3"]], p[L["w3"]:L["b3"]],
-            p[L["b3"]])
+    return (p[L["xm"]:L["xs"]], p[L["xs"]:L["W1"]], p[L["W1"]:L["b1"]].reshape(H, F),
+            p[L["b1"]:L["W2"]], p[L["W2"]:L["b2"]].reshape(H, H), p[L["b2"]:L["w3"]],
+            p[L["w3"]:L["b3"]], p[L["b3"]])
 
 
 def gelu(z):
@@ -49,18 +62,18 @@ def gelu_grad(z):
     return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * C * (1.0 + 3.0 * A * z * z)
 
 
-def forward(X, p, D):
-    W1, b1, W2, b2, w3, b3 = unpack(p, D)
-    X = np.asarray(X, dtype=np.float64)
-    Z1 = X @ W1.T + b1
+def forward(X, p):
+    xm, xs, W1, b1, W2, b2, w3, b3 = unpack(p)
+    Z = (np.asarray(X, dtype=np.float64) - xm) * xs
+    Z1 = Z @ W1.T + b1
     H1 = gelu(Z1)
     Z2 = H1 @ W2.T + b2
     H2 = gelu(Z2)
-    return H2 @ w3 + b3, (X, Z1, H1, Z2, H2)
+    return H2 @ w3 + b3, (Z, Z1, H1, Z2, H2)
 
 
-def loss(X, y, group_ptr, p, D):
-    yhat, _ = forward(X, p, D)
+def loss(X, y, group_ptr, p):
+    yhat, _ = forward(X, p)
     d = yhat - np.asarray(y, dtype=np.float64)
     n = d.shape[0]
     tot = 0.0
@@ -70,10 +83,10 @@ def loss(X, y, group_ptr, p, D):
     return tot / n
 
 
-def loss_grad(X, y, group_ptr, p, D):
+def loss_grad(X, y, group_ptr, p):
     """(loss, flat gradient) in FP64."""
-    W1, b1, W2, b2, w3, b3 = unpack(p, D)
-    yhat, (X, Z1, H1, Z2, H2) = forward(X, p, D)
+    xm, xs, W1, b1, W2, b2, w3, b3 = unpack(p)
+    yhat, (Z, Z1, H1, Z2, H2) = forward(X, p)
     d = yhat - np.asarray(y, dtype=np.float64)
     n = d.shape[0]
     dy = np.zeros(n)
@@ -84,7 +97,7 @@ def loss_grad(X, y, group_ptr, p, D):
         seg = d[g0:g1] - d[g0:g1].mean()
         tot += float((seg ** 2).sum())
         dy[g0:g1] = 2.0 * seg / n
-    L = layout(D)
+    L = layout()
     g = np.zeros(L["total"])
     g[L["w3"]:L["b3"]] = H2.T @ dy
     g[L["b3"]] = dy.sum()
@@ -92,6 +105,60 @@ def loss_grad(X, y, group_ptr, p, D):
     g[L["W2"]:L["b2"]] = (dZ2.T @ H1).reshape(-1)
     g[L["b2"]:L["w3"]] = dZ2.sum(0)
     dZ1 = (dZ2 @ W2) * gelu_grad(Z1)
-    g[L["W1"]:L["b1"]] = (dZ1.T @ X).reshape(-1)
+    g[L["W1"]:L["b1"]] = (dZ1.T @ Z).reshape(-1)
     g[L["b1"]:L["W2"]] = dZ1.sum(0)
     return tot / n, g
+
+
+def _wstats(x, w):
+    w = np.asarray(w, dtype=np.float64)
+    m = float(np.sum(w * x) / w.sum())
+    s = float(np.sqrt(np.sum(w * (x - m) ** 2) / w.sum()))
+    return [m, s, float(x.min()), float(x.max()), float(np.log(len(x)))]
+
+
+def _midrank(x, w):
+    """(sum of w over entries with smaller x, ties by position, + w/2) / sum w."""
+    o = np.lexsort((np.arange(len(x)), x))
+    cum = np.cumsum(w[o]) - 0.5 * w[o]
+    out = np.empty(len(x))
+    out[o] = cum / w.sum()
+    return out, o
+
+
+def pair_features(vecs, q_words, q_w, d_words, d_vals, lam_abs):
+    """[n_words, 15] features of one (query, document) pair, FP64."""
+    V = np.asarray(vecs, dtype=np.float64)
+    nq = np.sqrt((V[q_words] ** 2).sum(1))
+    nd = np.sqrt((V[d_words] ** 2).sum(1))
+    qw = np.asarray(q_w, dtype=np.float64)
+    dv = np.asarray(d_vals, dtype=np.float64)
+    cdf, _ = _midrank(nd, dv)
+    qc, oq = _midrank(nq, qw)
+    T = np.interp(cdf, qc[oq], nq[oq])
+    M = np.sqrt(((V[d_words][:, None, :] - V[q_words][None, :, :]) ** 2).sum(-1))
+    mrow = M.min(1)
+    X = np.empty((len(d_words), F))
+    X[:, 0] = nd
+    X[:, 1] = nd ** 2
+    X[:, 2:7] = _wstats(nd, dv)
+    X[:, 7:12] = _wstats(nq, qw)
+    X[:, 12] = cdf
+    X[:, 13] = T
+    X[:, 14] = lam_abs * (np.sqrt(T ** 2 + nd ** 2) - mrow)
+    return X
+
+
+def batch_features(vecs, q_ptr, q_words, q_w, doc_ptr, doc_words, doc_vals, lam_abs):
+    """[n_queries, nnz, 15] features of every (query, nonzero), FP64."""
+    nq = len(q_ptr) - 1
+    N = len(doc_ptr) - 1
+    out = np.empty((nq, int(doc_ptr[-1]), F))
+    for Q in range(nq):
+        qs = slice(q_ptr[Q], q_ptr[Q + 1])
+        for j in range(N):
+            b, e = doc_ptr[j], doc_ptr[j + 1]
+            if e > b:
+                out[Q, b:e] = pair_features(vecs, q_words[qs], q_w[qs], doc_words[b:e],
+                                            doc_vals[b:e], lam_abs)
+    return out

@@ -182,17 +182,26 @@ def test_tile_class_boundaries_and_maximum_sizes(P, dg, oracle):
         w15 = e.result()[0]
         e.run(-10.0, max_iter=20000, tol=1e-4, accel=True)
         wt, itt, rt = e.result()
+        # the log-domain kernel on the same class edges (its own shapes)
+        e.run(-10.0, max_iter=15, log_domain=True)
+        wl15 = e.result()[0]
+        e.run(-10.0, max_iter=20000, tol=1e-4, accel=True, log_domain=True)
+        wlt, itl, rlt = e.result()
     wo15, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, v64, ptr, words, vals,
                                      -10.0, 15)
+    wl15o, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, v64, ptr, words, vals,
+                                      -10.0, 15, log_domain=True)
     truth, _, _ = oracle.perdoc_batch(qs.q_ptr, qs.words, qs.weights, v64, ptr, words, vals,
                                       -10.0, 100000, tol=1e-10)
     for k, q in enumerate(qs.lengths()):
         e15, et = rel_err(w15[k], wo15[k]), rel_err(wt[k], truth[k])
+        l15, lt = rel_err(wl15[k], wl15o[k]), rel_err(wlt[k], truth[k])
         say(f"  q={q:3d}: 15 iterations {e15:.2e}; Anderson to 1e-4 {et:.2e} "
-            f"(iters max {itt[k].max()})")
-        assert e15 <= 1e-5
-        assert et <= 2e-4
-    assert np.all(rt < 1e-4)
+            f"(iters max {itt[k].max()}); log domain: 15 iterations {l15:.2e}, Anderson "
+            f"{lt:.2e} (iters max {itl[k].max()})")
+        assert e15 <= 1e-5 and l15 <= 1e-5
+        assert et <= 1e-4 and lt <= 1e-4
+    assert np.all(rt < 1e-4) and np.all(rlt < 1e-4)
 
 
 def test_solver_graph_replays_match_direct_launches(P, dg):
@@ -513,25 +522,61 @@ def test_non_finite_embeddings_are_rejected_on_the_device(P, dg):
     assert rel_err(w[0], ref) < 1e-6
 
 
-def test_batched_full_size_properties(P, dg, oracle):
-    # the batched workload at its full size: 256 queries x 50,000 documents
-    say("[case] batched full size, residual 1e-4")
+def test_benchmarked_configurations_full_size(P, dg, oracle):
+    """Exactly what bench.py times, at its full size (256 queries x 50,000
+    documents): warm-start network trained with the bench's recipe, tile
+    shapes autotuned with Anderson steps, the solver stage replayed from its
+    captured graph, uniform and learned starts, Anderson steps to residual
+    1e-4; plus the plain-step uniform start.  The shortest, median and longest
+    queries against the first 400 documents: within 1e-4 of converged FP64
+    (the north star's bound), and the plain steps within 1e-4 of the FP64
+    oracle stopped by the same rule (sinkhorn_wmd_fused.cpp:245-393 iterated
+    to the residual)."""
     corpus, qs, p = dg.workload("batched", seed=0)
+    tc, tq = dg.training_set(corpus.vecs, n_queries=1000, n_docs=20, seed=101)
+    say("[case] benchmarked configurations at full size (256 x 50,000)")
+    with engine(P, tc) as e:
+        e.set_queries(tq.q_ptr, tq.words, tq.weights)
+        e.run(p["lamda"], max_iter=20000, tol=1e-6, export_duals=True, accel=True,
+              summarize=False)
+        e.mlp_set(P.mlp_init(seed=0))
+        e.mlp_train(epochs=6, seed=0)
+        params = e.mlp_get()
+    kw = dict(max_iter=p["max_iter"], tol=p["tol"], graph=True)
+    out = {}
     with engine(P, corpus) as e:
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
-        st = e.run(p["lamda"], max_iter=p["max_iter"], tol=p["tol"])
-        w, it, res = e.result()
-    say(f"  {st}")
-    assert st["variant"] == RES
-    assert np.all(np.isfinite(w)) and np.all(w > 0)
-    assert np.all(res < 1e-4)
+        e.mlp_set(params)
+        e.autotune(p["lamda"], iters=16, accel=True)
+        for name, extra in (("uniform, Anderson", dict(accel=True)),
+                            ("learned, Anderson", dict(accel=True, warm_start=True)),
+                            ("uniform, plain steps", dict())):
+            e.run(p["lamda"], summarize=False, **kw, **extra)  # capture
+            st = e.run(p["lamda"], **kw, **extra)  # replay
+            w, it, res = e.result()
+            say(f"  {name}: {st['ms_total']:.1f} ms, iterations mean {st['iters_mean']:.2f} "
+                f"max {st['iters_max']}, fallbacks {st['fallbacks']}")
+            assert st["variant"] == RES
+            assert np.all(np.isfinite(w)) and np.all(w > 0)
+            assert np.all(res < p["tol"])
+            out[name] = w
+    L = qs.lengths()
+    order = np.argsort(L, kind="stable")
+    sub = 400
+    b = int(corpus.doc_ptr[sub])
     vecs = corpus.vecs.astype(np.float64)
-    for k in (0, int(np.argmax(qs.lengths())), qs.n - 1):
-        wo, ito, _ = oracle.perdoc(*qs.query(k), vecs, corpus.doc_ptr, corpus.doc_words,
-                                   corpus.doc_vals, p["lamda"], p["max_iter"], tol=p["tol"])
-        err = rel_err(w[k], wo)
-        say(f"  query {k} (q={qs.lengths()[k]}): err {err:.2e}")
-        assert err <= 1e-4
+    for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        args = (*qs.query(k), vecs, corpus.doc_ptr[:sub + 1], corpus.doc_words[:b],
+                corpus.doc_vals[:b], p["lamda"])
+        truth, _, _ = oracle.perdoc(*args, 200000, tol=1e-10)
+        same, _, _ = oracle.perdoc(*args, p["max_iter"], tol=p["tol"])
+        errs = {name: rel_err(w[k, :sub], truth) for name, w in out.items()}
+        e_same = rel_err(out["uniform, plain steps"][k, :sub], same)
+        say(f"  query {k} (q={L[k]}): vs converged FP64 " +
+            ", ".join(f"{n} {v:.2e}" for n, v in errs.items()) +
+            f"; plain steps vs FP64 with the same stop rule {e_same:.2e}")
+        assert max(errs.values()) <= 1e-4
+        assert e_same <= 1e-4
 
 
 @pytest.mark.parametrize("omega", [1.3, 1.5, 1.7])
@@ -553,7 +598,7 @@ def test_over_relaxation_reaches_the_same_fixed_point(P, dg, oracle, omega):
         f"ratio {ito.sum() / itp.sum():.3f}; error vs converged FP64: plain {ep:.2e}, "
         f"omega {eo:.2e}")
     assert np.all(reso < 1e-4)
-    assert eo <= 2e-4
+    assert eo <= 1e-4
 
 
 def test_anderson_acceleration_reaches_the_same_fixed_point(P, dg, oracle):
@@ -578,7 +623,7 @@ def test_anderson_acceleration_reaches_the_same_fixed_point(P, dg, oracle):
         f"(max {ita.max()}), ratio {ita.sum() / its.sum():.3f}; error vs converged FP64: "
         f"omega {es:.2e}, Anderson {ea:.2e}")
     assert np.all(resa < 1e-4)
-    assert ea <= 2e-4
+    assert ea <= 1e-4
     assert ita.sum() < its.sum()
 
 
@@ -610,4 +655,4 @@ def test_anderson_progress_guard_on_hard_pairs(P, dg, oracle):
             f"{ita[0, 0]}; error vs converged FP64 {err:.2e}")
         assert np.isfinite(wa[0, 0]) and ra[0, 0] < 1e-4
         assert ita[0, 0] <= itp[0, 0] + 192
-        assert err <= 2e-4
+        assert err <= 1e-4
