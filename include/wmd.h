@@ -217,6 +217,29 @@ int wmd_context_set_queries(wmd_context* ctx, uint64_t n_queries, const int64_t*
 /* Solve every (query, doc) pair on the device; results stay on the device. */
 int wmd_context_run(wmd_context* ctx, double lamda, const wmd_options* opt, wmd_stats* stats);
 
+/* Allocation-free run of one query against the context's corpus (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The variant of wmd_context_set_queries + wmd_context_run for a
+ * single query whose q words (int32 vocabulary ids) and q weights (FP32, > 0)
+ * are in device memory, writing wmd[N] (FP64) and optionally iters[N] and
+ * resid[N] (may be NULL) to device memory.  Every per-query buffer (the
+ * kernel block, row minima, matrix-core query planes, counters) lives in a
+ * library-owned device scratch array of 128 MiB per GPU, held until the
+ * process ends; a query whose buffers exceed it (Vt * 4 * ceil4(q) bytes for
+ * the kernel block, Vt the corpus's distinct words) returns
+ * WMD_ERR_UNSUPPORTED.  The call allocates nothing (no hipMalloc, hipFree or
+ * allocating library routine) and blocks until the outputs are written.
+ * Resident variant, uniform start, no dual export; the matrix-core distance
+ * stage needs the corpus planes of an earlier wmd_context_run with it.  An
+ * invalid word or weight is detected on the device and returns WMD_ERR_ARG.
+ * All calls on one GPU share its scratch array: calls that overlap on one
+ * GPU (from several threads or contexts) race and must be serialised by the
+ * caller. */
+int wmd_context_run_query_noalloc(wmd_context* ctx, int32_t q, const int32_t* q_words,
+                                  const float* q_weights, double lamda, const wmd_options* opt,
+                                  double* wmd, int32_t* iters, float* resid);
+
 /* Copy results to the host: wmd[n_queries * N], iters and resid may be NULL. */
 int wmd_context_result(wmd_context* ctx, double* wmd, int32_t* iters, float* resid);
 

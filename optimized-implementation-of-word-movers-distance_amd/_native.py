@@ -45,7 +45,8 @@ MLP_FEATURES = 15
 EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
            "wmd_context_create", "wmd_context_destroy", "wmd_context_set_corpus",
            "wmd_context_set_queries", "wmd_context_run", "wmd_context_result",
-           "wmd_context_device_result", "wmd_num_shapes", "wmd_shape_info", "wmd_shape_domains",
+           "wmd_context_device_result", "wmd_context_run_query_noalloc", "wmd_num_shapes",
+           "wmd_shape_info", "wmd_shape_domains",
            "wmd_last_error",
            "wmd_mlp_param_count", "wmd_context_mlp_set", "wmd_context_mlp_get",
            "wmd_train_options_default", "wmd_context_mlp_train", "wmd_context_duals",
@@ -120,6 +121,8 @@ def load():
     L.wmd_context_set_queries.argtypes = [P, U64, P, P, P]
     L.wmd_context_run.argtypes = [P, C.c_double, C.POINTER(Options), C.POINTER(Stats)]
     L.wmd_context_result.argtypes = [P, P, P, P]
+    L.wmd_context_run_query_noalloc.argtypes = [P, C.c_int32, P, P, C.c_double,
+                                                C.POINTER(Options), P, P, P]
     L.wmd_context_device_result.argtypes = [P]
     L.wmd_context_device_result.restype = C.c_void_p
     L.wmd_num_shapes.argtypes = []
@@ -165,7 +168,8 @@ def load():
                  "wmd_prepared_text",
                  "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_sinkhorn_ex",
                  "wmd_context_create", "wmd_context_set_corpus", "wmd_context_set_queries",
-                 "wmd_context_run", "wmd_context_result", "wmd_num_shapes", "wmd_shape_info",
+                 "wmd_context_run", "wmd_context_result", "wmd_context_run_query_noalloc",
+                 "wmd_num_shapes", "wmd_shape_info",
                  "wmd_shape_domains",
                  "wmd_context_mlp_set", "wmd_context_mlp_get", "wmd_context_mlp_train",
                  "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm",

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -122,6 +123,42 @@ double iter_cost(int s, int cls) {
 
 }  // namespace
 
+// Device scratch of the allocation-free entry (wmd_context_run_query_noalloc):
+// a file-scope array of kScratchBytes on every GPU the library runs on, held
+// until the process ends.  Host code never passes the array's name to a
+// kernel: its device address, per device, comes from hipGetSymbolAddress at
+// context creation (scratch_address) and every use goes through that.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// Resolves the scratch array's address on `device` (which must be current)
+// once; later calls return the kept pointer.
+int scratch_address(int device, unsigned char** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    set_err("device %d beyond the %d the scratch table covers", device, kMaxDevices);
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_err("hipGetSymbolAddress(scratch) failed on device %d: %s", device,
+              hipGetErrorString(e));
+      return WMD_ERR_HIP;
+    }
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return WMD_OK;
+}
+}  // namespace
+
 // Side streams of a context.  The resident launches of a run fan out over
 // four lanes, the context stream and three side streams: the runtime maps
 // consecutive streams of a process to its (by default four) hardware queues
@@ -136,6 +173,7 @@ constexpr int kTuneCandidates = 6;
 
 struct wmd_context {
   int device = 0;
+  unsigned char* scratch = nullptr;  // this device's scratch array (scratch_address)
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipStream_t side[kSideStreams] = {};
@@ -772,8 +810,12 @@ int wmd_context_create(int device, wmd_context** out) {
     return WMD_ERR_ARG;
   }
   HIP_TRY(hipSetDevice(device));
+  unsigned char* scr = nullptr;
+  int rc = scratch_address(device, &scr);
+  if (rc != WMD_OK) return rc;
   wmd_context* c = new wmd_context();
   c->device = device;
+  c->scratch = scr;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = hipEventCreate(&c->ev[i]);
   for (int i = 0; i < kSideStreams && e == hipSuccess; ++i) {
@@ -1508,6 +1550,199 @@ int wmd_context_set_tuning(wmd_context* c, int32_t domain, const int32_t* table,
     }
   c->tune[lg ? 1 : 0].assign(table, table + (size_t)wmd::kNumDocClasses * kTuneStride);
   ++c->plan_epoch;
+  return WMD_OK;
+}
+
+int wmd_context_run_query_noalloc(wmd_context* c, int32_t q, const int32_t* q_words,
+                                  const float* q_weights, double lamda, const wmd_options* opt_in,
+                                  double* wmd_out, int32_t* iters_out, float* resid_out) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->has_corpus) {
+    set_err("set_corpus must be called before a query");
+    return WMD_ERR_STATE;
+  }
+  if (!q_words || !q_weights || !wmd_out || q < 1 || q > wmd::kMaxQ) {
+    set_err("need device arrays of 1..%d query words and weights and a result array (q=%d)",
+            wmd::kMaxQ, q);
+    return WMD_ERR_ARG;
+  }
+  wmd_options o;
+  if (opt_in) o = *opt_in; else wmd_options_default(&o);
+  if (!(lamda < 0.0) || !std::isfinite(lamda)) {
+    set_err("lamda must be finite and < 0 (K = exp(lamda * M))");
+    return WMD_ERR_ARG;
+  }
+  if (o.max_iter < 0 || o.max_iter > (1 << 24) || !(o.tol >= 0.0) ||
+      !(o.omega >= 1.0 && o.omega < 2.0) ||
+      (o.accel != WMD_ACCEL_NONE && o.accel != WMD_ACCEL_ANDERSON) ||
+      (o.domain != WMD_DOMAIN_LINEAR && o.domain != WMD_DOMAIN_LOG) ||
+      (o.cdist != WMD_CDIST_FP32 && o.cdist != WMD_CDIST_MFMA) || o.shape >= wmd::kNumShapes) {
+    set_err("bad options");
+    return WMD_ERR_ARG;
+  }
+  if (o.variant != WMD_VARIANT_RESIDENT || o.warm_start != WMD_WARM_UNIFORM || o.export_duals) {
+    set_err("the allocation-free entry runs the resident variant from the uniform start");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (o.accel != WMD_ACCEL_NONE && !(o.tol > 0.0)) {
+    set_err("Anderson acceleration runs with a tolerance (tol > 0)");
+    return WMD_ERR_UNSUPPORTED;
+  }
+  const bool log_domain = o.domain == WMD_DOMAIN_LOG;
+  const bool mfma = o.cdist == WMD_CDIST_MFMA;
+  if (c->nmax > wmd::kShapes[0].cap_n()) {
+    set_err("documents of %d words are beyond the resident tiles (%d)", c->nmax,
+            wmd::kShapes[0].cap_n());
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (o.shape >= 0 ? !shape_fits(o.shape, q, log_domain) : pick_shape(q, 0, log_domain) < 0) {
+    set_err("no tile shape of the domain holds %d query words", q);
+    return WMD_ERR_UNSUPPORTED;
+  }
+  if (mfma && !c->split_ready) {
+    set_err("the matrix-core distance stage needs the corpus planes of an earlier "
+            "wmd_context_run with cdist = WMD_CDIST_MFMA (or use WMD_CDIST_FP32)");
+    return WMD_ERR_STATE;
+  }
+  // Scratch layout, every part 256-byte aligned.
+  const int ks = round_up4(q);
+  const int Dp = ((int)c->D + 15) & ~15;
+  size_t at = 0;
+  auto take = [&](size_t bytes) {
+    const size_t off = at;
+    at += (bytes + 255) & ~(size_t)255;
+    return off;
+  };
+  const size_t o_words = take((size_t)q * 4), o_weights = take((size_t)q * 4);
+  const size_t o_qptr = take(8), o_koff = take(8), o_kstride = take(4), o_bucket = take(4);
+  const size_t o_fb = take(4), o_status = take(4);
+  const size_t o_K = take((size_t)c->Vt * ks * 4), o_mrow = take((size_t)c->Vt * 4);
+  const size_t o_Qs = mfma ? take((size_t)3 * q * Dp * 2) : 0;
+  const size_t o_nqw = mfma ? take((size_t)q * 8) : 0;
+  const size_t o_iters = iters_out ? 0 : take((size_t)c->N * 4);
+  const size_t o_resid = resid_out ? 0 : take((size_t)c->N * 4);
+  if (at > kScratchBytes) {
+    set_err("this query needs %zu bytes of scratch (kernel block %lld x %d), the scratch holds "
+            "%zu", at, (long long)c->Vt, ks, kScratchBytes);
+    return WMD_ERR_UNSUPPORTED;
+  }
+  unsigned char* S = c->scratch;
+  int32_t* words = reinterpret_cast<int32_t*>(S + o_words);
+  float* weights = reinterpret_cast<float*>(S + o_weights);
+  int32_t* qptr = reinterpret_cast<int32_t*>(S + o_qptr);
+  int64_t* koff = reinterpret_cast<int64_t*>(S + o_koff);
+  int32_t* kstride = reinterpret_cast<int32_t*>(S + o_kstride);
+  int32_t* bucket = reinterpret_cast<int32_t*>(S + o_bucket);
+  uint32_t* fb = reinterpret_cast<uint32_t*>(S + o_fb);
+  uint32_t* status = reinterpret_cast<uint32_t*>(S + o_status);
+  float* K = reinterpret_cast<float*>(S + o_K);
+  float* mrow = reinterpret_cast<float*>(S + o_mrow);
+
+  wmd::NoAllocSetupArgs sa{};
+  sa.words_in = q_words;
+  sa.weights_in = q_weights;
+  sa.q = q;
+  sa.V = c->V;
+  sa.words = words;
+  sa.weights = weights;
+  sa.q_ptr = qptr;
+  sa.k_off = koff;
+  sa.k_stride = kstride;
+  sa.bucket = bucket;
+  sa.fallbacks = fb;
+  sa.status = status;
+  HIP_TRY(wmd::launch_noalloc_setup(sa, c->stream));
+  // Distance stage for the one query: its words are the only rows.
+  if (mfma) {
+    uint16_t* Qs = reinterpret_cast<uint16_t*>(S + o_Qs);
+    double* nqw = reinterpret_cast<double*>(S + o_nqw);
+    wmd::SplitArgs sp{};
+    sp.E = c->E.p;
+    sp.ids = words;
+    sp.rows = q;
+    sp.D = (int)c->D;
+    sp.Dp = Dp;
+    sp.out = Qs;
+    sp.norms = nqw;
+    HIP_TRY(wmd::launch_split_rows(sp, c->stream));
+    wmd::MfmaCdistArgs ma{};
+    ma.Es = c->Es.p;
+    ma.Qs = Qs;
+    ma.nt = c->nt.p;
+    ma.nq = nqw;
+    ma.E = c->E.p;
+    ma.touched = c->touched.p;
+    ma.q_words = words;
+    ma.q_ptr = qptr;
+    ma.k_off = koff;
+    ma.k_stride = kstride;
+    ma.K = K;
+    ma.mrow = mrow;
+    ma.D = (int)c->D;
+    ma.Dp = Dp;
+    ma.Vt = (int)c->Vt;
+    ma.Sq = q;
+    ma.lamda = (float)lamda;
+    ma.log_out = log_domain ? 1 : 0;
+    HIP_TRY(wmd::launch_cdist_mfma(ma, 1, c->stream));
+  } else {
+    wmd::CdistArgs ca{};
+    ca.E = c->E.p;
+    ca.touched = c->touched.p;
+    ca.q_ptr = qptr;
+    ca.q_words = words;
+    ca.k_off = koff;
+    ca.k_stride = kstride;
+    ca.K = K;
+    ca.mrow = mrow;
+    ca.D = (int)c->D;
+    ca.Vt = (int)c->Vt;
+    ca.lamda = (float)lamda;
+    ca.log_out = log_domain ? 1 : 0;
+    HIP_TRY(wmd::launch_cdist(ca, 1, c->stream));
+  }
+  // Solver: one launch per document-length class, the shapes wmd_context_run
+  // picks for this query length.
+  wmd::SolveArgs a{};
+  a.K = K;
+  a.mrow = mrow;
+  a.k_off = koff;
+  a.k_stride = kstride;
+  a.q_ptr = qptr;
+  a.q_r = weights;
+  a.doc_ptr = c->doc_ptr.p;
+  a.doc_t = c->doc_t.p;
+  a.doc_val = c->doc_val.p;
+  a.nnz = c->nnz;
+  a.N = (int)c->N;
+  a.Vt = (int)c->Vt;
+  a.max_iter = o.max_iter;
+  a.tol = (float)o.tol;
+  a.omega = (float)o.omega;
+  a.accel = o.accel;
+  a.inv_lamda = (float)(1.0 / lamda);
+  a.lamda = (float)lamda;
+  a.wmd = wmd_out;
+  a.iters = iters_out ? iters_out : reinterpret_cast<int32_t*>(S + o_iters);
+  a.resid = resid_out ? resid_out : reinterpret_cast<float*>(S + o_resid);
+  a.fallbacks = fb;
+  a.lds_q = ks;
+  a.lds_n = round_up4(std::max(c->nmax, 1));
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
+    const int64_t d0 = c->h_class_off[cls], d1 = c->h_class_off[cls + 1];
+    if (d1 == d0) continue;
+    ResidentLaunch l{shape_for(c, cls, q, log_domain, o.shape), cls, 0, 1, d0, d1, 0.0};
+    HIP_TRY(launch_one(c, a, l, bucket, log_domain, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(c->h_status, status, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (c->h_status[0] != 0) {
+    set_err("a query word is outside [0, %lld) or a weight is not finite and > 0; the results "
+            "are not valid", (long long)c->V);
+    return WMD_ERR_ARG;
+  }
   return WMD_OK;
 }
 

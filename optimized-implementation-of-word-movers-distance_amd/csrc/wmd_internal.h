@@ -331,4 +331,22 @@ hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 hipError_t launch_embed(const double* src64, float* dst, uint64_t n, unsigned long long* first_bad,
                         hipStream_t s);
 
+// Allocation-free single-query run: the query's small arrays in the caller's
+// scratch (k_noalloc_setup, wmd_kernels.hip).
+struct NoAllocSetupArgs {
+  const int32_t* words_in;  // q vocabulary ids (caller's device memory)
+  const float* weights_in;  // q weights (caller's device memory)
+  int q;
+  int64_t V;
+  int32_t* words;           // q, validated copy
+  float* weights;           // q, validated copy
+  int32_t* q_ptr;           // 2
+  int64_t* k_off;           // 1
+  int32_t* k_stride;        // 1
+  int32_t* bucket;          // 1
+  uint32_t* fallbacks;      // 1
+  uint32_t* status;         // 1: 0 valid input, 1 invalid word or weight
+};
+hipError_t launch_noalloc_setup(const NoAllocSetupArgs& a, hipStream_t s);
+
 }  // namespace wmd

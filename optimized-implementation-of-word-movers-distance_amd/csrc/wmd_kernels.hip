@@ -829,4 +829,38 @@ hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s) {
   return hipGetLastError();
 }
 
+// One query's small device-side arrays for the allocation-free entry, in a
+// single block: the query words and weights validated and copied (an id
+// outside [0, V) or a weight that is not finite and > 0 is replaced by word 0,
+// weight 1, and sets *status to 1 so the host reports the input), q_ptr,
+// k_off, k_stride and bucket of a one-query batch, and a zeroed fallback
+// counter.  Every value is written, so nothing depends on what an earlier
+// call left in the scratch memory.
+__global__ __launch_bounds__(256) void k_noalloc_setup(NoAllocSetupArgs a) {
+  int bad = 0;
+  for (int i = threadIdx.x; i < a.q; i += 256) {
+    const int32_t w = a.words_in[i];
+    const float v = a.weights_in[i];
+    const bool ok = w >= 0 && (int64_t)w < a.V && v > 0.f && isfinite(v);
+    a.words[i] = ok ? w : 0;
+    a.weights[i] = ok ? v : 1.f;
+    bad |= ok ? 0 : 1;
+  }
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) {
+    a.q_ptr[0] = 0;
+    a.q_ptr[1] = a.q;
+    a.k_off[0] = 0;
+    a.k_stride[0] = (a.q + 3) & ~3;
+    a.bucket[0] = 0;
+    *a.fallbacks = 0u;
+    *a.status = bad ? 1u : 0u;
+  }
+}
+
+hipError_t launch_noalloc_setup(const NoAllocSetupArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_noalloc_setup, dim3(1), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 }  // namespace wmd
