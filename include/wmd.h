@@ -47,7 +47,10 @@ enum wmd_status {
 
 enum wmd_variant {
   /* one launch per tile shape; each (query, doc) tile stays in registers for
-   * all iterations (default) */
+   * all iterations (default).  Documents longer than the tiles (160 words)
+   * run in the same call in one more launch that iterates each of their
+   * pairs inside the kernel, re-reading the kernel rows from memory (plain
+   * steps from the uniform start; wmd_stats.streamed_docs counts them). */
   WMD_VARIANT_RESIDENT = 0,
   /* one fused SDDMM+SpMM launch per iteration (sinkhorn_wmd_fused.cpp:249-319) */
   WMD_VARIANT_FUSED = 1,
@@ -59,7 +62,7 @@ enum wmd_domain {
   /* FP32 scalings u, v on the row-rescaled kernel exp(lamda (M - min_i M)) */
   WMD_DOMAIN_LINEAR = 0,
   /* FP32 base-2 log potentials with log-sum-exp steps: for large |lamda|,
-   * where the kernel underflows FP32 (resident variant only) */
+   * where the kernel underflows FP32 (resident and fused variants) */
   WMD_DOMAIN_LOG = 1
 };
 
@@ -161,7 +164,11 @@ typedef struct wmd_stats {
   double ms_warm;      /* learned warm start: MLP inference stage */
   int32_t shared;      /* 1: the distance stage ran in the shared form */
   int32_t distinct;    /* distinct query words of the batch */
-  int32_t reserved[2];
+  int32_t streamed_docs;/* documents whose pairs ran outside the register
+                          tiles: in resident runs those longer than 160 words
+                          (plain steps from the uniform start, same stop
+                          rule), in streaming runs every document */
+  int32_t reserved[1];
 } wmd_stats;
 
 /* Default options: max_iter 15, resident variant, tol 0, automatic shape,

@@ -76,7 +76,8 @@ class Stats(C.Structure):
                 ("pairs", C.c_int64), ("iters_max", C.c_int32), ("launches", C.c_int32),
                 ("iters_mean", C.c_double), ("resid_max", C.c_double),
                 ("variant", C.c_int32), ("fallbacks", C.c_int32), ("ms_warm", C.c_double),
-                ("shared", C.c_int32), ("distinct", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("shared", C.c_int32), ("distinct", C.c_int32), ("streamed_docs", C.c_int32),
+                ("reserved", C.c_int32 * 1)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -100,7 +100,9 @@ struct ScopedDev {
 };
 
 // One resident launch: queries bucket[off, off + cnt) against the documents
-// doc_perm[d0, d1) of one length class, with one tile shape.
+// doc_perm[d0, d1) of one length class, with one tile shape; or, with shape
+// kLongLaunch, against the documents long_docs[d0, d1) beyond the tiles.
+constexpr int kLongLaunch = -1;
 struct ResidentLaunch {
   int shape, cls;
   size_t off;
@@ -205,6 +207,9 @@ struct wmd_context {
   DevBuf<float> E;
   DevBuf<int32_t> touched, doc_ptr, doc_t;
   DevBuf<int32_t> doc_perm;             // documents sorted by length class
+  DevBuf<int32_t> long_docs;            // documents longer than the resident tiles
+  std::vector<int32_t> h_long;          // host copy
+  int n_long = 0;
   DevBuf<uint16_t> Es;                  // matrix-core stage: BF16 planes of touched rows
   DevBuf<double> nt;                    // FP64 squared norms of touched rows
   bool split_ready = false;
@@ -345,6 +350,11 @@ void add_launch(std::vector<ResidentLaunch>* plan, std::vector<int32_t>* flat,
   plan->push_back(l);
 }
 
+// Per-iteration estimate of a long-document pair (q query words, n document
+// words) in the units of iter_cost (wave instructions): two passes over the
+// q x n kernel entries by 64-lane waves.
+double long_iter_cost(int q, int n) { return 2.0 * (double)q * (double)n / 64.0; }
+
 // Every (document class, tile shape) group of the current queries; the
 // bucket array receives the query ids of each group (query-major pairs, so
 // concurrently running waves share one query's kernel rows in cache).
@@ -362,6 +372,17 @@ int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
     }
     for (int s = 0; s < wmd::kNumShapes; ++s) add_launch(plan, &flat, groups[s], s, cls, d0, d1);
   }
+  if (c->n_long > 0) {
+    // Every query against the documents beyond the tiles, in one launch.
+    ResidentLaunch l{kLongLaunch, -1, flat.size(), (int64_t)c->nq, 0, c->n_long, 0.0};
+    for (int Q = 0; Q < c->nq; ++Q) {
+      flat.push_back(Q);
+      const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
+      for (int32_t j : c->h_long)
+        l.work += 4.0 * kPriorIters * long_iter_cost(q, c->h_doc_ptr[j + 1] - c->h_doc_ptr[j]);
+    }
+    plan->push_back(l);
+  }
   HIP_TRY(c->plan_bucket.upload(flat.data(), flat.size(), c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->h_plan_flat = std::move(flat);
@@ -376,6 +397,20 @@ int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
 void measure_plan(wmd_context* c, const std::vector<int32_t>& it) {
   if (c->plan_measured || c->plan.empty() || c->h_doc_perm.empty()) return;
   for (ResidentLaunch& l : c->plan) {
+    if (l.shape == kLongLaunch) {
+      double w = 0.0;
+      for (int64_t k = 0; k < l.cnt; ++k) {
+        const int Q = c->h_plan_flat[l.off + k];
+        const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
+        for (int64_t d = l.d0; d < l.d1; ++d) {
+          const int32_t j = c->h_long[d];
+          w += (double)it[(size_t)Q * c->N + j] *
+               long_iter_cost(q, c->h_doc_ptr[j + 1] - c->h_doc_ptr[j]);
+        }
+      }
+      l.work = w;
+      continue;
+    }
     double iters = 0.0;
     for (int64_t k = 0; k < l.cnt; ++k) {
       const int32_t* row = it.data() + (size_t)c->h_plan_flat[l.off + k] * c->N;
@@ -392,9 +427,13 @@ hipError_t launch_one(wmd_context* c, const wmd::SolveArgs& base, const Resident
                       const int32_t* bucket, bool log_domain, hipStream_t s) {
   wmd::SolveArgs a = base;
   a.bucket_q = bucket + l.off;
-  a.doc_list = c->doc_perm.p + l.d0;
   a.n_list = (int)(l.d1 - l.d0);
   a.n_pairs = l.cnt * (l.d1 - l.d0);
+  if (l.shape == kLongLaunch) {
+    a.doc_list = c->long_docs.p + l.d0;
+    return wmd::launch_long(a, log_domain, s);
+  }
+  a.doc_list = c->doc_perm.p + l.d0;
   return wmd::launch_resident(l.shape, l.cls, log_domain, a, s);
 }
 
@@ -404,10 +443,15 @@ hipError_t launch_one(wmd_context* c, const wmd::SolveArgs& base, const Resident
 // fork from the context stream before and join it after.
 int launch_plan(wmd_context* c, const wmd::SolveArgs& base,
                 const std::vector<ResidentLaunch>& plan, bool log_domain, int* launches) {
+  // The long-document launch (few blocks, each iterating in the kernel)
+  // goes first so its blocks overlap the whole stage; the rest largest first.
   std::vector<size_t> order(plan.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
-                   [&](size_t x, size_t y) { return plan[x].work > plan[y].work; });
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    const bool lx = plan[x].shape == kLongLaunch, ly = plan[y].shape == kLongLaunch;
+    if (lx != ly) return lx;
+    return plan[x].work > plan[y].work;
+  });
   HIP_TRY(hipEventRecord(c->fork, c->stream));
   for (int k = 0; k < kSideStreams; ++k) HIP_TRY(hipStreamWaitEvent(c->side[k], c->fork, 0));
   double load[kLanes] = {};
@@ -844,6 +888,7 @@ void wmd_context_destroy(wmd_context* c) {
   c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
   c->doc_perm.release();
+  c->long_docs.release();
   c->Es.release(); c->nt.release(); c->Qs.release(); c->nqw.release();
   c->doc_of.release(); c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release();
   c->wnorm.release(); c->dstat.release(); c->dcdf.release(); c->qstat.release();
@@ -937,8 +982,8 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   HIP_TRY(c->doc_t.upload(dt.data(), dt.size(), c->stream));
   HIP_TRY(c->doc_val.upload(dv.data(), dv.size(), c->stream));
   // Documents grouped by length class (stable, so ids ascend inside a
-  // class); documents beyond the last class are left out, and the resident
-  // variant is then not used for this corpus.
+  // class); documents beyond the last class go to the streaming kernel in
+  // resident runs (long_docs, ascending).
   std::vector<int64_t> class_off(wmd::kNumDocClasses + 1, 0);
   std::vector<int32_t> perm;
   perm.reserve(N);
@@ -952,6 +997,10 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   }
   class_off[wmd::kNumDocClasses] = (int64_t)perm.size();
   HIP_TRY(c->doc_perm.upload(perm.data(), perm.size(), c->stream));
+  std::vector<int32_t> long_docs;
+  for (uint64_t j = 0; j < N; ++j)
+    if (dptr[j + 1] - dptr[j] > wmd::kShapes[0].cap_n()) long_docs.push_back((int32_t)j);
+  HIP_TRY(c->long_docs.upload(long_docs.data(), long_docs.size(), c->stream));
   std::vector<int32_t> doc_of(nnz);
   for (uint64_t j = 0; j < N; ++j)
     for (int32_t pos = dptr[j]; pos < dptr[j + 1]; ++pos) doc_of[pos] = (int32_t)j;
@@ -970,6 +1019,8 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   c->nnz = (int64_t)nnz;
   c->Vt = (int64_t)touched.size();
   c->nmax = nmax;
+  c->n_long = (int)long_docs.size();
+  c->h_long = long_docs;
   c->split_ready = false;
   c->has_corpus = true;
   ++c->plan_epoch;
@@ -1185,22 +1236,26 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     if (rc != WMD_OK) return rc;
   }
   if (variant == WMD_VARIANT_RESIDENT) {
+    // Documents beyond the tiles' length run on the streaming kernel (below);
+    // a query beyond every tile sends the whole run there.
     bool fits = true;
     for (int Q = 0; Q < c->nq && fits; ++Q) {
       const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
-      if (o.shape >= 0)
-        fits = shape_fits(o.shape, q, log_domain) && wmd::kShapes[o.shape].cap_n() >= c->nmax;
-      else
-        fits = pick_shape(q, c->nmax, log_domain) >= 0;
+      fits = o.shape >= 0 ? shape_fits(o.shape, q, log_domain) : pick_shape(q, 0, log_domain) >= 0;
     }
     if (!fits) {
       if (o.shape >= 0) {
-        set_err("tile shape %d cannot hold q or the longest document (%d words), or has no "
-                "log-domain form", o.shape, c->nmax);
+        set_err("tile shape %d cannot hold the longest query (%d words), or has no log-domain "
+                "form", o.shape, c->qmax);
         return WMD_ERR_UNSUPPORTED;
       }
-      variant = WMD_VARIANT_FUSED;  // documents or queries beyond the register tiles
+      variant = WMD_VARIANT_FUSED;  // queries beyond the register tiles
     }
+  }
+  const bool long_docs = variant == WMD_VARIANT_RESIDENT && c->n_long > 0;
+  if (long_docs && wmd::long_lds_bytes(round_up4(c->qmax), round_up4(c->nmax)) > 64 * 1024) {
+    set_err("documents of %d words are too long for the long-document kernel", c->nmax);
+    return WMD_ERR_UNSUPPORTED;
   }
   if ((learned || given || o.export_duals) && (variant != WMD_VARIANT_RESIDENT || log_domain)) {
     set_err("learned warm start and dual export run on the resident variant in the linear domain");
@@ -1210,10 +1265,8 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     set_err("Anderson acceleration runs on the resident variant with a tolerance (tol > 0)");
     return WMD_ERR_UNSUPPORTED;
   }
-  if (log_domain && variant != WMD_VARIANT_RESIDENT) {
-    set_err("the log domain runs on the resident variant only (documents up to %d words, "
-            "queries up to %d words)", wmd::kShapes[0].cap_n(),
-            wmd::kShapes[wmd::kNumShapes - 1].cap_q());
+  if (log_domain && variant == WMD_VARIANT_NONFUSED) {
+    set_err("the log domain runs on the resident and fused variants");
     return WMD_ERR_UNSUPPORTED;
   }
   if (variant != WMD_VARIANT_RESIDENT) {
@@ -1328,6 +1381,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     HIP_TRY(c->duals.ensure((size_t)c->nq * (size_t)c->nnz));
     a.duals = c->duals.p;
   }
+  a.log_domain = log_domain ? 1 : 0;
   HIP_TRY(hipEventRecord(c->ev[3], c->stream));
   if (variant == WMD_VARIANT_RESIDENT)
     rc = run_resident(c, a, o.shape, log_domain, o.graph != 0, &launches);
@@ -1358,6 +1412,8 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     stats->variant = variant;
     stats->shared = share ? 1 : 0;
     stats->distinct = c->Sd;
+    stats->streamed_docs = variant == WMD_VARIANT_RESIDENT ? (long_docs ? c->n_long : 0)
+                                                           : (int32_t)c->N;
     if (!o.summarize) return WMD_OK;
     const size_t np = (size_t)c->nq * c->N;
     std::vector<int32_t> it(np);
@@ -1435,10 +1491,8 @@ int wmd_context_autotune(wmd_context* c, double lamda, const wmd_options* opt_in
   o.warm_start = WMD_WARM_UNIFORM;
   o.export_duals = 0;
   o.summarize = 0;
-  if (c->has_corpus && c->has_queries &&
-      (c->nmax > wmd::kShapes[0].cap_n() || c->qmax > wmd::kMaxQ)) {
-    set_err("documents (%d words) or queries (%d words) beyond the resident tiles", c->nmax,
-            c->qmax);
+  if (c->has_corpus && c->has_queries && c->qmax > wmd::kMaxQ) {
+    set_err("queries (%d words) beyond the resident tiles", c->qmax);
     return WMD_ERR_UNSUPPORTED;
   }
   rc = wmd_context_run(c, lamda, &o, nullptr);

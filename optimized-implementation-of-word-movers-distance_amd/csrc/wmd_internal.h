@@ -98,8 +98,8 @@ struct SolveArgs {
   const int32_t* doc_t;    // nnz compact word ids (index into touched)
   const float* doc_val;    // nnz
   const int32_t* bucket_q; // queries of this launch
-  const int32_t* doc_list; // resident launches: documents of this launch
-  int n_list;              // resident launches: number of documents in doc_list
+  const int32_t* doc_list; // documents of this launch (streaming: null = all N)
+  int n_list;              // number of documents in doc_list
   int64_t n_pairs;
   int64_t nnz;
   int N;
@@ -118,6 +118,7 @@ struct SolveArgs {
   const int64_t* u_off;    // nq
   uint8_t* done;           // nq x N
   float* ow;               // nq x nnz SDDMM output (nonfused)
+  int log_domain;          // streaming: 1 = base-2 logits in K and log2 u in u
   // resident variant, learned warm start and training targets
   const float* warm;       // nq x nnz predicted log v (row-rescaled convention), or null
   float* duals;            // nq x nnz converged log v written by the final pass, or null
@@ -323,6 +324,10 @@ hipError_t launch_resident(int shape, int doc_class, bool log_domain, const Solv
 WMD_FOR_EACH_SHAPE(WMD_DECLARE_SHAPE)
 #undef WMD_DECLARE_SHAPE
 hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);
+// Pairs with documents beyond the register tiles (doc_list / n_list, every
+// query of bucket_q): one block per pair iterating in the kernel.
+hipError_t launch_long(const SolveArgs& a, bool log_domain, hipStream_t s);
+size_t long_lds_bytes(int lds_q, int lds_n);
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s);

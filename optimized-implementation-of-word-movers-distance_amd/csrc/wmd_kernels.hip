@@ -505,10 +505,17 @@ struct PairRef {
   int64_t out;
 };
 
+// Pair p of a streaming launch: query bucket_q[p / n] and document p % n of
+// every document (n = N) or of doc_list (n = n_list).
 __device__ __forceinline__ PairRef pair_ref(const SolveArgs& a, int64_t pair) {
   PairRef p;
-  p.Q = a.bucket_q[pair / a.N];
-  p.j = (int)(pair % a.N);
+  if (a.doc_list) {
+    p.Q = a.bucket_q[pair / a.n_list];
+    p.j = a.doc_list[pair % a.n_list];
+  } else {
+    p.Q = a.bucket_q[pair / a.N];
+    p.j = (int)(pair % a.N);
+  }
   p.qb = a.q_ptr[p.Q];
   p.q = a.q_ptr[p.Q + 1] - p.qb;
   p.ks = a.k_stride[p.Q];
@@ -523,7 +530,9 @@ __device__ __forceinline__ float wave_sum(float v) { return bfly_sum<1, 32>(v); 
 __global__ __launch_bounds__(64) void k_stream_init(SolveArgs a) {
   const PairRef p = pair_ref(a, blockIdx.x);
   float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
-  for (int i = threadIdx.x; i < p.ks; i += 64) ug[i] = (i < p.q) ? (float)p.q : 0.f;
+  // u = q (x = 1/q, sinkhorn_wmd_fused.cpp:188-191), or its base-2 log
+  const float u0 = a.log_domain ? __log2f((float)p.q) : (float)p.q;
+  for (int i = threadIdx.x; i < p.ks; i += 64) ug[i] = (i < p.q) ? u0 : 0.f;
   if (threadIdx.x == 0) {
     a.done[p.out] = 0;
     a.iters[p.out] = 0;
@@ -597,12 +606,82 @@ __global__ __launch_bounds__(64) void k_fused_step(SolveArgs a, int final_pass) 
     float s = 0.f;
     for (int i = 0; i < p.q; ++i) s = fmaf(row[i], u_sh[i], s);
     o_sh[e] = a.doc_val[p.db + e] / s;
+    if (final_pass && a.duals) a.duals[(int64_t)p.Q * a.nnz + p.db + e] = __logf(o_sh[e]);
   }
   __syncthreads();
   if (final_pass)
     final_sum(a, p, Kq, u_sh, o_sh, t_sh, m_sh);
   else
     spmm_update(a, p, Kq, u_sh, o_sh, t_sh, ug);
+}
+
+// Log-domain form of k_fused_step (large lambda, where the kernel underflows
+// FP32): the block holds base-2 logits L = lamda (M - mrow) log2(e) and u
+// holds f = log2 u.  Per iteration (main.py:75-77 in log-sum-exp form, as
+// k_resident_log):
+//   g_e = log2 c_e - LSE2_i(L[e,i] + f_i)    (lanes over document words)
+//   lx_i = LSE2_e(L[e,i] + g_e)              (lanes over query words)
+//   res = sum_i |2^(f_i + lx_i) - r_i|,  f_i = log2 r_i - lx_i
+// with two-pass (maximum, then sum of exp2) reductions; the final pass forms
+// sum P_ie M_ie in FP64 with M = mrow + L ln2 / lamda.
+__global__ __launch_bounds__(64) void k_fused_step_log(SolveArgs a, int final_pass) {
+  extern __shared__ float sh[];
+  const PairRef p = pair_ref(a, blockIdx.x);
+  if (!final_pass && a.done[p.out]) return;  // block-uniform
+  float* f_sh = sh;
+  float* g_sh = f_sh + a.lds_q;
+  float* m_sh = g_sh + a.lds_n;
+  int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
+  const float* Lq = a.K + a.k_off[p.Q];
+  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  for (int i = threadIdx.x; i < p.q; i += 64) f_sh[i] = ug[i];
+  for (int e = threadIdx.x; e < p.n; e += 64) {
+    const int t = a.doc_t[p.db + e];
+    t_sh[e] = t;
+    if (final_pass) m_sh[e] = a.mrow[(int64_t)p.Q * a.Vt + t];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < p.n; e += 64) {
+    const float* row = Lq + (int64_t)t_sh[e] * p.ks;
+    float mx = kLogZero;
+    for (int i = 0; i < p.q; ++i) mx = fmaxf(mx, row[i] + f_sh[i]);
+    float s = 0.f;
+    for (int i = 0; i < p.q; ++i) s += exp2f(row[i] + f_sh[i] - mx);
+    g_sh[e] = __log2f(a.doc_val[p.db + e]) - mx - __log2f(s);
+  }
+  __syncthreads();
+  if (final_pass) {
+    const double ln2 = 0.6931471805599453;
+    const double m_per_l = ln2 / (double)a.lamda;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < p.q; i += 64) {
+      for (int e = 0; e < p.n; ++e) {
+        const float L = Lq[(int64_t)t_sh[e] * p.ks + i];
+        const double lp = (double)L + (double)f_sh[i] + (double)g_sh[e];
+        acc += exp(lp * ln2) * ((double)m_sh[e] + (double)L * m_per_l);
+      }
+    }
+    acc = bfly_sum_d<1, 32>(acc);
+    if (threadIdx.x == 0) a.wmd[p.out] = (p.n > 0) ? acc : NAN;
+    return;
+  }
+  float part = 0.f;
+  for (int i = threadIdx.x; i < p.q; i += 64) {
+    float mx = kLogZero;
+    for (int e = 0; e < p.n; ++e) mx = fmaxf(mx, Lq[(int64_t)t_sh[e] * p.ks + i] + g_sh[e]);
+    float s = 0.f;
+    for (int e = 0; e < p.n; ++e) s += exp2f(Lq[(int64_t)t_sh[e] * p.ks + i] + g_sh[e] - mx);
+    const float lx = mx + __log2f(s);
+    const float ri = a.q_r[p.qb + i];
+    part += fabsf(exp2f(f_sh[i] + lx) - ri);
+    ug[i] = __log2f(ri) - lx;
+  }
+  part = wave_sum(part);
+  if (threadIdx.x == 0) {
+    a.resid[p.out] = part;
+    a.iters[p.out] += 1;
+    if (a.tol > 0.f && part < a.tol) a.done[p.out] = 1;
+  }
 }
 
 // Nonfused SDDMM (sinkhorn_wmd_nonfused.cpp:72-90): w_e = c_e / (K[t_e,:] . u).
@@ -650,9 +729,307 @@ __global__ __launch_bounds__(64) void k_spmm(SolveArgs a, int final_pass) {
     spmm_update(a, p, Kq, u_sh, o_sh, t_sh, ug);
 }
 
-// status[0] += number of pairs still iterating; status[1] = max residual bits.
-__global__ __launch_bounds__(256) void k_status(const uint8_t* done, const float* resid,
-                                                int64_t n, uint32_t* status) {
+// ---------------------------------------------------------------------------
+// Pairs of a resident run whose documents are longer than the register tiles
+// (documents doc_list[0 .. n_list), queries bucket_q): one block of
+// kLongThreads per pair iterates inside the kernel until the pair's stop
+// test, with the query-side scalings, the document-side quotients and the
+// word ids in LDS and the kernel rows re-read from the K block (L2) every
+// half-step.  Plain steps from the uniform start; the same iteration, stop
+// rule and final pass as k_fused_step / k_fused_step_log
+// (sinkhorn_wmd_fused.cpp:245-393), so one launch replaces the streaming
+// loop and runs concurrently with the other launches of the resident plan.
+constexpr int kLongThreads = 512;  // one thread per query word of the longest query
+constexpr int kLongWaves = kLongThreads / 64;
+
+// Block-wide sum of per-thread partials (every thread gets the same total:
+// the waves' sums added in wave order).
+template <typename T>
+__device__ __forceinline__ T long_block_sum(T v, T* red) {
+  if constexpr (sizeof(T) == 8) v = wave_sum_d(v); else v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T t = red[0];
+#pragma unroll
+  for (int w = 1; w < kLongWaves; ++w) t += red[w];
+  return t;
+}
+
+template <bool LOG>
+__global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
+  extern __shared__ float sh[];
+  __shared__ float red[kLongWaves];
+  __shared__ double redd[kLongWaves];
+  constexpr int kC = (kMaxQ + 63) / 64;  // 64-lane chunks of the longest query
+  const int64_t pair = blockIdx.x;
+  const int Q = a.bucket_q[pair / a.n_list];
+  const int j = a.doc_list[pair % a.n_list];
+  const int qb = a.q_ptr[Q];
+  const int q = a.q_ptr[Q + 1] - qb;
+  const int ks = a.k_stride[Q];
+  const int db = a.doc_ptr[j];
+  const int n = a.doc_ptr[j + 1] - db;
+  const int64_t out = (int64_t)Q * a.N + j;
+  const float* Kq = a.K + a.k_off[Q];
+  float* u_sh = sh;                  // u (linear) or f = log2 u
+  float* o_sh = u_sh + a.lds_q;      // o = c / s (linear) or g = log2 v
+  float* m_sh = o_sh + a.lds_n;      // row minima
+  int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
+  float* xw = reinterpret_cast<float*>(t_sh + a.lds_n);  // kLongWaves x lds_q partials
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int i = threadIdx.x;         // the query word this thread updates
+  const bool own = i < q;
+  const float ri = own ? a.q_r[qb + i] : 0.f;
+  const float lri = own ? __log2f(ri) : 0.f;
+  const float u0 = LOG ? __log2f((float)q) : (float)q;  // x = 1/q, u = q (:188-191)
+  if (own) u_sh[i] = u0;
+  for (int e = threadIdx.x; e < n; e += kLongThreads) {
+    const int t = a.doc_t[db + e];
+    t_sh[e] = t;
+    m_sh[e] = a.mrow[(int64_t)Q * a.Vt + t];
+  }
+  __syncthreads();
+  // Document-side half-step: o_e = c_e / sum_i K u (or its log-sum-exp
+  // form).  One wave per document word, lanes over the query words (the
+  // kernel row is read coalesced), four words in flight per wave.
+  auto sddmm = [&]() {
+    for (int e0 = wv; e0 < n; e0 += 4 * kLongWaves) {
+      const float* row[4];
+      bool live[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int e = e0 + r * kLongWaves;
+        live[r] = e < n;
+        row[r] = Kq + (int64_t)(live[r] ? t_sh[e] : 0) * ks;
+      }
+      float v[4];
+      if constexpr (LOG) {
+        float mx[4] = {kLogZero, kLogZero, kLogZero, kLogZero};
+        for (int ii = lane; ii < q; ii += 64) {
+          const float f = u_sh[ii];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) mx[r] = fmaxf(mx[r], row[r][ii] + f);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mx[r] = bfly_red<1, 32, kMax>(mx[r]);
+        float sm[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int ii = lane; ii < q; ii += 64) {
+          const float f = u_sh[ii];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sm[r] += exp2f(row[r][ii] + f - mx[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = mx[r] + __log2f(wave_sum(sm[r]));
+      } else {
+        float sm[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int ii = lane; ii < q; ii += 64) {
+          const float u = u_sh[ii];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) sm[r] = fmaf(row[r][ii], u, sm[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = wave_sum(sm[r]);
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int e = e0 + r * kLongWaves;
+          if (live[r]) {
+            const float c = a.doc_val[db + e];
+            o_sh[e] = LOG ? __log2f(c) - v[r] : c / v[r];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  };
+  // Query-side half-step, the sums over document words split across the
+  // waves (wave w takes words w, w + W, ...) with lanes over query words;
+  // thread i adds the waves' partials in wave order.  Returns x_i (log
+  // domain: lx_i = log2 x_i) for the thread's word.
+  auto spmm = [&]() -> float {
+    float p[kC];
+    if constexpr (LOG) {
+#pragma unroll
+      for (int c = 0; c < kC; ++c) p[c] = kLogZero;
+#pragma unroll 4
+      for (int e = wv; e < n; e += kLongWaves) {
+        const float* row = Kq + (int64_t)t_sh[e] * ks;
+        const float g = o_sh[e];
+#pragma unroll
+        for (int c = 0; c < kC; ++c) {
+          const int ii = lane + 64 * c;
+          if (ii < q) p[c] = fmaxf(p[c], row[ii] + g);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kC; ++c)
+        if (lane + 64 * c < q) xw[wv * a.lds_q + lane + 64 * c] = p[c];
+      __syncthreads();
+      float mx = kLogZero;
+      if (own)
+        for (int w = 0; w < kLongWaves; ++w) mx = fmaxf(mx, xw[w * a.lds_q + i]);
+      __syncthreads();
+      if (own) xw[i] = mx;  // the maxima, read by every wave below
+      __syncthreads();
+      float mxc[kC];
+#pragma unroll
+      for (int c = 0; c < kC; ++c) {
+        const int ii = lane + 64 * c;
+        mxc[c] = ii < q ? xw[ii] : 0.f;
+        p[c] = 0.f;
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int e = wv; e < n; e += kLongWaves) {
+        const float* row = Kq + (int64_t)t_sh[e] * ks;
+        const float g = o_sh[e];
+#pragma unroll
+        for (int c = 0; c < kC; ++c) {
+          const int ii = lane + 64 * c;
+          if (ii < q) p[c] += exp2f(row[ii] + g - mxc[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kC; ++c)
+        if (lane + 64 * c < q) xw[wv * a.lds_q + lane + 64 * c] = p[c];
+      __syncthreads();
+      float sum = 0.f;
+      if (own)
+        for (int w = 0; w < kLongWaves; ++w) sum += xw[w * a.lds_q + i];
+      __syncthreads();
+      return own ? mx + __log2f(sum) : 0.f;
+    } else {
+#pragma unroll
+      for (int c = 0; c < kC; ++c) p[c] = 0.f;
+#pragma unroll 4
+      for (int e = wv; e < n; e += kLongWaves) {
+        const float* row = Kq + (int64_t)t_sh[e] * ks;
+        const float o = o_sh[e];
+#pragma unroll
+        for (int c = 0; c < kC; ++c) {
+          const int ii = lane + 64 * c;
+          if (ii < q) p[c] = fmaf(row[ii], o, p[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kC; ++c)
+        if (lane + 64 * c < q) xw[wv * a.lds_q + lane + 64 * c] = p[c];
+      __syncthreads();
+      float x = 0.f;
+      if (own)
+        for (int w = 0; w < kLongWaves; ++w) x += xw[w * a.lds_q + i];
+      __syncthreads();
+      return x;
+    }
+  };
+  // The update rules of the resident solver on the thread's word: plain
+  // steps (f <- log2 r - log2 x), or Anderson acceleration with a history of
+  // one step on the centred potentials, the acceptance test on the residual
+  // and the 64-iteration progress guard (accel_update in wmd_resident.h).
+  bool plain = !a.accel;
+  float l = 0.f, dl = 0.f, pf = 0.f;  // centred potential, last step, its f
+  bool prev = false, check_due = false;
+  float res_prev = INFINITY, res_best = INFINITY, res_mark = INFINITY;
+  const float inv_q = 1.f / (float)q;
+  int it = 0;
+  float res = 0.f;
+  while (it < a.max_iter) {
+    sddmm();
+    const float xs = spmm();  // x_i, or log2 x_i in the log domain
+    const float cur = own ? u_sh[i] : 0.f;
+    const float ux = LOG ? exp2f(cur + xs) : cur * xs;
+    res = long_block_sum(own ? fabsf(ux - ri) : 0.f, red);
+    const float lx = LOG ? xs : (own ? __log2f(xs) : 0.f);
+    const float g = own ? lri - lx : 0.f;  // the plain update of log2 u
+    ++it;
+    const bool stop = a.tol > 0.f && res < a.tol;
+    if (plain) {
+      // the step is taken, as in the streaming form
+      if (own) u_sh[i] = LOG ? g : ri / xs;
+      __syncthreads();
+      if (stop) break;
+      continue;
+    }
+    if (stop) break;  // u keeps the verified point
+    res_best = fminf(res_best, res);
+    bool fire = false;
+    if (((it - 1) & 63) == 63) check_due = true;
+    if (check_due && res < INFINITY) {
+      check_due = false;
+      fire = res_best >= res_mark;
+      res_mark = res_best;
+    }
+    if (fire) {  // progress guard: plain steps from this point's plain update
+      plain = true;
+      if (own) u_sh[i] = LOG ? g : exp2f(g);
+      __syncthreads();
+      continue;
+    }
+    const bool ok = !prev || (res <= res_prev && res < INFINITY);
+    const float mean = long_block_sum(g, red) * inv_q;
+    const float f = own ? g - mean - l : 0.f;
+    const float df = f - pf;
+    const float d0 = long_block_sum(df * df, red);
+    const float d1 = long_block_sum(df * f, red);
+    const float gam = (ok && prev) ? d1 / fmaf(d0, 1.f + 1e-6f, 1e-30f) : 0.f;
+    if (ok) {
+      const float step = f - gam * (dl + df);
+      l += step;
+      dl = step;
+      pf = f;
+      res_prev = res;
+    } else {
+      l = (l - dl) + pf;  // the previous point's plain update
+    }
+    prev = ok;
+    if (own) u_sh[i] = LOG ? l : exp2f(l);
+    __syncthreads();
+  }
+  // Final half-step and the FP64 distance sum.
+  sddmm();
+  double acc = 0.0;
+  if constexpr (LOG) {
+    const double ln2 = 0.6931471805599453;
+    const double m_per_l = ln2 / (double)a.lamda;
+    if (own)
+      for (int e = 0; e < n; ++e) {
+        const float L = Kq[(int64_t)t_sh[e] * ks + i];
+        const double lp = (double)L + (double)u_sh[i] + (double)o_sh[e];
+        acc += exp(lp * ln2) * ((double)m_sh[e] + (double)L * m_per_l);
+      }
+  } else {
+    if (a.duals)
+      for (int e = threadIdx.x; e < n; e += kLongThreads)
+        a.duals[(int64_t)Q * a.nnz + db + e] = __logf(o_sh[e]);
+    if (own) {
+      double x = 0.0;
+      for (int e = 0; e < n; ++e) {
+        const float kv = Kq[(int64_t)t_sh[e] * ks + i];
+        if (kv >= kFltMin) {
+          const float Mv = m_sh[e] + __logf(kv) * a.inv_lamda;
+          x += (double)kv * (double)Mv * (double)o_sh[e];
+        }
+      }
+      acc = (double)u_sh[i] * x;
+    }
+  }
+  acc = long_block_sum(acc, redd);
+  if (threadIdx.x == 0) {
+    a.wmd[out] = (n > 0) ? acc : NAN;
+    a.iters[out] = it;
+    a.resid[out] = res;
+  }
+}
+
+// status[0] += number of the launch's pairs still iterating; status[1] = max
+// residual bits.
+__global__ __launch_bounds__(256) void k_status(SolveArgs a, uint32_t* status) {
+  const uint8_t* done = a.done;
+  const float* resid = a.resid;
+  const int64_t n = a.n_pairs;
   __shared__ uint32_t s_cnt;
   __shared__ uint32_t s_max;
   if (threadIdx.x == 0) { s_cnt = 0; s_max = 0; }
@@ -660,8 +1037,10 @@ __global__ __launch_bounds__(256) void k_status(const uint8_t* done, const float
   uint32_t cnt = 0, mx = 0;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
        p += (int64_t)gridDim.x * blockDim.x) {
-    cnt += done[p] ? 0u : 1u;
-    const uint32_t bits = __float_as_uint(resid[p]);  // residuals are >= 0 (or NaN)
+    const int64_t o = a.doc_list ? (int64_t)a.bucket_q[p / a.n_list] * a.N + a.doc_list[p % a.n_list]
+                                 : (int64_t)a.bucket_q[p / a.N] * a.N + p % a.N;
+    cnt += done[o] ? 0u : 1u;
+    const uint32_t bits = __float_as_uint(resid[o]);  // residuals are >= 0 (or NaN)
     mx = bits > mx ? bits : mx;
   }
   atomicAdd(&s_cnt, cnt);
@@ -755,6 +1134,20 @@ static size_t stream_lds(const SolveArgs& a) {
   return sizeof(float) * ((size_t)a.lds_q + 3 * (size_t)a.lds_n);
 }
 
+size_t long_lds_bytes(int lds_q, int lds_n) {
+  return sizeof(float) * ((size_t)lds_q + 3 * (size_t)lds_n + (size_t)kLongWaves * lds_q);
+}
+
+hipError_t launch_long(const SolveArgs& a, bool log_domain, hipStream_t s) {
+  if (a.n_pairs <= 0) return hipSuccess;
+  const size_t lds = long_lds_bytes(a.lds_q, a.lds_n);
+  if (log_domain)
+    hipLaunchKernelGGL(k_long<true>, dim3((unsigned)a.n_pairs), dim3(kLongThreads), lds, s, a);
+  else
+    hipLaunchKernelGGL(k_long<false>, dim3((unsigned)a.n_pairs), dim3(kLongThreads), lds, s, a);
+  return hipGetLastError();
+}
+
 hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_stream_init, dim3((unsigned)a.n_pairs), dim3(64), 0, s, a);
@@ -763,8 +1156,12 @@ hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s) {
 
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_fused_step, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
-                     final_pass ? 1 : 0);
+  if (a.log_domain)
+    hipLaunchKernelGGL(k_fused_step_log, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s,
+                       a, final_pass ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_fused_step, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
+                       final_pass ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -825,7 +1222,7 @@ hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s) {
   int blocks = (int)((n + 255) / 256);
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_status, dim3(blocks), dim3(256), 0, s, a.done, a.resid, n, status);
+  hipLaunchKernelGGL(k_status, dim3(blocks), dim3(256), 0, s, a, status);
   return hipGetLastError();
 }
 

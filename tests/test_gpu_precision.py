@@ -141,14 +141,32 @@ def test_log_domain_anderson_every_tile_shape(P, dg, oracle):
         assert err <= 1e-4, q
 
 
-def test_log_domain_needs_resident_variant(P, dg):
-    corpus, qs, p = dg.workload("tiny", seed=0)
+def test_log_domain_fused_streaming_matches_oracle(P, dg, oracle):
+    # the streaming form of the log-domain step (the long documents of a
+    # resident run take it): fixed iterations and residual 1e-4 at lambda = 50
+    # against the FP64 log-domain oracle; the nonfused variant has no log form
+    say("[case] log domain, fused streaming variant, lambda=50")
+    corpus, qs, p = dg.workload("precision", seed=0, N=400)
+    v64 = corpus.vecs.astype(np.float64)
     with P.WMDEngine(0) as e:
         e.set_corpus(corpus.vecs, corpus.word_major())
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(p["lamda"], max_iter=15, variant=P.VARIANT_FUSED, log_domain=True)
+        w15 = e.result()[0]
+        e.run(p["lamda"], max_iter=10000, tol=1e-4, variant=P.VARIANT_FUSED, log_domain=True,
+              check_every=4)
+        wt, it, res = e.result()
         with pytest.raises(P.WMDError) as ei:
-            e.run(-50.0, variant=P.VARIANT_FUSED, log_domain=True)
+            e.run(p["lamda"], variant=P.VARIANT_NONFUSED, log_domain=True)
         assert ei.value.status == P._native.WMD_ERR_UNSUPPORTED
+    args = (*qs.query(0), v64, corpus.doc_ptr, corpus.doc_words, corpus.doc_vals, p["lamda"])
+    o15, _, _ = oracle.perdoc(*args, 15, log_domain=True)
+    truth, _, _ = oracle.perdoc(*args, 100000, tol=1e-10, log_domain=True)
+    e15, et = rel_err(w15[0], o15), rel_err(wt[0], truth)
+    say(f"  15 iterations {e15:.2e}; residual 1e-4 (iters mean {it.mean():.1f}) vs converged "
+        f"{et:.2e}")
+    assert e15 <= 1e-5
+    assert np.all(res < 1e-4) and et <= 1e-4
 
 
 # ---------------------------------------------------------------------------
