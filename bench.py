@@ -325,6 +325,38 @@ def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") ->
     return 0
 
 
+def counter_sum(path: str, stage: str = "k_cdist") -> int:
+    """Per-step sums of every counter in a rocprofv3 counter_collection.csv,
+    per kernel family (k_resident, k_long, k_warm_infer, the distance stage,
+    other).  Dispatches are taken in dispatch order; a step starts at each
+    distance-stage dispatch that follows a non-distance dispatch."""
+    import csv
+    from collections import defaultdict
+    rows = []
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            rows.append((int(row.get("Dispatch_Id") or row.get("Correlation_Id") or 0),
+                         row["Kernel_Name"], row["Counter_Name"], float(row["Counter_Value"])))
+    rows.sort(key=lambda r: r[0])
+    fams = ("k_resident", "k_long", "k_warm_infer", "k_cdist", "k_share_rescale", "k_split_rows")
+    step, last_stage, last_id = 0, True, None
+    sums = defaultdict(float)
+    disp = defaultdict(set)
+    for did, name, cname, val in rows:
+        is_stage = stage in name
+        if did != last_id:
+            if is_stage and not last_stage:
+                step += 1
+            last_stage, last_id = is_stage, did
+        fam = next((f for f in fams if f in name), "other")
+        sums[(step, fam, cname)] += val
+        disp[(step, fam)].add(did)
+    for (st, fam, cname), v in sorted(sums.items()):
+        print(json.dumps({"step": st, "kernel": fam, "counter": cname, "sum": v,
+                          "dispatches": len(disp[(st, fam)])}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,11 +387,20 @@ def main() -> int:
                     help="only time a device copy of this many MiB (counter calibration)")
     ap.add_argument("--trace-span", metavar="KERNEL_TRACE_CSV",
                     help="only print the per-step span of the solver kernels in a trace")
+    ap.add_argument("--counter-sum", metavar="COUNTER_CSV",
+                    help="only print per-step counter sums per kernel family")
+    ap.add_argument("--save-mlp-params", metavar="NPY",
+                    help="save the trained warm-start network's parameters")
+    ap.add_argument("--mlp-params", metavar="NPY",
+                    help="load the warm-start network's parameters instead of training "
+                         "(profiled runs)")
     args = ap.parse_args()
     if args.hbm_copy_mib > 0:
         return hbm_copy(args.hbm_copy_mib)
     if args.trace_span:
         return trace_span(args.trace_span)
+    if args.counter_sum:
+        return counter_sum(args.counter_sum)
     mfma = args.cdist == "mfma"
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -379,8 +420,13 @@ def main() -> int:
     n_err = min(ERR_DOCS, corpus.N)
 
     learned = None
-    if args.warm == "learned" or not args.no_variants:
+    if args.mlp_params:
+        learned = {"params": np.load(args.mlp_params), "loaded_from": args.mlp_params}
+        log(f"warm-start network loaded from {args.mlp_params}")
+    elif args.warm == "learned" or not args.no_variants:
         learned = train_warm_start(P, dg, corpus.vecs, lamda, args.train_queries, args.train_docs)
+        if args.save_mlp_params:
+            np.save(args.save_mlp_params, learned["params"])
 
     eng = P.WMDEngine(0)
     eng.set_corpus(corpus.vecs, corpus.word_major())
