@@ -205,7 +205,7 @@ class WMDEngine:
         nat.check(self._L.wmd_context_mlp_get(self._h, out.ctypes.data, n), "wmd_context_mlp_get")
         return out
 
-    def mlp_train(self, epochs=6, pairs_per_batch=64, lr=3e-3, weight_decay=0.0, seed=0,
+    def mlp_train(self, epochs=30, pairs_per_batch=64, lr=3e-3, weight_decay=0.0, seed=0,
                   standardize=True):
         """Train on the duals of the last run with export_duals; returns the
         per-epoch mean centred squared error."""

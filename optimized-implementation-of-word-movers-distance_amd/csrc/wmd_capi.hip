@@ -1917,7 +1917,7 @@ uint64_t wmd_mlp_param_count(void) { return (uint64_t)wmd::kWarmParams; }
 void wmd_train_options_default(wmd_train_options* opt) {
   if (!opt) return;
   std::memset(opt, 0, sizeof(*opt));
-  opt->epochs = 6;
+  opt->epochs = 30;
   opt->pairs_per_batch = 64;
   opt->lr = 3e-3;
   opt->weight_decay = 0.0;

@@ -16,6 +16,7 @@
 #include "../../include/wmd.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cerrno>
 #include <cstdarg>
 #include <cstdio>
@@ -78,7 +79,10 @@ struct Scanner {
     if (p >= end) return false;
     char* q = nullptr;
     *v = std::strtod(p, &q);
-    if (q == p) return false;
+    // A value that is not a finite double ("nan", "inf", or beyond the range,
+    // which strtod saturates to infinity) is refused, as the reference's
+    // operator>> reads fail on it.
+    if (q == p || !std::isfinite(*v)) return false;
     p = q;
     return true;
   }

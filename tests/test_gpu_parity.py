@@ -595,7 +595,7 @@ def test_benchmarked_configurations_full_size(P, dg, oracle):
         e.run(p["lamda"], max_iter=20000, tol=1e-6, export_duals=True, accel=True,
               summarize=False)
         e.mlp_set(P.mlp_init(seed=0))
-        e.mlp_train(epochs=6, seed=0)
+        e.mlp_train(epochs=30, seed=0)
         params = e.mlp_get()
     kw = dict(max_iter=p["max_iter"], tol=p["tol"], graph=True)
     out = {}
