@@ -199,6 +199,12 @@ struct wmd_context {
   hipGraphExec_t solve_exec = nullptr;
   uint64_t solve_plan_id = 0;
   wmd::SolveArgs solve_args{};
+  // Streaming variants: the last captured chunk of iterations, replayed while
+  // its kernel arguments, length and form stay the same.
+  hipGraphExec_t stream_exec = nullptr;
+  wmd::SolveArgs stream_args{};
+  int stream_len = -1;
+  bool stream_fused = false, stream_tol = false;
 
   // corpus
   bool has_corpus = false;
@@ -229,6 +235,7 @@ struct wmd_context {
   DevBuf<int32_t> q_ptr, q_words, k_stride, bucket, plan_bucket;
   DevBuf<float> q_r;
   DevBuf<int64_t> k_off, u_off;
+  DevBuf<wmd::StreamQuery> qmeta;
   DevBuf<float> K, mrow;
   // shared distance stage: the batch's distinct query words (ascending), the
   // distinct index of every query word, pseudo-queries of 64 distinct words
@@ -279,10 +286,12 @@ int check_ctx(wmd_context* ctx) {
 // Fills SolveArgs fields common to every variant.
 wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   wmd::SolveArgs a{};
+  std::memset(&a, 0, sizeof(a));  // padding too: cached graphs compare arguments bytewise
   a.K = c->K.p;
   a.mrow = c->mrow.p;
   a.k_off = c->k_off.p;
   a.k_stride = c->k_stride.p;
+  a.qmeta = c->qmeta.p;
   a.q_ptr = c->q_ptr.p;
   a.q_r = c->q_r.p;
   a.doc_ptr = c->doc_ptr.p;
@@ -531,13 +540,15 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
   if (!fused) HIP_TRY(c->ow.ensure((size_t)c->nq * (size_t)c->nnz));
   a.bucket_q = c->bucket.p;
   a.n_pairs = (int64_t)c->nq * c->N;
+  if (a.n_pairs > INT32_MAX) {  // one block per pair, blockIdx.x the pair index
+    set_err("%lld (query, document) pairs exceed the streaming launch's 2^31 - 1 blocks",
+            (long long)a.n_pairs);
+    return WMD_ERR_UNSUPPORTED;
+  }
   a.u = c->u.p;
   a.u_off = c->u_off.p;
   a.done = c->done.p;
   a.ow = c->ow.p;
-  HIP_TRY(wmd::launch_stream_init(a, c->stream));
-  ++*launches;
-
   auto step = [&](void) -> hipError_t {
     if (fused) return wmd::launch_fused_step(a, false, c->stream);
     hipError_t e = wmd::launch_sddmm(a, c->stream);
@@ -545,17 +556,28 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
     return wmd::launch_spmm(a, false, c->stream);
   };
   const int per_step = fused ? 1 : 2;
+  int done_iters = 0;
+  if (o.max_iter > 0 && !a.log_domain) {
+    // The first iteration starts from the uniform u itself and initializes
+    // the per-pair state (no separate initialization launch).
+    a.first = 1;
+    HIP_TRY(step());
+    a.first = 0;
+    *launches += per_step;
+    done_iters = 1;
+  } else {
+    HIP_TRY(wmd::launch_stream_init(a, c->stream));
+    ++*launches;
+  }
   const bool use_tol = o.tol > 0.0;
   const int chunk = use_tol ? std::max(1, o.check_every > 0 ? o.check_every : 8) : o.max_iter;
-  int done_iters = 0;
-  hipGraphExec_t exec = nullptr;
-  int exec_len = -1;
   int rc = WMD_OK;
   while (done_iters < o.max_iter) {
     const int len = std::min(chunk, o.max_iter - done_iters);
-    if (len != exec_len) {
-      if (exec) (void)hipGraphExecDestroy(exec);
-      exec = nullptr;
+    if (!c->stream_exec || c->stream_len != len || c->stream_fused != fused ||
+        c->stream_tol != use_tol || std::memcmp(&c->stream_args, &a, sizeof(a)) != 0) {
+      if (c->stream_exec) (void)hipGraphExecDestroy(c->stream_exec);
+      c->stream_exec = nullptr;
       hipGraph_t graph = nullptr;
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       hipError_t cap = hipSuccess;
@@ -567,14 +589,19 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
         set_err("graph capture failed: %s", hipGetErrorString(cap != hipSuccess ? cap : end));
         return WMD_ERR_HIP;
       }
-      hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      hipError_t ie = hipGraphInstantiate(&c->stream_exec, graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ie != hipSuccess) {
+        c->stream_exec = nullptr;
         set_err("hipGraphInstantiate failed: %s", hipGetErrorString(ie));
         return WMD_ERR_HIP;
       }
-      exec_len = len;
+      c->stream_args = a;
+      c->stream_len = len;
+      c->stream_fused = fused;
+      c->stream_tol = use_tol;
     }
+    hipGraphExec_t exec = c->stream_exec;
     hipError_t le = hipGraphLaunch(exec, c->stream);
     if (le != hipSuccess) {
       set_err("hipGraphLaunch failed: %s", hipGetErrorString(le));
@@ -595,7 +622,6 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
       if (c->h_status[0] == 0) break;  // every pair has converged
     }
   }
-  if (exec) (void)hipGraphExecDestroy(exec);
   if (rc != WMD_OK) return rc;
   if (fused) {
     HIP_TRY(wmd::launch_fused_step(a, true, c->stream));
@@ -884,6 +910,7 @@ void wmd_context_destroy(wmd_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->E.release(); c->touched.release(); c->doc_ptr.release(); c->doc_t.release();
   c->doc_val.release(); c->q_ptr.release(); c->q_words.release(); c->k_stride.release();
+  c->qmeta.release();
   c->bucket.release(); c->q_r.release(); c->k_off.release(); c->u_off.release();
   c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();
@@ -900,6 +927,7 @@ void wmd_context_destroy(wmd_context* c) {
   c->d_words.release(); c->slot.release(); c->dq_ptr.release(); c->dk_stride.release();
   c->dk_off.release(); c->Dd.release();
   drop_solve_graph(c);
+  if (c->stream_exec) (void)hipGraphExecDestroy(c->stream_exec);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (int i = 0; i < kSideStreams; ++i) {
@@ -1108,6 +1136,10 @@ int wmd_context_set_queries(wmd_context* c, uint64_t n, const int64_t* qp, const
   HIP_TRY(c->k_stride.upload(hs.data(), hs.size(), c->stream));
   HIP_TRY(c->k_off.upload(ko.data(), ko.size(), c->stream));
   HIP_TRY(c->u_off.upload(uo.data(), uo.size(), c->stream));
+  std::vector<wmd::StreamQuery> qm(n);
+  for (uint64_t Q = 0; Q < n; ++Q)
+    qm[Q] = wmd::StreamQuery{hq[Q], (int32_t)(qp[Q + 1] - qp[Q]), hs[Q], 0, ko[Q], uo[Q]};
+  HIP_TRY(c->qmeta.upload(qm.data(), qm.size(), c->stream));
   HIP_TRY(c->K.ensure((size_t)ktot));
   HIP_TRY(c->mrow.ensure((size_t)n * c->Vt));
   HIP_TRY(c->wmd.ensure((size_t)n * c->N));
@@ -1270,7 +1302,8 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     return WMD_ERR_UNSUPPORTED;
   }
   if (variant != WMD_VARIANT_RESIDENT) {
-    const size_t lds = sizeof(float) * ((size_t)round_up4(c->qmax) + 3 * (size_t)round_up4(c->nmax));
+    const size_t lds = wmd::stream_lds_bytes(round_up4(std::max(c->qmax, 1)),
+                                             round_up4(std::max(c->nmax, 1)));
     if (lds > 64 * 1024) {
       set_err("query (%d words) or document (%d words) too long for the streaming kernels",
               c->qmax, c->nmax);

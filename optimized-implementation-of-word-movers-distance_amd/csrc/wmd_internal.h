@@ -87,6 +87,14 @@ constexpr int kShareLds = 64 * 1024;     // LDS bytes per block of the shared ep
 
 // Sinkhorn stage.  A pair is (query, document); pairs of one launch are
 // bucket_q[p / N] x (p % N), query-major.
+// Per-query fields of a streaming pair, loaded together (one scalar load):
+// first word in the batch, word count, kernel/u row stride and the query's
+// kernel and u block offsets.
+struct alignas(32) StreamQuery {
+  int32_t qb, q, ks, pad;
+  int64_t k_off, u_off;
+};
+
 struct SolveArgs {
   const float* K;
   const float* mrow;
@@ -118,7 +126,10 @@ struct SolveArgs {
   const int64_t* u_off;    // nq
   uint8_t* done;           // nq x N
   float* ow;               // nq x nnz SDDMM output (nonfused)
+  const StreamQuery* qmeta; // streaming: nq per-query fields
   int log_domain;          // streaming: 1 = base-2 logits in K and log2 u in u
+  int first;               // streaming, linear: the launch starts from the uniform u = q
+                           // (reads no u, stop flag or count; writes every stop flag)
   // resident variant, learned warm start and training targets
   const float* warm;       // nq x nnz predicted log v (row-rescaled convention), or null
   float* duals;            // nq x nnz converged log v written by the final pass, or null
@@ -328,6 +339,9 @@ hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s);
 // query of bucket_q): one block per pair iterating in the kernel.
 hipError_t launch_long(const SolveArgs& a, bool log_domain, hipStream_t s);
 size_t long_lds_bytes(int lds_q, int lds_n);
+// LDS bytes of a streaming launch (u and x / r per query word, word ids,
+// weights and row minima per document word).
+size_t stream_lds_bytes(int lds_q, int lds_n);
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s);

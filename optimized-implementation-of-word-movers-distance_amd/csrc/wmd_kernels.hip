@@ -20,6 +20,7 @@
 #include "wmd_internal.h"
 
 #include <cfloat>
+#include <type_traits>
 
 #include "wmd_lanes.h"
 
@@ -502,25 +503,28 @@ __global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
 // Streaming variants: one wave (block of 64) per pair per launch.
 struct PairRef {
   int Q, j, qb, q, ks, db, n;
-  int64_t out;
+  int64_t out, k_off, u_off;
 };
 
 // Pair p of a streaming launch: query bucket_q[p / n] and document p % n of
-// every document (n = N) or of doc_list (n = n_list).
-__device__ __forceinline__ PairRef pair_ref(const SolveArgs& a, int64_t pair) {
+// every document (n = N) or of doc_list (n = n_list).  The query's fields
+// come from one StreamQuery record, so the block's scalar loads form two
+// dependent rounds (bucket and document range, then the record).
+__device__ __forceinline__ PairRef pair_ref(const SolveArgs& a, uint32_t pair) {
   PairRef p;
-  if (a.doc_list) {
-    p.Q = a.bucket_q[pair / a.n_list];
-    p.j = a.doc_list[pair % a.n_list];
-  } else {
-    p.Q = a.bucket_q[pair / a.N];
-    p.j = (int)(pair % a.N);
-  }
-  p.qb = a.q_ptr[p.Q];
-  p.q = a.q_ptr[p.Q + 1] - p.qb;
-  p.ks = a.k_stride[p.Q];
+  const bool listed = a.doc_list != nullptr;
+  const uint32_t nd = listed ? (uint32_t)a.n_list : (uint32_t)a.N;
+  const uint32_t qi = pair / nd, ji = pair % nd;
+  p.Q = a.bucket_q[qi];
+  p.j = listed ? a.doc_list[ji] : (int)ji;
   p.db = a.doc_ptr[p.j];
   p.n = a.doc_ptr[p.j + 1] - p.db;
+  const StreamQuery m = a.qmeta[p.Q];
+  p.qb = m.qb;
+  p.q = m.q;
+  p.ks = m.ks;
+  p.k_off = m.k_off;
+  p.u_off = m.u_off;
   p.out = (int64_t)p.Q * a.N + p.j;
   return p;
 }
@@ -529,7 +533,7 @@ __device__ __forceinline__ float wave_sum(float v) { return bfly_sum<1, 32>(v); 
 
 __global__ __launch_bounds__(64) void k_stream_init(SolveArgs a) {
   const PairRef p = pair_ref(a, blockIdx.x);
-  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  float* ug = a.u + p.u_off + (int64_t)p.j * p.ks;
   // u = q (x = 1/q, sinkhorn_wmd_fused.cpp:188-191), or its base-2 log
   const float u0 = a.log_domain ? __log2f((float)p.q) : (float)p.q;
   for (int i = threadIdx.x; i < p.ks; i += 64) ug[i] = (i < p.q) ? u0 : 0.f;
@@ -540,79 +544,347 @@ __global__ __launch_bounds__(64) void k_stream_init(SolveArgs a) {
   }
 }
 
-// SpMM half of an iteration for one pair, lanes over query words: reads o (or
-// the nonfused w) from LDS, updates u in HBM, residual and stop flag.
-__device__ __forceinline__ void spmm_update(const SolveArgs& a, const PairRef& p,
-                                            const float* Kq, const float* u_sh,
-                                            const float* o_sh, const int* t_sh, float* ug) {
-  float part = 0.f;
-  for (int i = threadIdx.x; i < p.q; i += 64) {
-    float x = 0.f;
-    for (int e = 0; e < p.n; ++e) x = fmaf(Kq[(int64_t)t_sh[e] * p.ks + i], o_sh[e], x);
-    const float ri = a.q_r[p.qb + i];
-    part += fabsf(u_sh[i] * x - ri);
-    ug[i] = ri / x;
-  }
-  part = wave_sum(part);
-  if (threadIdx.x == 0) {
-    a.resid[p.out] = part;
-    a.iters[p.out] += 1;
-    if (a.tol > 0.f && part < a.tol) a.done[p.out] = 1;
-  }
+// Streaming iteration of one pair, one wave per pair.  A kernel row is read
+// as ks/4 16-byte chunks (rows are 16-byte aligned, padded with zeros to ks =
+// round_up4(q) floats).  S lanes take one document word row per pass, each
+// lane KS chunks of it (chunk ss + k S, k < KS), so a wave instruction reads
+// 64/S whole rows; KS = 1 up to 64 chunks (S = 2^ceil(log2(ks/4))), else 2 or
+// 4 with S = 64.  Eight chunks per lane are in flight (8/KS passes).  A row
+// serves both halves of the iteration:
+//   SDDMM  o_e = c_e / (K[t_e,:] . u)   per-lane partial dots summed over S lanes
+//   SpMM   x_i += K[t_e,i] o_e          per-lane partials, summed over the
+//                                       64/S row groups after the last pass
+//   u_i = r_i / x_i, residual sum_i |u_i x_i - r_i| with the old u
+// Everything a pair needs before its first pass (stop flag, word ids,
+// weights, row minima, u, r, iteration count) is loaded in one round ahead of
+// the block's only barrier, every address valid and every store unconditional
+// (lanes past the end write a spare LDS slot), so no load waits on a branch.
+// Queries of more than 1024 words take one row per step with u and x in LDS.
+// LDS (floats): c (or w), row minima, word ids (lds_n + 4 each), u and x / r
+// (lds_q + 4 each).
+constexpr int kStreamChunks = 8;  // 16-byte chunks in flight per lane
+constexpr int kStreamMaxKS = 4;   // chunks per lane per row (ks <= 1024)
+
+enum StreamPart { kFusedPart = 0, kSddmmPart = 1, kSpmmPart = 2 };
+
+// Smallest power of two >= x (x >= 1).
+__device__ __forceinline__ int ceil_pow2(int x) { return x <= 1 ? 1 : 1 << (32 - __clz(x - 1)); }
+
+// Sum over aligned groups of S consecutive lanes (S a block-uniform power of
+// two <= 64); every lane of a group gets the group's sum.
+__device__ __forceinline__ float group_sum(float v, int S) {
+  if (S > 1) v = xor_step<1>(v);
+  if (S > 2) v = xor_step<2>(v);
+  if (S > 4) v = xor_step<4>(v);
+  if (S > 8) v = xor_step<8>(v);
+  if (S > 16) v = xor_step<16>(v);
+  if (S > 32) v = xor_step<32>(v);
+  return v;
 }
 
-// Final pass for one pair: sum_i u_i sum_e K M o_e in FP64.
-__device__ __forceinline__ void final_sum(const SolveArgs& a, const PairRef& p,
-                                          const float* Kq, const float* u_sh,
-                                          const float* o_sh, const int* t_sh,
-                                          const float* m_sh) {
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < p.q; i += 64) {
-    double x = 0.0;
-    for (int e = 0; e < p.n; ++e) {
-      const float kv = Kq[(int64_t)t_sh[e] * p.ks + i];
-      if (kv >= kFltMin) {
-        const float Mv = m_sh[e] + __logf(kv) * a.inv_lamda;
-        x += (double)kv * (double)Mv * (double)o_sh[e];
+// Sum over the 64 / G lanes that share lane % G (G a block-uniform power of
+// two <= 64).
+__device__ __forceinline__ float strided_sum(float v, int G) {
+  if (G <= 1) v = xor_step<1>(v);
+  if (G <= 2) v = xor_step<2>(v);
+  if (G <= 4) v = xor_step<4>(v);
+  if (G <= 8) v = xor_step<8>(v);
+  if (G <= 16) v = xor_step<16>(v);
+  if (G <= 32) v = xor_step<32>(v);
+  return v;
+}
+
+__device__ __forceinline__ float comp(const float4& v, int j) {
+  return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w;
+}
+
+// One pair's streaming iteration (PART = kFusedPart), its SDDMM half writing
+// w to HBM (kSddmmPart) or its SpMM half reading w (kSpmmPart); the final
+// pass (FINAL, its own kernels so the iterations keep their registers) forms
+// sum_i u_i sum_e K M o_e in FP64 instead of the u update.
+template <int PART, bool FINAL>
+__device__ __forceinline__ void stream_pair(const SolveArgs& a) {
+  extern __shared__ float sh[];
+  const PairRef p = pair_ref(a, blockIdx.x);
+  const int lane = threadIdx.x;
+  const int q = p.q, n = p.n;
+  const int C4 = p.ks >> 2;  // chunks per row
+  const bool wide = C4 > 64 * kStreamMaxKS;
+  const int KS = C4 <= 64 ? 1 : (C4 <= 128 ? 2 : 4);
+  const int S = wide ? 64 : (KS == 1 ? ceil_pow2(C4) : 64);
+  const int lgS = 31 - __clz(S);
+  const int R = 64 >> lgS;
+  const int ss = lane & (S - 1), rs = lane >> lgS;
+  const int nstr = a.lds_n + 4, qstr = a.lds_q + 4;
+  float* c_sh = sh;
+  float* m_sh = c_sh + nstr;
+  int* t_sh = reinterpret_cast<int*>(m_sh + nstr);
+  float* u_sh = reinterpret_cast<float*>(t_sh + nstr);
+  float* xr_sh = u_sh + qstr;  // x (wide queries) or r
+  const float* Kq = a.K + p.k_off;
+  const float4* K4 = reinterpret_cast<const float4*>(Kq);
+  float* ug = a.u + p.u_off + (int64_t)p.j * p.ks;
+  float* w = nullptr;
+  if constexpr (PART != kFusedPart) w = a.ow + (int64_t)p.Q * a.nnz + p.db;
+
+  // One round of loads ahead of the barrier: u, r and the first 256
+  // document words' ids and weights (longer documents and queries loop).
+  // A first launch starts from the uniform u = q (sinkhorn_wmd_fused.cpp:188-191)
+  // and reads no u, stop flag or count.
+  // (The stop flag and count are read unconditionally, then ignored by a
+  // first launch: a read under a branch would be waited for at the branch.)
+  const bool first = a.first != 0;
+  const int stop_raw = FINAL ? 0 : (int)a.done[p.out];
+  const int iters_raw = (PART != kSddmmPart && !FINAL) ? a.iters[p.out] : 0;
+  float4 uq[kStreamMaxKS];
+  if (!wide) {
+    const float4* u4 = reinterpret_cast<const float4*>(ug);
+#pragma unroll
+    for (int k = 0; k < kStreamMaxKS; ++k) {
+      const float u0 = (float)q;
+      uq[k] = first ? make_float4(u0, u0, u0, u0) : u4[min(ss + k * S, C4 - 1)];
+    }
+  }
+  constexpr bool kNeedR = PART != kSddmmPart && !FINAL;
+  float rv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) rv[k] = kNeedR ? a.q_r[p.qb + min(lane + 64 * k, q - 1)] : 0.f;
+  for (int e0 = 0; e0 < n; e0 += 256) {
+    int tv[4];
+    float cv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = min(e0 + lane + 64 * k, n - 1);
+      tv[k] = a.doc_t[p.db + e];
+      if constexpr (PART == kSpmmPart) cv[k] = w[e];
+      else cv[k] = a.doc_val[p.db + e];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int e = e0 + lane + 64 * k;
+      const int d = e < n ? e : a.lds_n;  // spare slot
+      t_sh[d] = tv[k];
+      c_sh[d] = cv[k];
+    }
+    if constexpr (FINAL) {
+      float mv[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) mv[k] = a.mrow[(int64_t)p.Q * a.Vt + tv[k]];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int e = e0 + lane + 64 * k;
+        m_sh[e < n ? e : a.lds_n] = mv[k];
       }
     }
-    acc += (double)u_sh[i] * x;
   }
-  acc = bfly_sum_d<1, 32>(acc);
-  if (threadIdx.x == 0) a.wmd[p.out] = (p.n > 0) ? acc : NAN;
+  if constexpr (kNeedR) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int i = lane + 64 * k;
+      xr_sh[i < q ? i : a.lds_q] = rv[k];
+    }
+    if (!wide)
+      for (int i = 256 + lane; i < q; i += 64) xr_sh[i] = a.q_r[p.qb + i];
+  }
+  if (wide) {
+    for (int i = lane; i < q; i += 64) {
+      u_sh[i] = first ? (float)q : ug[i];
+      if (kNeedR) xr_sh[i] = 0.f;
+    }
+  }
+  __syncthreads();
+  if (!first && stop_raw) return;  // block-uniform
+  const int iters0 = first ? 0 : iters_raw;
+
+  // Final pass: sum_{e,i} u_i K_ei M_ei o_e with M = mrow + ln(K) / lamda and
+  // o_e s_e = c_e is sum_e c_e mrow_e + (ln 2 / lamda) sum_e o_e T_e,
+  // T_e = sum_i u_i K_ei log2 K_ei (FP32 partials of one sign; K + FLT_MIN
+  // keeps 0 log2 0 finite and changes no normal K); both sums in FP64.
+  double acc = 0.0, cm = 0.0;
+  float part = 0.f;  // residual
+  if (!wide) {
+#pragma unroll
+    for (int k = 0; k < kStreamMaxKS; ++k) {  // slots and words past the query
+      const int c = ss + k * S;
+      if (k >= KS || c >= C4) uq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (4 * c + 1 >= q) uq[k].y = 0.f;
+      if (4 * c + 2 >= q) uq[k].z = 0.f;
+      if (4 * c + 3 >= q) uq[k].w = 0.f;
+    }
+    const int npass = (n + R - 1) / R;
+    auto body = [&](auto ks_c) {
+      constexpr int KSC = decltype(ks_c)::value;
+      // passes in flight
+      constexpr int P = KSC == 1 ? 10 : kStreamChunks / KSC;
+      float4 xa[KSC];
+#pragma unroll
+      for (int k = 0; k < KSC; ++k) xa[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int ps = 0; ps < npass; ps += P) {
+        float4 kv[P][KSC];
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+          if (ps + pp < npass) {  // block-uniform
+            const int e = min((ps + pp) * R + rs, n - 1);
+            const uint32_t row = (uint32_t)t_sh[e] * (uint32_t)C4;
+#pragma unroll
+            for (int k = 0; k < KSC; ++k) kv[pp][k] = K4[row + (uint32_t)min(ss + k * S, C4 - 1)];
+          }
+        }
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+          if (ps + pp >= npass) break;  // block-uniform
+          const int e = (ps + pp) * R + rs;
+          const bool ok = e < n;
+          const int ec = ok ? e : n - 1;
+          float o;
+          float tl = 0.f;  // final pass: sum_i u_i K_ei log2 K_ei over the lane's words
+          if constexpr (FINAL) {
+#pragma unroll
+            for (int k = 0; k < KSC; ++k) {
+#pragma unroll
+              for (int jj = 0; jj < 4; ++jj) {
+                const float kvj = comp(kv[pp][k], jj);
+                tl = fmaf(comp(uq[k], jj) * kvj, __builtin_amdgcn_logf(kvj + kFltMin), tl);
+              }
+            }
+            tl = group_sum(tl, S);
+          }
+          float s = 0.f;  // K[t_e,:] . u
+          if constexpr (PART != kSpmmPart || FINAL) {
+#pragma unroll
+            for (int k = 0; k < KSC; ++k) {
+              s = fmaf(kv[pp][k].x, uq[k].x, s);
+              s = fmaf(kv[pp][k].y, uq[k].y, s);
+              s = fmaf(kv[pp][k].z, uq[k].z, s);
+              s = fmaf(kv[pp][k].w, uq[k].w, s);
+            }
+            s = group_sum(s, S);
+          }
+          if constexpr (PART == kSpmmPart) {
+            o = ok ? c_sh[ec] : 0.f;
+          } else {
+            o = ok ? fdiv(c_sh[ec], s) : 0.f;
+            if constexpr (PART == kSddmmPart) {
+              if (ok && ss == 0) w[e] = o;
+              continue;
+            }
+            if (FINAL && a.duals && ok && ss == 0)
+              a.duals[(int64_t)p.Q * a.nnz + p.db + e] = __logf(o);
+          }
+          if constexpr (FINAL) {
+            // (once per row: the group's first lane)
+            // (c_e = o_e s_e; the SpMM half holds w = o, not c)
+            if (ok && ss == 0) {
+              acc += (double)(o * tl);
+              const double ce = PART == kSpmmPart ? (double)o * (double)s : (double)c_sh[e];
+              cm += ce * (double)m_sh[e];
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < KSC; ++k) {
+              xa[k].x = fmaf(kv[pp][k].x, o, xa[k].x);
+              xa[k].y = fmaf(kv[pp][k].y, o, xa[k].y);
+              xa[k].z = fmaf(kv[pp][k].z, o, xa[k].z);
+              xa[k].w = fmaf(kv[pp][k].w, o, xa[k].w);
+            }
+          }
+        }
+      }
+      if constexpr (PART != kSddmmPart && !FINAL) {
+#pragma unroll
+        for (int k = 0; k < KSC; ++k) {
+          if (k * S < C4) {  // block-uniform
+            const float x4[4] = {strided_sum(xa[k].x, S), strided_sum(xa[k].y, S),
+                                 strided_sum(xa[k].z, S), strided_sum(xa[k].w, S)};
+            const int c = ss + k * S;
+            if (rs == 0 && c < C4) {
+#pragma unroll
+              for (int jj = 0; jj < 4; ++jj) {
+                const int i = 4 * c + jj;
+                if (i < q) {
+                  const float ri = xr_sh[i];
+                  part += fabsf(comp(uq[k], jj) * x4[jj] - ri);
+                  ug[i] = fdiv(ri, x4[jj]);
+                }
+              }
+            }
+          }
+        }
+      }
+    };
+    if (KS == 1) body(std::integral_constant<int, 1>{});
+    else if (KS == 2) body(std::integral_constant<int, 2>{});
+    else body(std::integral_constant<int, 4>{});
+    if constexpr (PART == kSddmmPart) return;
+  } else {
+    for (int e = 0; e < n; ++e) {
+      const float* row = Kq + (int64_t)t_sh[e] * p.ks;
+      float o;
+      if constexpr (PART == kSpmmPart) {
+        o = c_sh[e];
+      } else {
+        float s = 0.f;
+        for (int i = lane; i < q; i += 64) s = fmaf(row[i], u_sh[i], s);
+        s = group_sum(s, 64);
+        o = fdiv(c_sh[e], s);
+        if constexpr (PART == kSddmmPart) {
+          if (lane == 0) w[e] = o;
+          continue;
+        }
+        if (FINAL && a.duals && lane == 0)
+          a.duals[(int64_t)p.Q * a.nnz + p.db + e] = __logf(o);
+      }
+      if constexpr (FINAL) {
+        float tl = 0.f;
+        for (int i = lane; i < q; i += 64) {
+          const float kv = row[i];
+          tl = fmaf(u_sh[i] * kv, __builtin_amdgcn_logf(kv + kFltMin), tl);
+        }
+        tl = group_sum(tl, 64);
+        if (lane == 0) {
+          acc += (double)(o * tl);
+          float sw = 0.f;  // the SpMM half holds w = o: c_e = o_e s_e
+          if constexpr (PART == kSpmmPart) {
+            for (int i = 0; i < q; ++i) sw = fmaf(row[i], u_sh[i], sw);
+          }
+          const double ce = PART == kSpmmPart ? (double)o * (double)sw : (double)c_sh[e];
+          cm += ce * (double)m_sh[e];
+        }
+      } else {
+        for (int i = lane; i < q; i += 64) xr_sh[i] = fmaf(row[i], o, xr_sh[i]);  // lane-owned
+      }
+    }
+    if constexpr (PART == kSddmmPart) return;
+    if constexpr (!FINAL) {
+      for (int i = lane; i < q; i += 64) {
+        const float x = xr_sh[i];
+        const float ri = a.q_r[p.qb + i];
+        part += fabsf(u_sh[i] * x - ri);
+        ug[i] = fdiv(ri, x);
+      }
+    }
+  }
+  if constexpr (FINAL) {
+    acc = wave_sum_d(fma(acc, 0.6931471805599453 / (double)a.lamda, cm));
+    if (lane == 0) a.wmd[p.out] = (n > 0) ? acc : NAN;
+    return;
+  }
+  part = wave_sum(part);
+  if (lane == 0) {
+    a.resid[p.out] = part;
+    a.iters[p.out] = iters0 + 1;
+    const bool conv = a.tol > 0.f && part < a.tol;
+    if (conv || first) a.done[p.out] = conv ? 1 : 0;
+  }
 }
 
 // Fused SDDMM + SpMM for one pair (the GPU form of sinkhorn_wmd_fused.cpp:249-319,
 // doc-major so the x accumulation needs no atomics).
-__global__ __launch_bounds__(64) void k_fused_step(SolveArgs a, int final_pass) {
-  extern __shared__ float sh[];
-  const PairRef p = pair_ref(a, blockIdx.x);
-  if (!final_pass && a.done[p.out]) return;  // block-uniform
-  float* u_sh = sh;
-  float* o_sh = u_sh + a.lds_q;
-  float* m_sh = o_sh + a.lds_n;
-  int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
-  const float* Kq = a.K + a.k_off[p.Q];
-  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
-  for (int i = threadIdx.x; i < p.q; i += 64) u_sh[i] = ug[i];
-  for (int e = threadIdx.x; e < p.n; e += 64) {
-    const int t = a.doc_t[p.db + e];
-    t_sh[e] = t;
-    if (final_pass) m_sh[e] = a.mrow[(int64_t)p.Q * a.Vt + t];
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < p.n; e += 64) {
-    const float* row = Kq + (int64_t)t_sh[e] * p.ks;
-    float s = 0.f;
-    for (int i = 0; i < p.q; ++i) s = fmaf(row[i], u_sh[i], s);
-    o_sh[e] = a.doc_val[p.db + e] / s;
-    if (final_pass && a.duals) a.duals[(int64_t)p.Q * a.nnz + p.db + e] = __logf(o_sh[e]);
-  }
-  __syncthreads();
-  if (final_pass)
-    final_sum(a, p, Kq, u_sh, o_sh, t_sh, m_sh);
-  else
-    spmm_update(a, p, Kq, u_sh, o_sh, t_sh, ug);
+__global__ __launch_bounds__(64) void k_fused_step(SolveArgs a) {
+  stream_pair<kFusedPart, false>(a);
+}
+__global__ __launch_bounds__(64) void k_fused_final(SolveArgs a) {
+  stream_pair<kFusedPart, true>(a);
 }
 
 // Log-domain form of k_fused_step (large lambda, where the kernel underflows
@@ -632,8 +904,8 @@ __global__ __launch_bounds__(64) void k_fused_step_log(SolveArgs a, int final_pa
   float* g_sh = f_sh + a.lds_q;
   float* m_sh = g_sh + a.lds_n;
   int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
-  const float* Lq = a.K + a.k_off[p.Q];
-  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
+  const float* Lq = a.K + p.k_off;
+  float* ug = a.u + p.u_off + (int64_t)p.j * p.ks;
   for (int i = threadIdx.x; i < p.q; i += 64) f_sh[i] = ug[i];
   for (int e = threadIdx.x; e < p.n; e += 64) {
     const int t = a.doc_t[p.db + e];
@@ -685,49 +957,11 @@ __global__ __launch_bounds__(64) void k_fused_step_log(SolveArgs a, int final_pa
 }
 
 // Nonfused SDDMM (sinkhorn_wmd_nonfused.cpp:72-90): w_e = c_e / (K[t_e,:] . u).
-__global__ __launch_bounds__(64) void k_sddmm(SolveArgs a) {
-  extern __shared__ float sh[];
-  const PairRef p = pair_ref(a, blockIdx.x);
-  if (a.done[p.out]) return;
-  float* u_sh = sh;
-  const float* Kq = a.K + a.k_off[p.Q];
-  const float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
-  for (int i = threadIdx.x; i < p.q; i += 64) u_sh[i] = ug[i];
-  __syncthreads();
-  float* w = a.ow + (int64_t)p.Q * a.nnz + p.db;
-  for (int e = threadIdx.x; e < p.n; e += 64) {
-    const float* row = Kq + (int64_t)a.doc_t[p.db + e] * p.ks;
-    float s = 0.f;
-    for (int i = 0; i < p.q; ++i) s = fmaf(row[i], u_sh[i], s);
-    w[e] = a.doc_val[p.db + e] / s;
-  }
-}
+__global__ __launch_bounds__(64) void k_sddmm(SolveArgs a) { stream_pair<kSddmmPart, false>(a); }
 
 // Nonfused SpMM (sinkhorn_wmd_nonfused.cpp:237-261, final :279-306).
-__global__ __launch_bounds__(64) void k_spmm(SolveArgs a, int final_pass) {
-  extern __shared__ float sh[];
-  const PairRef p = pair_ref(a, blockIdx.x);
-  if (!final_pass && a.done[p.out]) return;
-  float* u_sh = sh;
-  float* o_sh = u_sh + a.lds_q;
-  float* m_sh = o_sh + a.lds_n;
-  int* t_sh = reinterpret_cast<int*>(m_sh + a.lds_n);
-  const float* Kq = a.K + a.k_off[p.Q];
-  float* ug = a.u + a.u_off[p.Q] + (int64_t)p.j * p.ks;
-  const float* w = a.ow + (int64_t)p.Q * a.nnz + p.db;
-  for (int i = threadIdx.x; i < p.q; i += 64) u_sh[i] = ug[i];
-  for (int e = threadIdx.x; e < p.n; e += 64) {
-    const int t = a.doc_t[p.db + e];
-    t_sh[e] = t;
-    o_sh[e] = w[e];
-    if (final_pass) m_sh[e] = a.mrow[(int64_t)p.Q * a.Vt + t];
-  }
-  __syncthreads();
-  if (final_pass)
-    final_sum(a, p, Kq, u_sh, o_sh, t_sh, m_sh);
-  else
-    spmm_update(a, p, Kq, u_sh, o_sh, t_sh, ug);
-}
+__global__ __launch_bounds__(64) void k_spmm(SolveArgs a) { stream_pair<kSpmmPart, false>(a); }
+__global__ __launch_bounds__(64) void k_spmm_final(SolveArgs a) { stream_pair<kSpmmPart, true>(a); }
 
 // ---------------------------------------------------------------------------
 // Pairs of a resident run whose documents are longer than the register tiles
@@ -1130,9 +1364,11 @@ hipError_t launch_resident(int shape, int doc_class, bool log_domain, const Solv
   return kResidentLaunch[shape](doc_class, log_domain, a, s);
 }
 
-static size_t stream_lds(const SolveArgs& a) {
-  return sizeof(float) * ((size_t)a.lds_q + 3 * (size_t)a.lds_n);
+size_t stream_lds_bytes(int lds_q, int lds_n) {
+  return sizeof(float) * (2 * ((size_t)lds_q + 4) + 3 * ((size_t)lds_n + 4));
 }
+
+static size_t stream_lds(const SolveArgs& a) { return stream_lds_bytes(a.lds_q, a.lds_n); }
 
 size_t long_lds_bytes(int lds_q, int lds_n) {
   return sizeof(float) * ((size_t)lds_q + 3 * (size_t)lds_n + (size_t)kLongWaves * lds_q);
@@ -1159,23 +1395,25 @@ hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s)
   if (a.log_domain)
     hipLaunchKernelGGL(k_fused_step_log, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s,
                        a, final_pass ? 1 : 0);
+  else if (final_pass)
+    hipLaunchKernelGGL(k_fused_final, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
   else
-    hipLaunchKernelGGL(k_fused_step, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
-                       final_pass ? 1 : 0);
+    hipLaunchKernelGGL(k_fused_step, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sddmm, dim3((unsigned)a.n_pairs), dim3(64),
-                     sizeof(float) * (size_t)a.lds_q, s, a);
+  hipLaunchKernelGGL(k_sddmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_spmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a,
-                     final_pass ? 1 : 0);
+  if (final_pass)
+    hipLaunchKernelGGL(k_spmm_final, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
+  else
+    hipLaunchKernelGGL(k_spmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
   return hipGetLastError();
 }
 
