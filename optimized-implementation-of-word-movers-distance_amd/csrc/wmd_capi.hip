@@ -236,6 +236,7 @@ struct wmd_context {
   DevBuf<float> q_r;
   DevBuf<int64_t> k_off, u_off;
   DevBuf<wmd::StreamQuery> qmeta;
+  DevBuf<int32_t> act;  // streaming: pairs still iterating (status scan's list)
   DevBuf<float> K, mrow;
   // shared distance stage: the batch's distinct query words (ascending), the
   // distinct index of every query word, pseudo-queries of 64 distinct words
@@ -540,8 +541,8 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
   if (!fused) HIP_TRY(c->ow.ensure((size_t)c->nq * (size_t)c->nnz));
   a.bucket_q = c->bucket.p;
   a.n_pairs = (int64_t)c->nq * c->N;
-  if (a.n_pairs > INT32_MAX) {  // one block per pair, blockIdx.x the pair index
-    set_err("%lld (query, document) pairs exceed the streaming launch's 2^31 - 1 blocks",
+  if (a.n_pairs > INT32_MAX) {  // pair indices (and the active list) are 32-bit
+    set_err("%lld (query, document) pairs exceed the streaming launch's 2^31 - 1 pairs",
             (long long)a.n_pairs);
     return WMD_ERR_UNSUPPORTED;
   }
@@ -571,6 +572,16 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
   }
   const bool use_tol = o.tol > 0.0;
   const int chunk = use_tol ? std::max(1, o.check_every > 0 ? o.check_every : 8) : o.max_iter;
+  const wmd::SolveArgs all_pairs = a;  // the final pass covers every pair
+  if (use_tol && done_iters < o.max_iter) {
+    // Iterations with a tolerance walk the list of pairs still iterating,
+    // rebuilt by each chunk's status scan (the first scan runs here).
+    HIP_TRY(c->act.ensure((size_t)a.n_pairs));
+    a.act = c->act.p;
+    a.n_act = c->status.p;
+    HIP_TRY(wmd::launch_status(a, c->status.p, c->stream));
+    *launches += 2;
+  }
   int rc = WMD_OK;
   while (done_iters < o.max_iter) {
     const int len = std::min(chunk, o.max_iter - done_iters);
@@ -623,6 +634,7 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
     }
   }
   if (rc != WMD_OK) return rc;
+  a = all_pairs;
   if (fused) {
     HIP_TRY(wmd::launch_fused_step(a, true, c->stream));
     ++*launches;
@@ -910,7 +922,7 @@ void wmd_context_destroy(wmd_context* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->E.release(); c->touched.release(); c->doc_ptr.release(); c->doc_t.release();
   c->doc_val.release(); c->q_ptr.release(); c->q_words.release(); c->k_stride.release();
-  c->qmeta.release();
+  c->qmeta.release(); c->act.release();
   c->bucket.release(); c->q_r.release(); c->k_off.release(); c->u_off.release();
   c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
   c->resid.release(); c->u.release(); c->ow.release(); c->done.release();

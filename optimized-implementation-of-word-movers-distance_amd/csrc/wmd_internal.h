@@ -128,6 +128,9 @@ struct SolveArgs {
   float* ow;               // nq x nnz SDDMM output (nonfused)
   const StreamQuery* qmeta; // streaming: nq per-query fields
   int log_domain;          // streaming: 1 = base-2 logits in K and log2 u in u
+  int32_t* act;            // streaming: pairs not yet stopped, listed by the status scan
+                           // (null: every pair)
+  const uint32_t* n_act;   // streaming: entries of act (the status scan's count)
   int first;               // streaming, linear: the launch starts from the uniform u = q
                            // (reads no u, stop flag or count; writes every stop flag)
   // resident variant, learned warm start and training targets

@@ -562,7 +562,6 @@ __global__ __launch_bounds__(64) void k_stream_init(SolveArgs a) {
 // Queries of more than 1024 words take one row per step with u and x in LDS.
 // LDS (floats): c (or w), row minima, word ids (lds_n + 4 each), u and x / r
 // (lds_q + 4 each).
-constexpr int kStreamChunks = 8;  // 16-byte chunks in flight per lane
 constexpr int kStreamMaxKS = 4;   // chunks per lane per row (ks <= 1024)
 
 enum StreamPart { kFusedPart = 0, kSddmmPart = 1, kSpmmPart = 2 };
@@ -602,15 +601,14 @@ __device__ __forceinline__ float comp(const float4& v, int j) {
 // w to HBM (kSddmmPart) or its SpMM half reading w (kSpmmPart); the final
 // pass (FINAL, its own kernels so the iterations keep their registers) forms
 // sum_i u_i sum_e K M o_e in FP64 instead of the u update.
-template <int PART, bool FINAL>
-__device__ __forceinline__ void stream_pair(const SolveArgs& a) {
+template <int PART, bool FINAL, int KSC>
+__device__ __forceinline__ void stream_pair_ks(const SolveArgs& a, const PairRef& p) {
   extern __shared__ float sh[];
-  const PairRef p = pair_ref(a, blockIdx.x);
   const int lane = threadIdx.x;
   const int q = p.q, n = p.n;
   const int C4 = p.ks >> 2;  // chunks per row
-  const bool wide = C4 > 64 * kStreamMaxKS;
-  const int KS = C4 <= 64 ? 1 : (C4 <= 128 ? 2 : 4);
+  constexpr bool wide = KSC == 0;
+  constexpr int KS = wide ? 1 : KSC;
   const int S = wide ? 64 : (KS == 1 ? ceil_pow2(C4) : 64);
   const int lgS = 31 - __clz(S);
   const int R = 64 >> lgS;
@@ -636,11 +634,11 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
   const bool first = a.first != 0;
   const int stop_raw = FINAL ? 0 : (int)a.done[p.out];
   const int iters_raw = (PART != kSddmmPart && !FINAL) ? a.iters[p.out] : 0;
-  float4 uq[kStreamMaxKS];
-  if (!wide) {
+  float4 uq[KS];
+  if constexpr (!wide) {
     const float4* u4 = reinterpret_cast<const float4*>(ug);
 #pragma unroll
-    for (int k = 0; k < kStreamMaxKS; ++k) {
+    for (int k = 0; k < KS; ++k) {
       const float u0 = (float)q;
       uq[k] = first ? make_float4(u0, u0, u0, u0) : u4[min(ss + k * S, C4 - 1)];
     }
@@ -683,10 +681,10 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
       const int i = lane + 64 * k;
       xr_sh[i < q ? i : a.lds_q] = rv[k];
     }
-    if (!wide)
+    if constexpr (!wide)
       for (int i = 256 + lane; i < q; i += 64) xr_sh[i] = a.q_r[p.qb + i];
   }
-  if (wide) {
+  if constexpr (wide) {
     for (int i = lane; i < q; i += 64) {
       u_sh[i] = first ? (float)q : ug[i];
       if (kNeedR) xr_sh[i] = 0.f;
@@ -702,32 +700,32 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
   // keeps 0 log2 0 finite and changes no normal K); both sums in FP64.
   double acc = 0.0, cm = 0.0;
   float part = 0.f;  // residual
-  if (!wide) {
+  if constexpr (!wide) {
 #pragma unroll
-    for (int k = 0; k < kStreamMaxKS; ++k) {  // slots and words past the query
+    for (int k = 0; k < KS; ++k) {  // slots and words past the query
       const int c = ss + k * S;
-      if (k >= KS || c >= C4) uq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (c >= C4) uq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
       if (4 * c + 1 >= q) uq[k].y = 0.f;
       if (4 * c + 2 >= q) uq[k].z = 0.f;
       if (4 * c + 3 >= q) uq[k].w = 0.f;
     }
     const int npass = (n + R - 1) / R;
-    auto body = [&](auto ks_c) {
-      constexpr int KSC = decltype(ks_c)::value;
-      // passes in flight
-      constexpr int P = KSC == 1 ? 10 : kStreamChunks / KSC;
-      float4 xa[KSC];
+    {
+      constexpr int KB = KS;
+      // passes in flight: chunks in flight per lane over chunks per row
+      constexpr int P = KB == 1 ? (PART == kFusedPart && !FINAL ? 10 : 8) : (KB == 2 ? 2 : 1);
+      float4 xa[KB];
 #pragma unroll
-      for (int k = 0; k < KSC; ++k) xa[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      for (int k = 0; k < KB; ++k) xa[k] = make_float4(0.f, 0.f, 0.f, 0.f);
       for (int ps = 0; ps < npass; ps += P) {
-        float4 kv[P][KSC];
+        float4 kv[P][KB];
 #pragma unroll
         for (int pp = 0; pp < P; ++pp) {
           if (ps + pp < npass) {  // block-uniform
             const int e = min((ps + pp) * R + rs, n - 1);
             const uint32_t row = (uint32_t)t_sh[e] * (uint32_t)C4;
 #pragma unroll
-            for (int k = 0; k < KSC; ++k) kv[pp][k] = K4[row + (uint32_t)min(ss + k * S, C4 - 1)];
+            for (int k = 0; k < KB; ++k) kv[pp][k] = K4[row + (uint32_t)min(ss + k * S, C4 - 1)];
           }
         }
 #pragma unroll
@@ -740,7 +738,7 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
           float tl = 0.f;  // final pass: sum_i u_i K_ei log2 K_ei over the lane's words
           if constexpr (FINAL) {
 #pragma unroll
-            for (int k = 0; k < KSC; ++k) {
+            for (int k = 0; k < KB; ++k) {
 #pragma unroll
               for (int jj = 0; jj < 4; ++jj) {
                 const float kvj = comp(kv[pp][k], jj);
@@ -752,7 +750,7 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
           float s = 0.f;  // K[t_e,:] . u
           if constexpr (PART != kSpmmPart || FINAL) {
 #pragma unroll
-            for (int k = 0; k < KSC; ++k) {
+            for (int k = 0; k < KB; ++k) {
               s = fmaf(kv[pp][k].x, uq[k].x, s);
               s = fmaf(kv[pp][k].y, uq[k].y, s);
               s = fmaf(kv[pp][k].z, uq[k].z, s);
@@ -781,7 +779,7 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
             }
           } else {
 #pragma unroll
-            for (int k = 0; k < KSC; ++k) {
+            for (int k = 0; k < KB; ++k) {
               xa[k].x = fmaf(kv[pp][k].x, o, xa[k].x);
               xa[k].y = fmaf(kv[pp][k].y, o, xa[k].y);
               xa[k].z = fmaf(kv[pp][k].z, o, xa[k].z);
@@ -792,7 +790,7 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
       }
       if constexpr (PART != kSddmmPart && !FINAL) {
 #pragma unroll
-        for (int k = 0; k < KSC; ++k) {
+        for (int k = 0; k < KB; ++k) {
           if (k * S < C4) {  // block-uniform
             const float x4[4] = {strided_sum(xa[k].x, S), strided_sum(xa[k].y, S),
                                  strided_sum(xa[k].z, S), strided_sum(xa[k].w, S)};
@@ -811,10 +809,7 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
           }
         }
       }
-    };
-    if (KS == 1) body(std::integral_constant<int, 1>{});
-    else if (KS == 2) body(std::integral_constant<int, 2>{});
-    else body(std::integral_constant<int, 4>{});
+    }
     if constexpr (PART == kSddmmPart) return;
   } else {
     for (int e = 0; e < n; ++e) {
@@ -880,11 +875,44 @@ __device__ __forceinline__ void stream_pair(const SolveArgs& a) {
 
 // Fused SDDMM + SpMM for one pair (the GPU form of sinkhorn_wmd_fused.cpp:249-319,
 // doc-major so the x accumulation needs no atomics).
-__global__ __launch_bounds__(64) void k_fused_step(SolveArgs a) {
-  stream_pair<kFusedPart, false>(a);
+template <int PART, bool FINAL>
+__device__ __forceinline__ void stream_pair(const SolveArgs& a, uint32_t pair) {
+  const PairRef p = pair_ref(a, pair);
+  const int C4 = p.ks >> 2;
+  if (C4 <= 64) stream_pair_ks<PART, FINAL, 1>(a, p);
+  else if (C4 <= 128) stream_pair_ks<PART, FINAL, 2>(a, p);
+  else if (C4 <= 64 * kStreamMaxKS) stream_pair_ks<PART, FINAL, 4>(a, p);
+  else stream_pair_ks<PART, FINAL, 0>(a, p);
 }
-__global__ __launch_bounds__(64) void k_fused_final(SolveArgs a) {
-  stream_pair<kFusedPart, true>(a);
+
+// A streaming launch's blocks walk its pairs (every pair, or the status
+// scan's list of the pairs still iterating) with a grid of at most
+// kStreamGrid blocks, so late iterations with few active pairs launch few
+// blocks' worth of work.
+// Five waves per SIMD (20 per CU) stay resident: the register budget the
+// launch bounds give the compiler, and the grid's size.
+constexpr int kStreamWaves = 5;
+constexpr int kStreamGrid = 256 * 4 * kStreamWaves;
+
+template <typename F>
+__device__ __forceinline__ void for_stream_pairs(const SolveArgs& a, F&& f) {
+  const uint32_t total = a.act ? *a.n_act : (uint32_t)a.n_pairs;
+  for (uint32_t idx = blockIdx.x; idx < total; idx += gridDim.x) {
+    f(a.act ? (uint32_t)a.act[idx] : idx);
+    __syncthreads();  // LDS reuse by the next pair
+  }
+}
+
+// One pair per block (every pair: fixed iterations, the first iteration and
+// the final pass), or the list walk of a tolerance run's later iterations.
+__global__ __launch_bounds__(64, kStreamWaves) void k_fused_step(SolveArgs a) {
+  stream_pair<kFusedPart, false>(a, blockIdx.x);
+}
+__global__ __launch_bounds__(64, kStreamWaves) void k_fused_step_list(SolveArgs a) {
+  for_stream_pairs(a, [&](uint32_t pr) { stream_pair<kFusedPart, false>(a, pr); });
+}
+__global__ __launch_bounds__(64, kStreamWaves) void k_fused_final(SolveArgs a) {
+  stream_pair<kFusedPart, true>(a, blockIdx.x);
 }
 
 // Log-domain form of k_fused_step (large lambda, where the kernel underflows
@@ -896,9 +924,10 @@ __global__ __launch_bounds__(64) void k_fused_final(SolveArgs a) {
 //   res = sum_i |2^(f_i + lx_i) - r_i|,  f_i = log2 r_i - lx_i
 // with two-pass (maximum, then sum of exp2) reductions; the final pass forms
 // sum P_ie M_ie in FP64 with M = mrow + L ln2 / lamda.
-__global__ __launch_bounds__(64) void k_fused_step_log(SolveArgs a, int final_pass) {
+__device__ __forceinline__ void stream_pair_log(const SolveArgs& a, int final_pass,
+                                                uint32_t pair) {
   extern __shared__ float sh[];
-  const PairRef p = pair_ref(a, blockIdx.x);
+  const PairRef p = pair_ref(a, pair);
   if (!final_pass && a.done[p.out]) return;  // block-uniform
   float* f_sh = sh;
   float* g_sh = f_sh + a.lds_q;
@@ -956,12 +985,28 @@ __global__ __launch_bounds__(64) void k_fused_step_log(SolveArgs a, int final_pa
   }
 }
 
+__global__ __launch_bounds__(64) void k_fused_step_log(SolveArgs a, int final_pass) {
+  for_stream_pairs(a, [&](uint32_t pr) { stream_pair_log(a, final_pass, pr); });
+}
+
 // Nonfused SDDMM (sinkhorn_wmd_nonfused.cpp:72-90): w_e = c_e / (K[t_e,:] . u).
-__global__ __launch_bounds__(64) void k_sddmm(SolveArgs a) { stream_pair<kSddmmPart, false>(a); }
+__global__ __launch_bounds__(64, kStreamWaves) void k_sddmm(SolveArgs a) {
+  stream_pair<kSddmmPart, false>(a, blockIdx.x);
+}
+__global__ __launch_bounds__(64, kStreamWaves) void k_sddmm_list(SolveArgs a) {
+  for_stream_pairs(a, [&](uint32_t pr) { stream_pair<kSddmmPart, false>(a, pr); });
+}
 
 // Nonfused SpMM (sinkhorn_wmd_nonfused.cpp:237-261, final :279-306).
-__global__ __launch_bounds__(64) void k_spmm(SolveArgs a) { stream_pair<kSpmmPart, false>(a); }
-__global__ __launch_bounds__(64) void k_spmm_final(SolveArgs a) { stream_pair<kSpmmPart, true>(a); }
+__global__ __launch_bounds__(64, kStreamWaves) void k_spmm(SolveArgs a) {
+  stream_pair<kSpmmPart, false>(a, blockIdx.x);
+}
+__global__ __launch_bounds__(64, kStreamWaves) void k_spmm_list(SolveArgs a) {
+  for_stream_pairs(a, [&](uint32_t pr) { stream_pair<kSpmmPart, false>(a, pr); });
+}
+__global__ __launch_bounds__(64, kStreamWaves) void k_spmm_final(SolveArgs a) {
+  stream_pair<kSpmmPart, true>(a, blockIdx.x);
+}
 
 // ---------------------------------------------------------------------------
 // Pairs of a resident run whose documents are longer than the register tiles
@@ -1260,28 +1305,50 @@ __global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
 
 // status[0] += number of the launch's pairs still iterating; status[1] = max
 // residual bits.
+// Status scan of a streaming launch's pairs: status[0] counts the pairs not
+// stopped, status[1] takes the largest residual (as bits: residuals are >= 0
+// or NaN).  With a.act set it also lists the pairs not stopped there, each
+// wave appending its lanes' pairs at one atomically reserved offset (the
+// list's order varies between runs; each pair's arithmetic does not depend
+// on it).
 __global__ __launch_bounds__(256) void k_status(SolveArgs a, uint32_t* status) {
   const uint8_t* done = a.done;
   const float* resid = a.resid;
   const int64_t n = a.n_pairs;
+  const int lane = threadIdx.x & 63;
   __shared__ uint32_t s_cnt;
   __shared__ uint32_t s_max;
   if (threadIdx.x == 0) { s_cnt = 0; s_max = 0; }
   __syncthreads();
   uint32_t cnt = 0, mx = 0;
-  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t o = a.doc_list ? (int64_t)a.bucket_q[p / a.n_list] * a.N + a.doc_list[p % a.n_list]
-                                 : (int64_t)a.bucket_q[p / a.N] * a.N + p % a.N;
-    cnt += done[o] ? 0u : 1u;
-    const uint32_t bits = __float_as_uint(resid[o]);  // residuals are >= 0 (or NaN)
-    mx = bits > mx ? bits : mx;
+  // whole waves step together, so a wave's pairs are listed in one append
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = base + lane;
+    bool active = false;
+    if (p < n) {
+      const int64_t o = a.doc_list
+                            ? (int64_t)a.bucket_q[p / a.n_list] * a.N + a.doc_list[p % a.n_list]
+                            : (int64_t)a.bucket_q[p / a.N] * a.N + p % a.N;
+      active = done[o] == 0;
+      const uint32_t bits = __float_as_uint(resid[o]);
+      mx = bits > mx ? bits : mx;
+    }
+    if (a.act) {
+      const uint64_t m = __ballot(active);
+      uint32_t b = 0;
+      if (lane == 0 && m != 0ull) b = atomicAdd(&status[0], (uint32_t)__popcll(m));
+      b = __shfl(b, 0);
+      if (active) a.act[b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (int32_t)p;
+    } else {
+      cnt += active ? 1u : 0u;
+    }
   }
   atomicAdd(&s_cnt, cnt);
   atomicMax(&s_max, mx);
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(&status[0], s_cnt);
+    if (!a.act) atomicAdd(&status[0], s_cnt);
     atomicMax(&status[1], s_max);
   }
 }
@@ -1370,6 +1437,13 @@ size_t stream_lds_bytes(int lds_q, int lds_n) {
 
 static size_t stream_lds(const SolveArgs& a) { return stream_lds_bytes(a.lds_q, a.lds_n); }
 
+// A block per pair, or at most kStreamGrid blocks walking the list (the
+// log-domain kernel always walks).
+static unsigned stream_grid(const SolveArgs& a) {
+  const bool walk = a.act || a.log_domain;
+  return (unsigned)(!walk || a.n_pairs < kStreamGrid ? a.n_pairs : kStreamGrid);
+}
+
 size_t long_lds_bytes(int lds_q, int lds_n) {
   return sizeof(float) * ((size_t)lds_q + 3 * (size_t)lds_n + (size_t)kLongWaves * lds_q);
 }
@@ -1392,28 +1466,35 @@ hipError_t launch_stream_init(const SolveArgs& a, hipStream_t s) {
 
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
+  const dim3 grid(stream_grid(a));
   if (a.log_domain)
-    hipLaunchKernelGGL(k_fused_step_log, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s,
-                       a, final_pass ? 1 : 0);
+    hipLaunchKernelGGL(k_fused_step_log, grid, dim3(64), stream_lds(a), s, a, final_pass ? 1 : 0);
   else if (final_pass)
-    hipLaunchKernelGGL(k_fused_final, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
+    hipLaunchKernelGGL(k_fused_final, grid, dim3(64), stream_lds(a), s, a);
+  else if (a.act)
+    hipLaunchKernelGGL(k_fused_step_list, grid, dim3(64), stream_lds(a), s, a);
   else
-    hipLaunchKernelGGL(k_fused_step, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
+    hipLaunchKernelGGL(k_fused_step, grid, dim3(64), stream_lds(a), s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_sddmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
+  if (a.act)
+    hipLaunchKernelGGL(k_sddmm_list, dim3(stream_grid(a)), dim3(64), stream_lds(a), s, a);
+  else
+    hipLaunchKernelGGL(k_sddmm, dim3(stream_grid(a)), dim3(64), stream_lds(a), s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s) {
   if (a.n_pairs <= 0) return hipSuccess;
   if (final_pass)
-    hipLaunchKernelGGL(k_spmm_final, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
+    hipLaunchKernelGGL(k_spmm_final, dim3(stream_grid(a)), dim3(64), stream_lds(a), s, a);
+  else if (a.act)
+    hipLaunchKernelGGL(k_spmm_list, dim3(stream_grid(a)), dim3(64), stream_lds(a), s, a);
   else
-    hipLaunchKernelGGL(k_spmm, dim3((unsigned)a.n_pairs), dim3(64), stream_lds(a), s, a);
+    hipLaunchKernelGGL(k_spmm, dim3(stream_grid(a)), dim3(64), stream_lds(a), s, a);
   return hipGetLastError();
 }
 
