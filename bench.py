@@ -290,8 +290,9 @@ def plain_gpu(P, eng, qs, lamda, max_iter, tol, nq):
         assert np.all(res < tol), "plain GPU method: a pair did not reach the residual"
     return {"value": pairs / tp, "unit": "pairs/s", "queries": nq, "seconds": round(tp, 2),
             "method": "one query per call, fused SDDMM+SpMM launch per iteration from a "
-                      "captured graph, uniform start, plain steps, convergence checked every 4 "
-                      "iterations"}
+                      "captured graph, uniform start, plain steps; every 4 iterations a device "
+                      "scan lists the pairs still iterating (later iterations walk that list), "
+                      "the host checks every 32 iterations"}
 
 
 def trace_span(path: str, kernel: str = "k_resident", stage: str = "k_cdist") -> int:

@@ -125,8 +125,10 @@ typedef struct wmd_options {
   double tol;          /* 0: exactly max_iter iterations (reference behaviour);
                           > 0: each (query, doc) pair stops once its marginal
                           residual sum_i |u_i (K v)_i - r_i| < tol */
-  int32_t check_every; /* streaming variants with tol > 0: iterations per
-                          graph launch between convergence checks (0 -> 8) */
+  int32_t check_every; /* streaming variants with tol > 0: iterations between
+                          device-side scans that list the pairs still
+                          iterating (later iterations walk that list); the
+                          host reads the list's length every 8 scans (0 -> 8) */
   int32_t shape;       /* resident variant: -1 automatic, else a tile-shape id */
   int32_t summarize;   /* 1: fill wmd_stats iteration/residual summaries (one
                           device-to-host copy of per-pair counters); 0: stage

@@ -203,7 +203,7 @@ struct wmd_context {
   // its kernel arguments, length and form stay the same.
   hipGraphExec_t stream_exec = nullptr;
   wmd::SolveArgs stream_args{};
-  int stream_len = -1;
+  int stream_len = -1, stream_chunk = -1;
   bool stream_fused = false, stream_tol = false;
 
   // corpus
@@ -570,30 +570,53 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
     HIP_TRY(wmd::launch_stream_init(a, c->stream));
     ++*launches;
   }
+  // With a tolerance, every check_every iterations a status scan lists the
+  // pairs still iterating (the iterations walk that list); the host reads the
+  // list's length after every kScansPerCheck scans.  Lists and counts are
+  // double-buffered: the iterations of chunk k read list/count k % 2 and zero
+  // count (k + 1) % 2, which the chunk's scan then fills.
+  constexpr int kScansPerCheck = 8;
   const bool use_tol = o.tol > 0.0;
   const int chunk = use_tol ? std::max(1, o.check_every > 0 ? o.check_every : 8) : o.max_iter;
   const wmd::SolveArgs all_pairs = a;  // the final pass covers every pair
+  int32_t* list[2] = {nullptr, nullptr};
+  uint32_t* count[2] = {c->status.p, c->status.p + 1};
   if (use_tol && done_iters < o.max_iter) {
-    // Iterations with a tolerance walk the list of pairs still iterating,
-    // rebuilt by each chunk's status scan (the first scan runs here).
-    HIP_TRY(c->act.ensure((size_t)a.n_pairs));
-    a.act = c->act.p;
-    a.n_act = c->status.p;
-    HIP_TRY(wmd::launch_status(a, c->status.p, c->stream));
+    HIP_TRY(c->act.ensure(2 * (size_t)a.n_pairs));
+    list[0] = c->act.p;
+    list[1] = c->act.p + a.n_pairs;
+    HIP_TRY(wmd::launch_zero_u32(count[0], 1, c->stream));
+    HIP_TRY(wmd::launch_scan(a, nullptr, nullptr, list[0], count[0], c->stream));
     *launches += 2;
   }
+  const int span = use_tol ? chunk * kScansPerCheck : chunk;  // iterations per graph
   int rc = WMD_OK;
   while (done_iters < o.max_iter) {
-    const int len = std::min(chunk, o.max_iter - done_iters);
+    const int len = std::min(span, o.max_iter - done_iters);
+    const int n_chunks = use_tol ? (len + chunk - 1) / chunk : 1;
     if (!c->stream_exec || c->stream_len != len || c->stream_fused != fused ||
-        c->stream_tol != use_tol || std::memcmp(&c->stream_args, &a, sizeof(a)) != 0) {
+        c->stream_tol != use_tol || c->stream_chunk != chunk ||
+        std::memcmp(&c->stream_args, &a, sizeof(a)) != 0) {
       if (c->stream_exec) (void)hipGraphExecDestroy(c->stream_exec);
       c->stream_exec = nullptr;
       hipGraph_t graph = nullptr;
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       hipError_t cap = hipSuccess;
-      for (int k = 0; k < len && cap == hipSuccess; ++k) cap = step();
-      if (cap == hipSuccess && use_tol) cap = wmd::launch_status(a, c->status.p, c->stream);
+      const wmd::SolveArgs a0 = a;
+      for (int k = 0, it = 0; k < n_chunks && cap == hipSuccess; ++k) {
+        const int steps = use_tol ? std::min(chunk, len - it) : len;
+        if (use_tol) {
+          a.act = list[k & 1];
+          a.n_act = count[k & 1];
+          a.n_clear = count[(k + 1) & 1];
+        }
+        for (int t = 0; t < steps && cap == hipSuccess; ++t) cap = step();
+        if (cap == hipSuccess && use_tol)
+          cap = wmd::launch_scan(a, list[k & 1], count[k & 1], list[(k + 1) & 1],
+                                 count[(k + 1) & 1], c->stream);
+        it += steps;
+      }
+      a = a0;
       hipError_t end = hipStreamEndCapture(c->stream, &graph);
       if (cap != hipSuccess || end != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -611,6 +634,7 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
       c->stream_len = len;
       c->stream_fused = fused;
       c->stream_tol = use_tol;
+      c->stream_chunk = chunk;
     }
     hipGraphExec_t exec = c->stream_exec;
     hipError_t le = hipGraphLaunch(exec, c->stream);
@@ -619,10 +643,12 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
       rc = WMD_ERR_HIP;
       break;
     }
-    *launches += len * per_step + (use_tol ? 2 : 0);
+    *launches += len * per_step + (use_tol ? n_chunks : 0);
     done_iters += len;
     if (use_tol) {
-      hipError_t ce = hipMemcpyAsync(c->h_status, c->status.p, 2 * sizeof(uint32_t),
+      // A graph of an even number of chunks leaves the list in slot 0.
+      uint32_t* last = count[n_chunks & 1];
+      hipError_t ce = hipMemcpyAsync(c->h_status, last, sizeof(uint32_t),
                                      hipMemcpyDeviceToHost, c->stream);
       if (ce == hipSuccess) ce = hipStreamSynchronize(c->stream);
       if (ce != hipSuccess) {
@@ -631,6 +657,7 @@ int run_streaming(wmd_context* c, wmd::SolveArgs a, bool fused, const wmd_option
         break;
       }
       if (c->h_status[0] == 0) break;  // every pair has converged
+      if (n_chunks & 1) break;  // an odd (last, partial) graph: max_iter reached
     }
   }
   if (rc != WMD_OK) return rc;

@@ -131,6 +131,7 @@ struct SolveArgs {
   int32_t* act;            // streaming: pairs not yet stopped, listed by the status scan
                            // (null: every pair)
   const uint32_t* n_act;   // streaming: entries of act (the status scan's count)
+  uint32_t* n_clear;       // streaming list walk: the count slot of the next scan, zeroed
   int first;               // streaming, linear: the launch starts from the uniform u = q
                            // (reads no u, stop flag or count; writes every stop flag)
   // resident variant, learned warm start and training targets
@@ -348,7 +349,10 @@ size_t stream_lds_bytes(int lds_q, int lds_n);
 hipError_t launch_fused_step(const SolveArgs& a, bool final_pass, hipStream_t s);
 hipError_t launch_sddmm(const SolveArgs& a, hipStream_t s);
 hipError_t launch_spmm(const SolveArgs& a, bool final_pass, hipStream_t s);
-hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s);
+// Streaming tolerance runs: list the pairs of list_in (null: every pair) whose
+// stop flag is clear into list_out, counting them in *n_out (zeroed before).
+hipError_t launch_scan(const SolveArgs& a, const int32_t* list_in, const uint32_t* n_in,
+                       int32_t* list_out, uint32_t* n_out, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, int n, hipStream_t s);
 hipError_t launch_embed(const double* src64, float* dst, uint64_t n, unsigned long long* first_bad,
                         hipStream_t s);

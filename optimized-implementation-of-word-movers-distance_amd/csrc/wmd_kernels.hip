@@ -896,6 +896,7 @@ constexpr int kStreamGrid = 256 * 4 * kStreamWaves;
 
 template <typename F>
 __device__ __forceinline__ void for_stream_pairs(const SolveArgs& a, F&& f) {
+  if (a.n_clear && blockIdx.x == 0 && threadIdx.x == 0) *a.n_clear = 0;  // the next scan's count
   const uint32_t total = a.act ? *a.n_act : (uint32_t)a.n_pairs;
   for (uint32_t idx = blockIdx.x; idx < total; idx += gridDim.x) {
     f(a.act ? (uint32_t)a.act[idx] : idx);
@@ -1305,56 +1306,36 @@ __global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
 
 // status[0] += number of the launch's pairs still iterating; status[1] = max
 // residual bits.
-// Status scan of a streaming launch's pairs: status[0] counts the pairs not
-// stopped, status[1] takes the largest residual (as bits: residuals are >= 0
-// or NaN).  With a.act set it also lists the pairs not stopped there, each
-// wave appending its lanes' pairs at one atomically reserved offset (the
-// list's order varies between runs; each pair's arithmetic does not depend
-// on it).
-__global__ __launch_bounds__(256) void k_status(SolveArgs a, uint32_t* status) {
-  const uint8_t* done = a.done;
-  const float* resid = a.resid;
-  const int64_t n = a.n_pairs;
+// Status scan of a streaming run's pairs still iterating: walks the list
+// the last iterations used (list_in, *n_in entries; null: every pair) and
+// appends the pairs whose stop flag is still clear to list_out, counting them
+// in *n_out (zeroed by the iterations before it).  Each wave appends its
+// lanes' pairs at one atomically reserved offset, so the list's order varies
+// between runs; each pair's arithmetic does not depend on it.
+__global__ __launch_bounds__(256) void k_scan(SolveArgs a, const int32_t* list_in,
+                                              const uint32_t* n_in, int32_t* list_out,
+                                              uint32_t* n_out) {
+  const int64_t total = list_in ? (int64_t)*n_in : a.n_pairs;
   const int lane = threadIdx.x & 63;
-  __shared__ uint32_t s_cnt;
-  __shared__ uint32_t s_max;
-  if (threadIdx.x == 0) { s_cnt = 0; s_max = 0; }
-  __syncthreads();
-  uint32_t cnt = 0, mx = 0;
   // whole waves step together, so a wave's pairs are listed in one append
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < n;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total;
        base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t p = base + lane;
+    const int64_t idx = base + lane;
     bool active = false;
-    if (p < n) {
+    int64_t p = 0;
+    if (idx < total) {
+      p = list_in ? (int64_t)list_in[idx] : idx;
       const int64_t o = a.doc_list
                             ? (int64_t)a.bucket_q[p / a.n_list] * a.N + a.doc_list[p % a.n_list]
                             : (int64_t)a.bucket_q[p / a.N] * a.N + p % a.N;
-      active = done[o] == 0;
-      const uint32_t bits = __float_as_uint(resid[o]);
-      mx = bits > mx ? bits : mx;
+      active = a.done[o] == 0;
     }
-    if (a.act) {
-      const uint64_t m = __ballot(active);
-      uint32_t b = 0;
-      if (lane == 0 && m != 0ull) b = atomicAdd(&status[0], (uint32_t)__popcll(m));
-      b = __shfl(b, 0);
-      if (active) a.act[b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (int32_t)p;
-    } else {
-      cnt += active ? 1u : 0u;
-    }
+    const uint64_t m = __ballot(active);
+    uint32_t b = 0;
+    if (lane == 0 && m != 0ull) b = atomicAdd(n_out, (uint32_t)__popcll(m));
+    b = __shfl(b, 0);
+    if (active) list_out[b + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (int32_t)p;
   }
-  atomicAdd(&s_cnt, cnt);
-  atomicMax(&s_max, mx);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (!a.act) atomicAdd(&status[0], s_cnt);
-    atomicMax(&status[1], s_max);
-  }
-}
-
-__global__ void k_zero_status(uint32_t* status) {
-  if (threadIdx.x < 2) status[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -1535,13 +1516,12 @@ hipError_t launch_embed(const double* src64, float* dst, uint64_t n, unsigned lo
   return hipGetLastError();
 }
 
-hipError_t launch_status(const SolveArgs& a, uint32_t* status, hipStream_t s) {
-  hipLaunchKernelGGL(k_zero_status, dim3(1), dim3(64), 0, s, status);
-  const int64_t n = a.n_pairs;
-  int blocks = (int)((n + 255) / 256);
+hipError_t launch_scan(const SolveArgs& a, const int32_t* list_in, const uint32_t* n_in,
+                       int32_t* list_out, uint32_t* n_out, hipStream_t s) {
+  int blocks = (int)((a.n_pairs + 255) / 256);
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(k_status, dim3(blocks), dim3(256), 0, s, a, status);
+  hipLaunchKernelGGL(k_scan, dim3(blocks), dim3(256), 0, s, a, list_in, n_in, list_out, n_out);
   return hipGetLastError();
 }
 
