@@ -265,7 +265,7 @@ int wmd_context_topk(wmd_context* ctx, int32_t k, int32_t* doc_ids, double* wmd)
 
 /* ---- Learned warm start (no reference counterpart; it replaces the
  * reference's uniform start x = 1/q, sinkhorn_wmd_fused.cpp:188-191).  A
- * 15 -> 16 -> 16 -> 1 MLP with tanh-form GELU hidden layers predicts, for
+ * 15 -> 32 -> 32 -> 1 MLP with ReLU hidden layers predicts, for
  * every (query, document word), the converged log v_e (row-rescaled kernel
  * convention) from norm features of the word's embedding, a document summary
  * and a query summary (feature k):
@@ -277,11 +277,13 @@ int wmd_context_topk(wmd_context* ctx, int32_t k, int32_t* doc_ids, double* wmd)
  *           the query's sorted norms and their mid-ranks)
  *   14      |lamda| (sqrt(T^2 + n^2) - min_i M[word, i])
  * Parameters are one flat FP32 array: xm[15], xs[15] (inputs enter as
- * (x - xm) * xs), W1 [16][15] row-major, b1 [16], W2 [16][16], b2 [16],
- * w3 [16], b3 [1].  Targets are centred per (query, document) pair: the
+ * (x - xm) * xs), W1 [32][15] row-major, b1 [32], W2 [32][32], b2 [32],
+ * w3 [32], b3 [1].  Inference runs the two hidden layers on the matrix cores
+ * (FP16 operands, FP32 accumulation); training and wmd_mlp_loss_grad run in
+ * FP32.  Targets are centred per (query, document) pair: the
  * scaling pair (a u, v / a) gives the same plan, so only differences of
  * log v within a document matter. */
-#define WMD_MLP_HIDDEN 16
+#define WMD_MLP_HIDDEN 32
 #define WMD_MLP_FEATURES 15
 
 uint64_t wmd_mlp_param_count(void);
@@ -312,6 +314,13 @@ int wmd_context_mlp_train(wmd_context* ctx, const wmd_train_options* opt, double
  * the last run's row minima give them: out[n_queries][nnz][15], nonzeros in
  * the document-major order of wmd_context_duals. */
 int wmd_context_warm_features(wmd_context* ctx, float* out);
+
+/* The learned start's prediction for every (query, nonzero) of the current
+ * batch with the context's network, from the last run's row minima, as the
+ * inference kernel computes it (hidden layers on the matrix cores, FP16
+ * operands, FP32 accumulation): out[n_queries][nnz] predicted log v, in the
+ * layout of wmd_context_duals. */
+int wmd_context_warm_predictions(wmd_context* ctx, float* out);
 
 /* Caller-supplied starting log v (n_queries x nnz, the layout of
  * wmd_context_duals) for runs with warm_start = WMD_WARM_GIVEN. */

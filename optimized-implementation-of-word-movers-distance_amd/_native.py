@@ -38,7 +38,7 @@ ACCEL_ANDERSON = 1
 SHARE_AUTO = 0
 SHARE_OFF = 1
 SHARE_ON = 2
-MLP_HIDDEN = 16
+MLP_HIDDEN = 32
 MLP_FEATURES = 15
 
 # Every symbol include/wmd.h declares.
@@ -51,6 +51,7 @@ EXPORTS = ("wmd_options_default", "wmd_sinkhorn", "wmd_sinkhorn_nonfused", "wmd_
            "wmd_mlp_param_count", "wmd_context_mlp_set", "wmd_context_mlp_get",
            "wmd_train_options_default", "wmd_context_mlp_train", "wmd_context_duals",
            "wmd_mlp_loss_grad", "wmd_context_set_warm", "wmd_context_warm_features",
+           "wmd_context_warm_predictions",
            "wmd_context_autotune",
            "wmd_tuning_dims", "wmd_context_get_tuning", "wmd_context_set_tuning",
            "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims", "wmd_read_vecs", "wmd_read_r",
@@ -143,6 +144,7 @@ def load():
     L.wmd_context_set_warm.argtypes = [P, P]
     L.wmd_mlp_loss_grad.argtypes = [C.c_int, U64, P, P, U64, P, P, P, P]
     L.wmd_context_warm_features.argtypes = [P, P]
+    L.wmd_context_warm_predictions.argtypes = [P, P]
     L.wmd_context_autotune.argtypes = [P, C.c_double, C.POINTER(Options), C.c_int32,
                                        C.POINTER(C.c_double)]
     L.wmd_tuning_dims.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -174,7 +176,7 @@ def load():
                  "wmd_shape_domains",
                  "wmd_context_mlp_set", "wmd_context_mlp_get", "wmd_context_mlp_train",
                  "wmd_context_duals", "wmd_mlp_loss_grad", "wmd_context_set_warm",
-                 "wmd_context_warm_features",
+                 "wmd_context_warm_features", "wmd_context_warm_predictions",
                  "wmd_context_autotune", "wmd_tuning_dims", "wmd_context_get_tuning",
                  "wmd_context_set_tuning", "wmd_mtx_header", "wmd_read_mtx", "wmd_vecs_dims",
                  "wmd_read_vecs", "wmd_read_r", "wmd_write_scores", "wmd_write_mtx",

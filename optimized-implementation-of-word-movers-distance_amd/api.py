@@ -227,6 +227,14 @@ class WMDEngine:
                   "wmd_context_warm_features")
         return out
 
+    def warm_predictions(self, nnz):
+        """[n_queries, nnz] the learned start's predicted log v for the
+        current batch (the inference kernel's output)."""
+        out = np.empty((self.n_queries, int(nnz)), dtype=np.float32)
+        nat.check(self._L.wmd_context_warm_predictions(self._h, out.ctypes.data),
+                  "wmd_context_warm_predictions")
+        return out
+
     def set_warm(self, log_v):
         log_v = np.ascontiguousarray(log_v, dtype=np.float32)
         nat.check(self._L.wmd_context_set_warm(self._h, log_v.ctypes.data), "wmd_context_set_warm")

@@ -264,7 +264,7 @@ struct wmd_context {
   bool feat_corpus_ready = false;       // wnorm / dstat / dcdf for the current corpus
   DevBuf<float> wnorm, dstat, dcdf;     // corpus features
   DevBuf<float> qstat, qc, qn;          // query features of the current batch
-  DevBuf<float> Pd, Pq, pred;           // folded layer-1 parts, predictions
+  DevBuf<float> pred;                   // predictions
   double last_lamda = 0.0;              // lamda of the last run (its mrow)
   DevBuf<float> duals;
   bool duals_valid = false;
@@ -817,22 +817,15 @@ wmd::WarmInferArgs warm_args(wmd_context* c, double lamda) {
 }
 
 // Inference stage (after the distance stage, which leaves mrow): query
-// features, layer 1's folded document and query parts, then every
-// (query, nonzero) sample into c->pred.
+// features, then every (query, nonzero) sample into c->pred.
 int mlp_predict(wmd_context* c, double lamda, int* launches) {
   int rc = warm_query_features(c, launches);
   if (rc != WMD_OK) return rc;
-  HIP_TRY(c->Pd.ensure((size_t)c->N * wmd::kMlpHidden));
-  HIP_TRY(c->Pq.ensure((size_t)c->nq * wmd::kMlpHidden));
   HIP_TRY(c->pred.ensure((size_t)c->nq * (size_t)c->nnz));
-  HIP_TRY(wmd::launch_warm_fold(c->mlp_p.p, c->dstat.p, (int)c->N, c->qstat.p, c->nq, c->Pd.p,
-                                c->Pq.p, c->stream));
   wmd::WarmInferArgs a = warm_args(c, lamda);
-  a.Pd = c->Pd.p;
-  a.Pq = c->Pq.p;
   a.pred = c->pred.p;
   HIP_TRY(wmd::launch_warm_infer(a, c->stream));
-  *launches += 2;
+  *launches += 1;
   return WMD_OK;
 }
 
@@ -958,7 +951,7 @@ void wmd_context_destroy(wmd_context* c) {
   c->Es.release(); c->nt.release(); c->Qs.release(); c->nqw.release();
   c->doc_of.release(); c->mlp_p.release(); c->mlp_m.release(); c->mlp_v.release();
   c->wnorm.release(); c->dstat.release(); c->dcdf.release(); c->qstat.release();
-  c->qc.release(); c->qn.release(); c->Pd.release(); c->Pq.release(); c->pred.release();
+  c->qc.release(); c->qn.release(); c->pred.release();
   c->duals.release();
   c->fallbacks.release(); c->warm_vals.release();
   c->status.release();
@@ -2102,6 +2095,36 @@ int wmd_context_warm_features(wmd_context* c, float* out) {
   feats.release();
   if (e != hipSuccess) {
     set_err("warm-start features failed: %s", hipGetErrorString(e));
+    return WMD_ERR_HIP;
+  }
+  return WMD_OK;
+}
+
+int wmd_context_warm_predictions(wmd_context* c, float* out) {
+  int rc = check_ctx(c);
+  if (rc != WMD_OK) return rc;
+  if (!c->ran) {
+    set_err("run must complete before the warm-start predictions (they use its row minima)");
+    return WMD_ERR_STATE;
+  }
+  if (!c->mlp_ready) {
+    set_err("set MLP parameters first (wmd_context_mlp_set)");
+    return WMD_ERR_STATE;
+  }
+  if (!out) {
+    set_err("null output");
+    return WMD_ERR_ARG;
+  }
+  rc = warm_corpus_features(c);
+  if (rc != WMD_OK) return rc;
+  int launches = 0;
+  rc = mlp_predict(c, c->last_lamda, &launches);
+  if (rc != WMD_OK) return rc;
+  hipError_t e = hipMemcpyAsync(out, c->pred.p, sizeof(float) * (size_t)c->nq * (size_t)c->nnz,
+                                hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    set_err("warm-start predictions failed: %s", hipGetErrorString(e));
     return WMD_ERR_HIP;
   }
   return WMD_OK;

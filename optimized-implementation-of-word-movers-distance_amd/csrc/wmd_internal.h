@@ -215,7 +215,7 @@ constexpr int kMaxQ = kShapes[kNumShapes - 1].B * kShapes[kNumShapes - 1].NI;
   X(31) X(32) X(33) X(34) X(35) X(36) X(37) X(38) X(39)
 
 // ---- learned warm start (wmd_mlp.hip) ----
-// A 15 -> 16 -> 16 -> 1 network (GELU hidden layers) predicts the converged
+// A 15 -> 32 -> 32 -> 1 network (ReLU hidden layers) predicts the converged
 // log v_e of every (query, document word) from norm features of the word,
 // the document and the query (feature index k):
 //   0 |e_w|, 1 |e_w|^2 (the word's embedding norm n)
@@ -229,7 +229,7 @@ constexpr int kMaxQ = kShapes[kNumShapes - 1].B * kShapes[kNumShapes - 1].NI;
 // from the training data, not trained), W1[H][F], b1[H], W2[H][H], b2[H],
 // w3[H], b3.
 constexpr int kWarmFeatures = 15;
-constexpr int kMlpHidden = 16;
+constexpr int kMlpHidden = 32;
 constexpr int kWarmXm = 0;
 constexpr int kWarmXs = kWarmXm + kWarmFeatures;
 constexpr int kWarmW1 = kWarmXs + kWarmFeatures;
@@ -267,8 +267,6 @@ struct WarmInferArgs {
   int Vt;
   float lam_abs;           // |lamda|
   const float* params;     // kWarmParams
-  const float* Pd;         // N x H folded document part of layer 1
-  const float* Pq;         // nq x H folded query part of layer 1 (with b1)
   float* pred;             // nq x nnz out
   float* feats;            // export mode: nq x nnz x F raw features (pred unused)
 };
@@ -299,8 +297,6 @@ hipError_t launch_doc_feats(const int32_t* doc_ptr, const int32_t* doc_t, const 
 hipError_t launch_query_feats(const int32_t* q_ptr, const int32_t* q_words, const float* q_r,
                               const float* E, int D, int nq, float* qstat, float* qc, float* qn,
                               hipStream_t s);
-hipError_t launch_warm_fold(const float* params, const float* dstat, int N, const float* qstat,
-                            int nq, float* Pd, float* Pq, hipStream_t s);
 hipError_t launch_warm_infer(const WarmInferArgs& a, hipStream_t s);
 hipError_t launch_warm_moments(const WarmInferArgs& a, double* part, int blocks, hipStream_t s);
 hipError_t launch_warm_train(const WarmTrainArgs& t, hipStream_t s);

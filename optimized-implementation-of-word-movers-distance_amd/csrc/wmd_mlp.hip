@@ -1,10 +1,9 @@
-// Learned warm start: a 15 -> 16 -> 16 -> 1 network (tanh-form GELU hidden
-// layers) predicts, for every (query, document word) nonzero, the converged
-// log of the column scaling v_e (in the solver's row-rescaled kernel
-// convention) from norm features of the word, its document and the query
-// (feature list in wmd_internal.h).  Because (u, v) -> (a u, v / a) leaves
-// the plan unchanged, targets and loss are centred per (query, document)
-// pair.
+// Learned warm start: a 15 -> 32 -> 32 -> 1 network (ReLU hidden layers)
+// predicts, for every (query, document word) nonzero, the converged log of
+// the column scaling v_e (in the solver's row-rescaled kernel convention)
+// from norm features of the word, its document and the query (feature list
+// in wmd_internal.h).  Because (u, v) -> (a u, v / a) leaves the plan
+// unchanged, targets and loss are centred per (query, document) pair.
 //
 // Why norm features: the embeddings' norms set most of every distance
 // (|a - b|^2 = |a|^2 + |b|^2 - 2 a.b with a small a.b), and that part of the
@@ -15,10 +14,11 @@
 // matched to; the network learns the dual from it.
 //
 // Kernels: the corpus features (once per corpus), the query features (per
-// run), the folding of the document and query parts of layer 1, inference
-// (one thread per sample), and training: one fused kernel per Adam step
+// run), inference (the hidden layers on the matrix cores, FP16 operands and
+// FP32 accumulation) and training in FP32: one fused kernel per Adam step
 // (forward, centred loss and backward for one pair per block, gradient
 // partials through LDS) plus the Adam update, which sums the partials.
+
 #include "wmd_internal.h"
 #include "wmd_lanes.h"
 
@@ -38,23 +38,7 @@ __device__ __forceinline__ const cfloat* as_const(const float* p) {
 // Feature slots taken per sample (the rest are per document and per query).
 constexpr int kSampleFeat[5] = {0, 1, 12, 13, 14};
 
-// GELU, tanh form, 0.5 x (1 + tanh(z)) with z = sqrt(2/pi) (x + 0.044715 x^3),
-// evaluated as x * sigmoid(2z) = x / (1 + 2^(x (c0 + c1 x^2))) with
-// c0 = -2 sqrt(2/pi) log2(e), c1 = 0.044715 c0.
-constexpr float kGeluC = 0.7978845608028654f;
-constexpr float kGeluA = 0.044715f;
-constexpr float kGeluE0 = -2.f * kGeluC * 1.4426950408889634f;
-constexpr float kGeluE1 = kGeluE0 * kGeluA;
-
-__device__ __forceinline__ float sig2z(float x) {
-  const float p = fmaf(kGeluE1, x * x, kGeluE0);
-  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * p));
-}
-__device__ __forceinline__ float gelu(float x) { return x * sig2z(x); }
-// d/dx x s(x), s = sigmoid(2z): s + x s (1 - s) 2 sqrt(2/pi) (1 + 3 a x^2)
-__device__ __forceinline__ float gelu_grad(float x, float s) {
-  return fmaf(x * s * (1.f - s), 2.f * kGeluC * fmaf(3.f * kGeluA, x * x, 1.f), s);
-}
+__device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
 
 // numpy.interp of x over ascending knots xc[0..q) with values yv: clamped
 // outside the knots, linear between the bracketing pair.
@@ -212,36 +196,40 @@ __global__ __launch_bounds__(256) void k_query_feats(const int32_t* q_ptr, const
   }
 }
 
-// Layer 1's document and query parts: Pd[j][h] = sum_{k=2..6} W1[h][k] z_k,
-// Pq[Q][h] = b1[h] + sum_{k=7..11} W1[h][k] z_k.
-__global__ void k_warm_fold(const float* p, const float* dstat, int N, const float* qstat, int nq,
-                            float* Pd, float* Pq) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t rows = (int64_t)N + nq;
-  if (i >= rows * H) return;
-  const int64_t row = i / H;
-  const int h = (int)(i % H);
-  const bool doc = row < N;
-  const int k0 = doc ? 2 : 7;
-  const float* st = doc ? dstat + row * 5 : qstat + (row - N) * 5;
-  float acc = doc ? 0.f : p[kWarmB1 + h];
-#pragma unroll
-  for (int k = 0; k < 5; ++k)
-    acc = fmaf(p[kWarmW1 + h * F + k0 + k], (st[k] - p[kWarmXm + k0 + k]) * p[kWarmXs + k0 + k], acc);
-  if (doc) Pd[row * H + h] = acc;
-  else Pq[(row - N) * H + h] = acc;
-}
-
 // ---------------------------------------------------------------------------
-// Inference.  Blocks of 256 threads over kInferChunks x 256 consecutive
-// nonzeros of one query (blockIdx.y); the query's sorted norm table sits in
-// LDS for the quantile lookups.  With export set the kernel writes the 15
-// raw features instead of the prediction.
+// Inference.  Blocks of 256 threads (four waves) over kInferChunks x 256
+// consecutive nonzeros of one query (blockIdx.y); the query's sorted norm
+// table sits in LDS for the quantile lookups.  Each wave takes 64 samples at
+// a time: lane l computes sample l's 15 standardised features, which pass
+// through LDS (FP16, 16 per sample, the last 0) into the B-operand layout of
+// v_mfma_f32_16x16x16_f16 (lane l holds rows 4 (l / 16) .. +3 of column
+// l % 16).  The hidden layers are computed transposed, H^T = W X^T (units
+// along M, samples along N), so a layer's accumulator tile, which has the
+// same lane layout, is the next layer's B operand after ReLU and conversion:
+//   layer 1: 2 unit tiles x 4 sample tiles, K = 16 (the features), b1 as C
+//   layer 2: 2 x 4 tiles, K = 32 (two steps), b2 as C
+//   output:  each lane's w3-weighted sum over its 4 units, summed over the
+//            four 16-lane groups (permlane swaps), + b3
+// With export set the kernel writes the 15 raw features instead.
 constexpr int kInferChunks = 8;
+constexpr int kStageRow = 20;  // FP16 per staged sample (16 + 4 padding: 8-byte aligned rows)
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ half4_t relu_h4(float4_t d) {
+  half4_t h;
+  h[0] = (_Float16)relu(d[0]);
+  h[1] = (_Float16)relu(d[1]);
+  h[2] = (_Float16)relu(d[2]);
+  h[3] = (_Float16)relu(d[3]);
+  return h;
+}
 
 template <bool EXPORT>
 __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
+  static_assert(H == 32 && F <= 16, "the matrix-core layout takes 32 units and 16 inputs");
   __shared__ float sqc[kWarmQueryMax], sqn[kWarmQueryMax];
+  __shared__ __attribute__((aligned(16))) _Float16 stage[4][64 * kStageRow];
   const int Q = blockIdx.y;
   const int qb = a.q.q_ptr[Q], q = a.q.q_ptr[Q + 1] - qb;
   if (q > kWarmQueryMax) return;  // refused on the host; never index past the tables
@@ -252,51 +240,82 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
   __syncthreads();
   const cfloat* p = as_const(a.params);
   const int64_t base = (int64_t)blockIdx.x * (256 * kInferChunks);
-  for (int ch = 0; ch < kInferChunks; ++ch) {
-    const int64_t pos = base + ch * 256 + threadIdx.x;
-    if (pos >= a.c.nnz) break;
-    if constexpr (EXPORT) {
+  if constexpr (EXPORT) {
+    for (int ch = 0; ch < kInferChunks; ++ch) {
+      const int64_t pos = base + ch * 256 + threadIdx.x;
+      if (pos >= a.c.nnz) break;
       float x[F];
       all_feats(a, sqc, sqn, q, Q, pos, x);
       float* o = a.feats + ((int64_t)Q * a.c.nnz + pos) * F;
 #pragma unroll
       for (int k = 0; k < F; ++k) o[k] = x[k];
-    } else {
-      const int t = a.c.doc_t[pos];
-      const int j = a.c.doc_of[pos];
-      float x[5];
-      sample_feats(a, sqc, sqn, q, Q, pos, t, x);
-      float z[5];
+    }
+    return;
+  } else {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lg = lane >> 4;
+    // This lane's operand fragments: A = weight rows (unit lr of each tile,
+    // inputs 4 lg .. +3 of each K step), C = b1 / b2 and the w3 factors of
+    // units 4 lg .. +3 of each tile.
+    half4_t A1[2], A2[2][2];
+    float4_t C1[2], C2[2], W3[2];
 #pragma unroll
-      for (int k = 0; k < 5; ++k)
-        z[k] = (x[k] - p[kWarmXm + kSampleFeat[k]]) * p[kWarmXs + kSampleFeat[k]];
-      const float4* pd = reinterpret_cast<const float4*>(a.Pd + (int64_t)j * H);
-      const float* pq = a.Pq + (int64_t)Q * H;
-      float h1[H];
+    for (int mt = 0; mt < 2; ++mt) {
+      const int unit = mt * 16 + lr;
 #pragma unroll
-      for (int v = 0; v < H / 4; ++v) {
-        const float4 d = pd[v];
-        h1[4 * v] = d.x;
-        h1[4 * v + 1] = d.y;
-        h1[4 * v + 2] = d.z;
-        h1[4 * v + 3] = d.w;
+      for (int i = 0; i < 4; ++i) {
+        const int k = 4 * lg + i;
+        A1[mt][i] = (_Float16)(k < F ? p[kWarmW1 + unit * F + k] : 0.f);
+        A2[mt][0][i] = (_Float16)p[kWarmW2 + unit * H + k];
+        A2[mt][1][i] = (_Float16)p[kWarmW2 + unit * H + 16 + k];
+        C1[mt][i] = p[kWarmB1 + mt * 16 + 4 * lg + i];
+        C2[mt][i] = p[kWarmB2 + mt * 16 + 4 * lg + i];
+        W3[mt][i] = p[kWarmW3 + mt * 16 + 4 * lg + i];
+      }
+    }
+    const float b3 = p[kWarmB3];
+    _Float16* st = stage[wv];
+    // Every wave runs every chunk (the barriers stay block-uniform); lanes
+    // past the last nonzero compute on the last one and store nothing.
+    for (int ch = 0; ch < kInferChunks; ++ch) {
+      const int64_t pos = base + ch * 256 + threadIdx.x;
+      const bool valid = pos < a.c.nnz;
+      float x[F];
+      all_feats(a, sqc, sqn, q, Q, valid ? pos : a.c.nnz - 1, x);
+      half4_t zh[4];
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+        zh[k >> 2][k & 3] = (_Float16)(k < F ? (x[k] - p[kWarmXm + k]) * p[kWarmXs + k] : 0.f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) *reinterpret_cast<half4_t*>(st + lane * kStageRow + 4 * v) = zh[v];
+      __syncthreads();
+      half4_t Bx[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+        Bx[nt] = *reinterpret_cast<const half4_t*>(st + (nt * 16 + lr) * kStageRow + 4 * lg);
+      __syncthreads();  // the stage is rewritten by the next chunk
+      half4_t H1[2][4];
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+          H1[mt][nt] = relu_h4(__builtin_amdgcn_mfma_f32_16x16x16f16(A1[mt], Bx[nt], C1[mt], 0, 0, 0));
+      float yp[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+          float4_t d = __builtin_amdgcn_mfma_f32_16x16x16f16(A2[mt][0], H1[0][nt], C2[mt], 0, 0, 0);
+          d = __builtin_amdgcn_mfma_f32_16x16x16f16(A2[mt][1], H1[1][nt], d, 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) yp[nt] = fmaf(W3[mt][i], relu(d[i]), yp[nt]);
+        }
       }
 #pragma unroll
-      for (int h = 0; h < H; ++h) {
-        float acc = h1[h] + pq[h];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) acc = fmaf(p[kWarmW1 + h * F + kSampleFeat[k]], z[k], acc);
-        h1[h] = gelu(acc);
-      }
-      float y = p[kWarmB3];
-#pragma unroll
-      for (int o = 0; o < H; ++o) {
-        float acc = p[kWarmB2 + o];
-#pragma unroll
-        for (int h = 0; h < H; ++h) acc = fmaf(p[kWarmW2 + o * H + h], h1[h], acc);
-        y = fmaf(p[kWarmW3 + o], gelu(acc), y);
-      }
-      a.pred[(int64_t)Q * a.c.nnz + pos] = y;
+      for (int nt = 0; nt < 4; ++nt) yp[nt] = bfly_red<16, 32>(yp[nt]);
+      // lane l = 16 lg + lr holds sample l in sample tile lg
+      const float y = lg == 0 ? yp[0] : lg == 1 ? yp[1] : lg == 2 ? yp[2] : yp[3];
+      if (valid) a.pred[(int64_t)Q * a.c.nnz + pos] = y + b3;
     }
   }
 }
@@ -336,17 +355,19 @@ __global__ __launch_bounds__(256) void k_warm_moments(WarmInferArgs a, double* p
 }
 
 // ---------------------------------------------------------------------------
-// Training: one block of kTrainThreads per group (pair).
-constexpr int kTrainThreads = 128;
+// Training (FP32): one block of kTrainThreads per group (pair).
+constexpr int kTrainThreads = 64;
 // LDS row per sample of a chunk: z[F], h1[H], dz1[H], dz2[H], h2 dy[H], dy.
 constexpr int kRowZ = 0, kRowH1 = F, kRowDz1 = F + H, kRowDz2 = F + 2 * H, kRowH2 = F + 3 * H,
               kRowDy = F + 4 * H, kRow = F + 4 * H + 1;
 
 struct Fwd {
-  float z[F], a1[H], s1[H], h1[H], a2[H], s2[H], h2[H];
+  float z[F], h1[H], h2[H];
   float y;
 };
 
+// ReLU's derivative is taken from its output: h > 0 exactly when the
+// pre-activation is > 0 (0 at the kink).
 __device__ __forceinline__ void forward(const cfloat* p, const float (&x)[F], Fwd& f) {
 #pragma unroll
   for (int k = 0; k < F; ++k) f.z[k] = (x[k] - p[kWarmXm + k]) * p[kWarmXs + k];
@@ -355,9 +376,7 @@ __device__ __forceinline__ void forward(const cfloat* p, const float (&x)[F], Fw
     float acc = p[kWarmB1 + h];
 #pragma unroll
     for (int k = 0; k < F; ++k) acc = fmaf(p[kWarmW1 + h * F + k], f.z[k], acc);
-    f.a1[h] = acc;
-    f.s1[h] = sig2z(acc);
-    f.h1[h] = acc * f.s1[h];
+    f.h1[h] = relu(acc);
   }
   float y = p[kWarmB3];
 #pragma unroll
@@ -365,9 +384,7 @@ __device__ __forceinline__ void forward(const cfloat* p, const float (&x)[F], Fw
     float acc = p[kWarmB2 + o];
 #pragma unroll
     for (int h = 0; h < H; ++h) acc = fmaf(p[kWarmW2 + o * H + h], f.h1[h], acc);
-    f.a2[o] = acc;
-    f.s2[o] = sig2z(acc);
-    f.h2[o] = acc * f.s2[o];
+    f.h2[o] = relu(acc);
     y = fmaf(p[kWarmW3 + o], f.h2[o], y);
   }
   f.y = y;
@@ -386,18 +403,26 @@ __device__ __forceinline__ void train_sample(const WarmTrainArgs& t, int64_t s, 
   y = t.ysrc[t.yidx ? t.yidx[s] : s];
 }
 
-// Block reduction of a double over kTrainThreads threads (two waves).
+// Block reduction of a double over the kTrainThreads threads (waves in order).
 __device__ __forceinline__ double block_sum_d(double v, double* red) {
+  constexpr int kWaves = kTrainThreads / 64;
   v = wave_sum_d(v);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  return red[0] + red[1];
+  if constexpr (kWaves == 1) {
+    return v;
+  } else {
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) t += red[w];
+    return t;
+  }
 }
 
 __global__ __launch_bounds__(kTrainThreads) void k_warm_train(WarmTrainArgs t) {
   __shared__ float rows[kTrainThreads * kRow];
-  __shared__ double red[2];
+  __shared__ double red[kTrainThreads / 64];
   const int g = blockIdx.x;
   const int64_t s0 = t.gptr[g], s1 = t.gptr[g + 1];
   const cfloat* p = as_const(t.params);
@@ -432,7 +457,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_warm_train(WarmTrainArgs t) {
       float dz2[H];
 #pragma unroll
       for (int o = 0; o < H; ++o) {
-        dz2[o] = dy * p[kWarmW3 + o] * gelu_grad(f.a2[o], f.s2[o]);
+        dz2[o] = f.h2[o] > 0.f ? dy * p[kWarmW3 + o] : 0.f;
         r[kRowDz2 + o] = dz2[o];
         r[kRowH2 + o] = dy * f.h2[o];
       }
@@ -441,7 +466,7 @@ __global__ __launch_bounds__(kTrainThreads) void k_warm_train(WarmTrainArgs t) {
         float bk = 0.f;
 #pragma unroll
         for (int o = 0; o < H; ++o) bk = fmaf(p[kWarmW2 + o * H + h], dz2[o], bk);
-        r[kRowDz1 + h] = bk * gelu_grad(f.a1[h], f.s1[h]);
+        r[kRowDz1 + h] = f.h1[h] > 0.f ? bk : 0.f;
         r[kRowH1 + h] = f.h1[h];
       }
 #pragma unroll
@@ -550,15 +575,6 @@ hipError_t launch_query_feats(const int32_t* q_ptr, const int32_t* q_words, cons
   if (nq <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_query_feats, dim3(nq), dim3(256), 0, s, q_ptr, q_words, q_r, E, D, qstat,
                      qc, qn);
-  return hipGetLastError();
-}
-
-hipError_t launch_warm_fold(const float* params, const float* dstat, int N, const float* qstat,
-                            int nq, float* Pd, float* Pq, hipStream_t s) {
-  const int64_t n = ((int64_t)N + nq) * H;
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_warm_fold, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, params,
-                     dstat, N, qstat, nq, Pd, Pq);
   return hipGetLastError();
 }
 

@@ -13,11 +13,11 @@ Features of a (query, document word) sample (15 per sample):
      mid-rank = (sum of r below + r_i / 2) / sum r
   14 |lamda| (sqrt(T^2 + n^2) - mrow), mrow = min_i |e_w - e_i| over the
      query's words
-Model: z = (x - xm) * xs, h1 = gelu(W1 z + b1), h2 = gelu(W2 h1 + b2),
-y = w3 . h2 + b3, gelu(a) = 0.5 a (1 + tanh(sqrt(2/pi) (a + 0.044715 a^3))).
+Model: z = (x - xm) * xs, h1 = relu(W1 z + b1), h2 = relu(W2 h1 + b2),
+y = w3 . h2 + b3, relu(a) = max(a, 0) (derivative 1 for a > 0, else 0).
 Loss: for contiguous groups g (one (query, document) pair each),
 L = sum_g sum_{s in g} (d_s - mean_g d)^2 / n, d = yhat - y.
-Parameters: one flat array xm [15], xs [15], W1 [16, 15], b1, W2 [16, 16],
+Parameters: one flat array xm [15], xs [15], W1 [32, 15], b1, W2 [32, 32],
 b2, w3, b3 (xm and xs are not trained: zero gradient).
 
 These functions have no counterpart in the reference.  The loss gradient is
@@ -28,10 +28,8 @@ from __future__ import annotations
 
 import numpy as np
 
-H = 16
+H = 32
 F = 15
-C = np.sqrt(2.0 / np.pi)
-A = 0.044715
 
 
 def layout():
@@ -53,22 +51,21 @@ def unpack(p):
             p[L["w3"]:L["b3"]], p[L["b3"]])
 
 
-def gelu(z):
-    return 0.5 * z * (1.0 + np.tanh(C * (z + A * z ** 3)))
+def relu(z):
+    return np.maximum(z, 0.0)
 
 
-def gelu_grad(z):
-    t = np.tanh(C * (z + A * z ** 3))
-    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * C * (1.0 + 3.0 * A * z * z)
+def relu_grad(z):
+    return (z > 0.0).astype(np.float64)
 
 
 def forward(X, p):
     xm, xs, W1, b1, W2, b2, w3, b3 = unpack(p)
     Z = (np.asarray(X, dtype=np.float64) - xm) * xs
     Z1 = Z @ W1.T + b1
-    H1 = gelu(Z1)
+    H1 = relu(Z1)
     Z2 = H1 @ W2.T + b2
-    H2 = gelu(Z2)
+    H2 = relu(Z2)
     return H2 @ w3 + b3, (Z, Z1, H1, Z2, H2)
 
 
@@ -101,10 +98,10 @@ def loss_grad(X, y, group_ptr, p):
     g = np.zeros(L["total"])
     g[L["w3"]:L["b3"]] = H2.T @ dy
     g[L["b3"]] = dy.sum()
-    dZ2 = np.outer(dy, w3) * gelu_grad(Z2)
+    dZ2 = np.outer(dy, w3) * relu_grad(Z2)
     g[L["W2"]:L["b2"]] = (dZ2.T @ H1).reshape(-1)
     g[L["b2"]:L["w3"]] = dZ2.sum(0)
-    dZ1 = (dZ2 @ W2) * gelu_grad(Z1)
+    dZ1 = (dZ2 @ W2) * relu_grad(Z1)
     g[L["W1"]:L["b1"]] = (dZ1.T @ Z).reshape(-1)
     g[L["b1"]:L["W2"]] = dZ1.sum(0)
     return tot / n, g

@@ -1,6 +1,7 @@
-"""Learned warm start: the GPU feature kernels and the training kernel's loss
-and gradient against the FP64 oracle (oracle/mlp_oracle.py, pinned by a
-hand-worked pair, library routines and finite differences), the warm-start
+"""Learned warm start: the GPU feature kernels, the matrix-core inference
+kernel and the training kernel's loss and gradient against the FP64 oracle
+(oracle/mlp_oracle.py, pinned by a hand-worked pair, library routines,
+a hand-wired forward and finite differences), the warm-start
 plumbing (exact duals give an immediate stop), a small training run whose
 learned-start results are judged against converged FP64, and the gain
 itself: the learned start's inference plus solve takes less device time than
@@ -67,6 +68,51 @@ def test_warm_features_match_oracle(P, dg):
     say("  max |err| / feature scale: " + " ".join(f"{x:.1e}" for x in err))
     assert np.all(err[:14] <= 1e-5)
     assert err[14] <= 1e-4  # |lamda| (norm-matched distance - row minimum of the FP32 distances)
+
+
+def _fp16(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float16).astype(np.float64)
+
+
+def test_inference_matches_oracle_forward(P, dg):
+    """The matrix-core inference kernel against the FP64 forward of
+    oracle/mlp_oracle.py with the kernel's FP16 roundings applied at the same
+    points (standardised inputs, W1, layer-1 outputs after ReLU, W2; biases,
+    w3 and every accumulation in full precision): agreement to FP32
+    accumulation error pins the operand layouts and the wiring, where a
+    transposed tile or a wrong lane group gives O(1) errors.  Against the
+    exact FP64 forward the difference is the FP16 rounding itself."""
+    from oracle import mlp_oracle as mo
+    say("[case] warm-start inference (FP16 matrix cores) vs FP64 forward")
+    corpus, _, _ = dg.workload("batched", seed=5, N=300, n_queries=1)
+    qs = dg.make_queries(np.random.default_rng(6), corpus.V, [1, 2, 7, 40, 130, 300])
+    rng = np.random.default_rng(7)
+    p = P.mlp_init(seed=4)
+    L = P.mlp_layout()
+    with P.WMDEngine(0) as e:
+        e.set_corpus(corpus.vecs, corpus.word_major())
+        e.set_queries(qs.q_ptr, qs.words, qs.weights)
+        e.run(-10.0, max_iter=15)
+        X = e.warm_features(corpus.nnz).reshape(-1, mo.F)
+        p[L["xm"]:L["xs"]] = X.mean(0)
+        p[L["xs"]:L["W1"]] = 1.0 / X.std(0)
+        p[L["b1"]:L["W2"]] = rng.normal(0, 0.3, mo.H)
+        p[L["b2"]:L["w3"]] = rng.normal(0, 0.3, mo.H)
+        p[L["b3"]] = 0.25
+        e.mlp_set(p)
+        got = e.warm_predictions(corpus.nnz).reshape(-1).astype(np.float64)
+    xm, xs, W1, b1, W2, b2, w3, b3 = mo.unpack(p)
+    z = (X.astype(np.float32) - p[L["xm"]:L["xs"]]) * p[L["xs"]:L["W1"]]
+    h1 = _fp16(np.maximum(_fp16(z) @ _fp16(W1).T + b1, 0.0))
+    emul = np.maximum(h1 @ _fp16(W2).T + b2, 0.0) @ w3 + b3
+    exact, _ = mo.forward(X, p)
+    scale = max(1.0, float(np.abs(exact).max()))
+    err_e = float(np.abs(got - emul).max()) / scale
+    err_x = float(np.abs(got - exact).max()) / scale
+    say(f"  {got.size} samples, |y| up to {scale:.2f}: max err vs FP16-rounded forward {err_e:.2e}, "
+        f"vs exact FP64 forward {err_x:.2e}")
+    assert err_e <= 1e-3
+    assert err_x <= 2e-2  # FP16 operands: unit roundoff 4.9e-4 on 15- and 32-term sums
 
 
 def _train_eval_sets(dg):

@@ -44,11 +44,31 @@ def test_loss_is_centred_per_group():
     assert abs(g[mo.layout()["b3"]]) < 1e-12  # the output bias cannot matter
 
 
-def test_gelu_derivative():
-    z = np.linspace(-6, 6, 101)
+def test_relu_derivative():
+    z = np.concatenate([np.linspace(-6, -0.01, 50), np.linspace(0.01, 6, 50)])
     h = 1e-6
-    fd = (mo.gelu(z + h) - mo.gelu(z - h)) / (2 * h)
-    assert np.max(np.abs(fd - mo.gelu_grad(z))) < 1e-8
+    fd = (mo.relu(z + h) - mo.relu(z - h)) / (2 * h)
+    assert np.max(np.abs(fd - mo.relu_grad(z))) < 1e-8
+    assert mo.relu_grad(np.array([0.0]))[0] == 0.0  # the kink takes the zero branch
+
+
+def test_forward_hand_worked_wiring():
+    # one path through the network: hidden unit 0 reads feature 0, layer-2
+    # unit 5 reads hidden unit 0 (weight 2, bias -1), the output reads unit 5
+    # (weight 3) plus 0.5: y = 3 max(2 max(x0, 0) - 1, 0) + 0.5
+    L = mo.layout()
+    p = np.zeros(L["total"])
+    p[L["xs"]:L["W1"]] = 1.0
+    p[L["W1"] + 0 * mo.F + 0] = 1.0          # W1[0][0]
+    p[L["W2"] + 5 * mo.H + 0] = 2.0          # W2[5][0]
+    p[L["b2"] + 5] = -1.0
+    p[L["w3"] + 5] = 3.0
+    p[L["b3"]] = 0.5
+    X = np.zeros((4, mo.F))
+    X[:, 0] = [-1.0, 0.25, 2.0, 0.75]
+    X[:, 1] = 7.0  # no weight reads it
+    y, _ = mo.forward(X, p)
+    assert np.allclose(y, [0.5, 0.5, 9.5, 2.0], rtol=0, atol=1e-12)
 
 
 def test_features_hand_worked_pair():
