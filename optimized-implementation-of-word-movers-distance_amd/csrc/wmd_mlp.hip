@@ -53,14 +53,45 @@ __device__ __forceinline__ float interp_knots(const float* xc, const float* yv, 
   return yv[lo] + (x - xc[lo]) * (yv[hi] - yv[lo]) / (xc[hi] - xc[lo]);
 }
 
+// interp_knots with the bracketing search started from a bucket table:
+// bk[b] is the last knot index i with xc[i] <= b / kKnotBuckets (0 if none),
+// so the search walks forward from the bucket below x (one below: x * K may
+// round up) over the few knots in between.  It ends at the same knot pair as
+// the binary search (the last i with xc[i] <= x), so the value is the same.
+constexpr int kKnotBuckets = 256;
+__device__ __forceinline__ float interp_bucketed(const float* xc, const float* yv,
+                                                 const int16_t* bk, int q, float x) {
+  if (x <= xc[0]) return yv[0];
+  if (x >= xc[q - 1]) return yv[q - 1];
+  const int b = min(max((int)(x * (float)kKnotBuckets) - 1, 0), kKnotBuckets);
+  int lo = bk[b];
+  while (xc[lo + 1] <= x) ++lo;  // ends below q - 1: x < xc[q - 1]
+  const int hi = lo + 1;
+  return yv[lo] + (x - xc[lo]) * (yv[hi] - yv[lo]) / (xc[hi] - xc[lo]);
+}
+
+// Fill bk (kKnotBuckets + 1 entries) for knots xc[0 .. q); block-wide.
+__device__ __forceinline__ void build_knot_buckets(const float* xc, int q, int16_t* bk) {
+  for (int b = threadIdx.x; b <= kKnotBuckets; b += blockDim.x) {
+    const float x = (float)b / (float)kKnotBuckets;
+    int lo = 0, hi = q;  // last i with xc[i] <= x, or 0
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (xc[mid] <= x) lo = mid; else hi = mid;
+    }
+    bk[b] = (int16_t)lo;
+  }
+}
+
 // The five per-sample features (slots 0, 1, 12, 13, 14) of nonzero pos
-// (touched word t) under query Q, whose sorted norm table is qc / qn.
+// (touched word t) under query Q, whose sorted norm table is qc / qn (with
+// its bucket table bk, or null for the plain binary search).
 __device__ __forceinline__ void sample_feats(const WarmInferArgs& a, const float* qc,
                                              const float* qn, int q, int Q, int64_t pos, int t,
-                                             float (&x)[5]) {
+                                             float (&x)[5], const int16_t* bk = nullptr) {
   const float n = a.c.wnorm[t];
   const float cdf = a.c.dcdf[pos];
-  const float T = interp_knots(qc, qn, q, cdf);
+  const float T = bk ? interp_bucketed(qc, qn, bk, q, cdf) : interp_knots(qc, qn, q, cdf);
   const float mr = a.mrow[(int64_t)Q * a.Vt + t];
   x[0] = n;
   x[1] = n * n;
@@ -71,11 +102,12 @@ __device__ __forceinline__ void sample_feats(const WarmInferArgs& a, const float
 
 // All 15 raw features of a sample.
 __device__ __forceinline__ void all_feats(const WarmInferArgs& a, const float* qc, const float* qn,
-                                          int q, int Q, int64_t pos, float (&x)[F]) {
+                                          int q, int Q, int64_t pos, float (&x)[F],
+                                          const int16_t* bk = nullptr) {
   const int t = a.c.doc_t[pos];
   const int j = a.c.doc_of[pos];
   float s5[5];
-  sample_feats(a, qc, qn, q, Q, pos, t, s5);
+  sample_feats(a, qc, qn, q, Q, pos, t, s5, bk);
 #pragma unroll
   for (int k = 0; k < 5; ++k) x[kSampleFeat[k]] = s5[k];
 #pragma unroll
@@ -229,6 +261,7 @@ template <bool EXPORT>
 __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
   static_assert(H == 32 && F <= 16, "the matrix-core layout takes 32 units and 16 inputs");
   __shared__ float sqc[kWarmQueryMax], sqn[kWarmQueryMax];
+  __shared__ int16_t sbk[kKnotBuckets + 1];
   __shared__ __attribute__((aligned(16))) _Float16 stage[4][64 * kStageRow];
   const int Q = blockIdx.y;
   const int qb = a.q.q_ptr[Q], q = a.q.q_ptr[Q + 1] - qb;
@@ -238,6 +271,8 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
     sqn[i] = a.q.qn[qb + i];
   }
   __syncthreads();
+  build_knot_buckets(sqc, q, sbk);
+  __syncthreads();
   const cfloat* p = as_const(a.params);
   const int64_t base = (int64_t)blockIdx.x * (256 * kInferChunks);
   if constexpr (EXPORT) {
@@ -245,7 +280,7 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
       const int64_t pos = base + ch * 256 + threadIdx.x;
       if (pos >= a.c.nnz) break;
       float x[F];
-      all_feats(a, sqc, sqn, q, Q, pos, x);
+      all_feats(a, sqc, sqn, q, Q, pos, x, sbk);
       float* o = a.feats + ((int64_t)Q * a.c.nnz + pos) * F;
 #pragma unroll
       for (int k = 0; k < F; ++k) o[k] = x[k];
@@ -281,7 +316,7 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
       const int64_t pos = base + ch * 256 + threadIdx.x;
       const bool valid = pos < a.c.nnz;
       float x[F];
-      all_feats(a, sqc, sqn, q, Q, valid ? pos : a.c.nnz - 1, x);
+      all_feats(a, sqc, sqn, q, Q, valid ? pos : a.c.nnz - 1, x, sbk);
       half4_t zh[4];
 #pragma unroll
       for (int k = 0; k < 16; ++k)
