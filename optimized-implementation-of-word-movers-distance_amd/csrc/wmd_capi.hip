@@ -212,6 +212,7 @@ struct wmd_context {
   int nmax = 0;
   DevBuf<float> E;
   DevBuf<int32_t> touched, doc_ptr, doc_t;
+  DevBuf<int32_t> doc_t1;               // doc_t + 1 (the resident solver's biased word ids)
   DevBuf<int32_t> doc_perm;             // documents sorted by length class
   DevBuf<int32_t> long_docs;            // documents longer than the resident tiles
   std::vector<int32_t> h_long;          // host copy
@@ -297,6 +298,7 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   a.q_r = c->q_r.p;
   a.doc_ptr = c->doc_ptr.p;
   a.doc_t = c->doc_t.p;
+  a.doc_t1 = c->doc_t1.p;
   a.doc_val = c->doc_val.p;
   a.nnz = c->nnz;
   a.N = (int)c->N;
@@ -313,6 +315,18 @@ wmd::SolveArgs base_args(wmd_context* c, double lamda, const wmd_options& o) {
   a.lds_q = round_up4(std::max(c->qmax, 1));
   a.lds_n = round_up4(std::max(c->nmax, 1));
   return a;
+}
+
+// The resident solver addresses a query's kernel block with 32-bit byte
+// offsets (buffer loads) and forms row offsets with 24-bit multiplies: every
+// block must stay below 2 GiB and the corpus below 2^24 distinct words.
+bool resident_blocks_fit(const wmd_context* c) {
+  if (c->Vt >= (int64_t)1 << 24) return false;
+  for (int Q = 0; Q < c->nq; ++Q) {
+    const int64_t ks = round_up4(c->h_q_ptr[Q + 1] - c->h_q_ptr[Q]);
+    if (c->Vt * ks * 4 >= ((int64_t)1 << 31)) return false;
+  }
+  return true;
 }
 
 // Whether shape s has a kernel for the domain.
@@ -941,7 +955,7 @@ void wmd_context_destroy(wmd_context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->E.release(); c->touched.release(); c->doc_ptr.release(); c->doc_t.release();
-  c->doc_val.release(); c->q_ptr.release(); c->q_words.release(); c->k_stride.release();
+  c->doc_t1.release(); c->doc_val.release(); c->q_ptr.release(); c->q_words.release(); c->k_stride.release();
   c->qmeta.release(); c->act.release();
   c->bucket.release(); c->q_r.release(); c->k_off.release(); c->u_off.release();
   c->K.release(); c->mrow.release(); c->wmd.release(); c->iters.release();
@@ -1040,6 +1054,12 @@ int wmd_context_set_corpus(wmd_context* c, const float* vecs_f32, const double* 
   HIP_TRY(c->touched.upload(touched.data(), touched.size(), c->stream));
   HIP_TRY(c->doc_ptr.upload(dptr.data(), dptr.size(), c->stream));
   HIP_TRY(c->doc_t.upload(dt.data(), dt.size(), c->stream));
+  {
+    std::vector<int32_t> dt1(dt.size());
+    for (size_t i = 0; i < dt.size(); ++i) dt1[i] = dt[i] + 1;
+    HIP_TRY(c->doc_t1.upload(dt1.data(), dt1.size(), c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
   HIP_TRY(c->doc_val.upload(dv.data(), dv.size(), c->stream));
   // Documents grouped by length class (stable, so ids ascend inside a
   // class); documents beyond the last class go to the streaming kernel in
@@ -1302,18 +1322,22 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
   if (variant == WMD_VARIANT_RESIDENT) {
     // Documents beyond the tiles' length run on the streaming kernel (below);
     // a query beyond every tile sends the whole run there.
-    bool fits = true;
+    bool fits = resident_blocks_fit(c);
     for (int Q = 0; Q < c->nq && fits; ++Q) {
       const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
       fits = o.shape >= 0 ? shape_fits(o.shape, q, log_domain) : pick_shape(q, 0, log_domain) >= 0;
     }
     if (!fits) {
       if (o.shape >= 0) {
-        set_err("tile shape %d cannot hold the longest query (%d words), or has no log-domain "
-                "form", o.shape, c->qmax);
+        set_err(resident_blocks_fit(c)
+                    ? "tile shape %d cannot hold the longest query (%d words), or has no "
+                      "log-domain form"
+                    : "tile shape %d: a query's kernel block (%d-word query) exceeds the resident "
+                      "solver's 2 GiB block or 2^24-word limit",
+                o.shape, c->qmax);
         return WMD_ERR_UNSUPPORTED;
       }
-      variant = WMD_VARIANT_FUSED;  // queries beyond the register tiles
+      variant = WMD_VARIANT_FUSED;  // queries beyond the register tiles or blocks
     }
   }
   const bool long_docs = variant == WMD_VARIANT_RESIDENT && c->n_long > 0;
@@ -1832,6 +1856,7 @@ int wmd_context_run_query_noalloc(wmd_context* c, int32_t q, const int32_t* q_wo
   a.q_r = weights;
   a.doc_ptr = c->doc_ptr.p;
   a.doc_t = c->doc_t.p;
+  a.doc_t1 = c->doc_t1.p;
   a.doc_val = c->doc_val.p;
   a.nnz = c->nnz;
   a.N = (int)c->N;

@@ -104,6 +104,7 @@ struct SolveArgs {
   const float* q_r;        // query word weights (r restricted to the query)
   const int32_t* doc_ptr;  // N + 1, doc-major CSR
   const int32_t* doc_t;    // nnz compact word ids (index into touched)
+  const int32_t* doc_t1;   // nnz compact word ids + 1 (resident solver: 0 marks no word)
   const float* doc_val;    // nnz
   const int32_t* bucket_q; // queries of this launch
   const int32_t* doc_list; // documents of this launch (streaming: null = all N)

@@ -44,70 +44,111 @@ __device__ __forceinline__ void slot_sum_n(float (&v)[N]) {
   }
 }
 
-// Quotients o_k = c_k / s_k of a 16-lane row from the row's per-lane
-// partials s_k (k < N), without every lane summing and dividing every k:
-// two reduce-scatter steps (partner lane^8 by DPP row_ror:8, then the mirror
-// lane of the 8-lane half by row_half_mirror, each lane keeping the half of
-// the values its lane bit selects) leave quad g = (lane >> 2) & 3 with the
-// partial sums of values g*P/4 .. g*P/4 + P/4 - 1 (P = N rounded up to four,
-// the padding summed as 0) over four lanes that together cover the row; two
-// quad butterflies finish the sums, the quad divides its P/4 values (cq holds
-// the matching c_k), and DPP row_newbcast hands each quotient from the first
-// lane of its quad to the whole row, so every lane of the row holds the same
-// bits of every o_k.
-template <int N>
-__device__ __forceinline__ void row_quotients(const float (&s)[N], const float (&cq)[(N + 3) / 4],
-                                              float (&o)[N], int lane) {
-  constexpr int P = (N + 3) / 4 * 4;
-  constexpr int H = P / 2, Q = P / 4;
-  const bool up8 = (lane & 8) != 0;
-  const bool up4 = (lane & 4) != 0;
-  float h[H];
+// Document-side quotients o_k = c_k / s_k of a B-lane query-word group from
+// its lanes' partial sums s_k, by a reduce-scatter: each value is summed and
+// divided once per 2^(log2 B - L) lanes instead of in every lane.
+//
+// The levels pair each lane x of the group (x = lane & (B - 1)) with
+//   B = 16: x ^ 8 (DPP row_ror:8), then the mirror lane of its 8-lane half
+//           (row_half_mirror, x ^ 7), then x ^ 2, x ^ 1 (quad_perm);
+//   B = 8: mirror, x ^ 2, x ^ 1;   B = 4: x ^ 2, x ^ 1;   B = 2: x ^ 1.
+// The first L levels scatter, the rest are butterflies.  A lane's P values
+// (the group's P tile rows) are blocks of R = P / 2^L, and the lane stores
+// block h in register block h ^ g(x), g(x) = x >> (log2 B - L).  At a scatter
+// level every lane keeps the lower half of its current register blocks and
+// adds its partner's upper half into them: the partner's block h sits in the
+// partner's register block h ^ g(partner), and g(x) ^ g(partner) is the same
+// constant D for every lane of the level, so the pairing of registers is a
+// compile-time permutation and no lane selects what to keep or send.  After
+// the L levels a lane holds the partial sums of its own block g(x) in
+// register block 0; the butterflies finish them, the lane divides its R
+// values, and the levels run backwards to hand every quotient to the whole
+// group (each lane receives its partner's register blocks into the upper
+// half, at the same constant permutation).  Every quotient is formed once and
+// copied, so every lane of the group holds the same bits.
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v >> 1); }
+
+// Partner of level lv as an xor pattern on the group position (7 is the
+// 8-lane mirror), in level order.
+constexpr int level_xor(int B, int lv) {
+  return B == 16 ? (lv == 0 ? 8 : lv == 1 ? 7 : lv == 2 ? 2 : 1)
+       : B == 8  ? (lv == 0 ? 7 : lv == 1 ? 2 : 1)
+       : B == 4  ? (lv == 0 ? 2 : 1)
+                 : 1;
+}
+
+template <int PAT>
+__device__ __forceinline__ float partner_mov(float v) {
+  if constexpr (PAT == 8) return dpp_mov<0x128>(v);       // row_ror:8
+  else if constexpr (PAT == 7) return dpp_mov<0x141>(v);  // row_half_mirror
+  else if constexpr (PAT == 2) return dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  else return dpp_mov<0xB1>(v);                           // quad_perm [1,0,3,2]
+}
+
+// Rows and scatter levels of a tile: P >= NE rows (the rows past NE are
+// padding with c = 0 and kernel entries 0), L scatter levels with 2^L | P.
+// Cost per sweep in VALU instructions: the scatter adds and the broadcast
+// moves (P - R each), the butterflies and divisions of the R owned values,
+// and the two FMAs per padded tile entry (weighted up for their registers).
+struct RowPlan {
+  int P, L;
+};
+constexpr int row_plan_cost(int B, int NE, int NI, int P, int L) {
+  return 2 * (P - (P >> L)) + (ilog2(B) - L + 2) * (P >> L) + 3 * (P - NE) * NI;
+}
+constexpr RowPlan row_plan(int B, int NE, int NI) {
+  RowPlan best{NE, 0};
+  int cost = row_plan_cost(B, NE, NI, NE, 0);
+  for (int L = 1; L <= ilog2(B); ++L)
+    for (int P = NE; P <= NE + 3; ++P)
+      if (P % (1 << L) == 0 && row_plan_cost(B, NE, NI, P, L) < cost) {
+        cost = row_plan_cost(B, NE, NI, P, L);
+        best = RowPlan{P, L};
+      }
+  return best;
+}
+
+template <int B, int P, int L, int LV>
+__device__ __forceinline__ void quot_scatter(float (&s)[P]) {
+  if constexpr (LV < L) {
+    constexpr int PAT = level_xor(B, LV), R = P >> L, D = PAT >> (ilog2(B) - L);
+    constexpr int H = (P >> LV) / 2;
 #pragma unroll
-  for (int j = 0; j < H; ++j) {
-    const float lo = (j < N) ? s[j] : 0.f;
-    const float hi = (j + H < N) ? s[j + H] : 0.f;
-    const float keep = up8 ? hi : lo;
-    const float send = up8 ? lo : hi;
-    h[j] = keep + dpp_mov<0x128>(send);  // row_ror:8 == lane ^ 8
-  }
-  float w[Q];
-#pragma unroll
-  for (int j = 0; j < Q; ++j) {
-    const float keep = up4 ? h[j + Q] : h[j];
-    const float send = up4 ? h[j] : h[j + Q];
-    w[j] = keep + dpp_mov<0x141>(send);  // row_half_mirror: lane i <-> 7 - i
-  }
-#pragma unroll
-  for (int j = 0; j < Q; ++j) w[j] = xor_step<2>(xor_step<1>(w[j]));
-#pragma unroll
-  for (int j = 0; j < Q; ++j) w[j] = fdiv(cq[j], w[j]);
-  // row_newbcast:n (DPP 0x150 + n): every lane of a 16-lane row reads lane n.
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    switch (k / Q) {
-      case 0: o[k] = dpp_mov<0x150>(w[k % Q]); break;
-      case 1: o[k] = dpp_mov<0x154>(w[k % Q]); break;
-      case 2: o[k] = dpp_mov<0x158>(w[k % Q]); break;
-      default: o[k] = dpp_mov<0x15C>(w[k % Q]); break;
-    }
+    for (int j = 0; j < H; ++j) s[j] += partner_mov<PAT>(s[((j / R) ^ D) * R + j % R]);
+    quot_scatter<B, P, L, LV + 1>(s);
   }
 }
 
-// The c_k that quad (lane >> 2) & 3 divides by in row_quotients.
-template <int N>
-__device__ __forceinline__ void row_quotient_c(const float (&c)[N], float (&cq)[(N + 3) / 4],
-                                               int lane) {
-  constexpr int Q = (N + 3) / 4;
-  const int g = (lane >> 2) & 3;
+template <int B, int P, int L, int LV>
+__device__ __forceinline__ void quot_butterfly(float (&s)[P]) {
+  if constexpr (LV < ilog2(B)) {
+    constexpr int PAT = level_xor(B, LV);
 #pragma unroll
-  for (int j = 0; j < Q; ++j) {
-    const float c0 = c[j];
-    const float c1 = (Q + j < N) ? c[Q + j] : 0.f;
-    const float c2 = (2 * Q + j < N) ? c[2 * Q + j] : 0.f;
-    const float c3 = (3 * Q + j < N) ? c[3 * Q + j] : 0.f;
-    cq[j] = (g == 0) ? c0 : (g == 1) ? c1 : (g == 2) ? c2 : c3;
+    for (int r = 0; r < (P >> L); ++r) s[r] += partner_mov<PAT>(s[r]);
+    quot_butterfly<B, P, L, LV + 1>(s);
   }
+}
+
+template <int B, int P, int L, int LV>
+__device__ __forceinline__ void quot_broadcast(float (&o)[P]) {
+  if constexpr (LV >= 0) {
+    constexpr int PAT = level_xor(B, LV), R = P >> L, D = PAT >> (ilog2(B) - L);
+    constexpr int H = (P >> LV) / 2;
+#pragma unroll
+    for (int j = 0; j < H; ++j) o[((j / R) ^ D) * R + j % R] = partner_mov<PAT>(o[j]);
+    quot_broadcast<B, P, L, LV - 1>(o);
+  }
+}
+
+// o_k = c_k / (sum over the group of s_k) for every register row k, with
+// c and s in the lane's row order (block h in register block h ^ g).
+template <int B, int P, int L>
+__device__ __forceinline__ void row_quot(float (&s)[P], const float (&c)[P], float (&o)[P]) {
+  quot_scatter<B, P, L, 0>(s);
+  quot_butterfly<B, P, L, L>(s);
+#pragma unroll
+  for (int r = 0; r < (P >> L); ++r) o[r] = fdiv(c[r], s[r]);
+  quot_broadcast<B, P, L, L - 1>(o);
 }
 
 // Select between two values under a wave-uniform condition as a bitwise
@@ -293,13 +334,28 @@ __device__ __forceinline__ int slot_of(int b, int m) {
   return B * (NI4 + R2) + b + B * (m - NI4 - R2);
 }
 
+// Buffer resource over `bytes` bytes from `base` (raw, no swizzle): a load
+// whose byte offset is at or past `bytes` returns 0 without touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ float buf_f32(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ uint32_t buf_u32(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+
 // Resident solver.  Lane grid per pair: A = 64W/B document-word lanes by B
 // query-word lanes (B <= 16, so a query-word group never leaves a 16-lane
-// row).  Lane (ea, b) holds Kt[e][i] for e = ea + A*k (k < NE) and
-// i = slot_of(b, m) (m < NI).  Per iteration (sinkhorn_wmd_fused.cpp:249-319):
-//   s_e = sum_i K[e,i] u_i      (partial over own i, DPP butterfly over the
-//                                B lanes of the row)
-//   o_e = c_e / s_e
+// row).  Lane (ea, b) holds Kt[e][i] for its P tile rows e (row k is
+// document word ea + A * v(k), v the row order of row_quot) and query-word
+// slots i = slot_of(b, m) (m < NI).  Per iteration
+// (sinkhorn_wmd_fused.cpp:249-319):
+//   s_e = sum_i K[e,i] u_i      (partial over own i; row_quot sums over the
+//   o_e = c_e / s_e              B lanes of the group and divides once per
+//                                value)
 //   x_i = sum_e K[e,i] o_e      (partial over own e; reduce-scatter across the
 //                                four rows, DPP steps inside a row, and for
 //                                W > 1 the other waves' sums of the same
@@ -312,13 +368,22 @@ __device__ __forceinline__ int slot_of(int b, int m) {
 // M recovered from the rescaled kernel as mrow + log(Kt) / lamda.
 //
 // A launch covers one tile shape and one document-length class: NE is the
-// class's slot count, so the iteration loop has no slot guards.  Slots past
-// the document's or the query's end hold K = 0 and contribute nothing.
+// class's slot count, so the iteration loop has no slot guards.  Rows past
+// the document's end and slots past the query's end hold K = 0 and
+// contribute nothing.
+//
+// The tile is gathered in two rounds of buffer loads with no branches:
+// first every row's biased word id (t + 1; 0 past the document's end, where
+// the load is out of range) and weight, then every row's kernel runs and row
+// minimum.  A biased id of 0, or a run that starts past q, makes the run's
+// byte offset wrap past the block, so the load returns 0 from no memory.
 template <int W, int B, int NI, int NE>
 __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a) {
   static_assert(B <= 16, "query-word groups stay inside a 16-lane row");
   constexpr int AW = 64 / B;
   constexpr int A = W * AW;
+  constexpr RowPlan kPlan = row_plan(B, NE, NI);
+  constexpr int P = kPlan.P, L = kPlan.L, R = P >> L;
   // After the scatter a lane owns NO = NIP / 4 of the x sums (NIP pads NI to
   // a multiple of 4): slot k <-> query-word slot own_slot<2>(k, rho).
   constexpr int NIP = (NI + 3) / 4 * 4;
@@ -335,65 +400,72 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   if (pair >= a.n_pairs) return;  // W == 1: per wave (no block barriers); W > 1: whole block
   const int b = lane & (B - 1);
   const int ea = ((W == 1) ? 0 : wv) * AW + lane / B;
+  // Register block h of the lane holds document block h ^ g (row_quot).
+  const int g = b >> (ilog2(B) - L);
 
   const int Q = a.bucket_q[pair / a.n_list];
   const int j = a.doc_list[pair % a.n_list];
   const int qb = a.q_ptr[Q];
   const int q = a.q_ptr[Q + 1] - qb;
   const int ks = a.k_stride[Q];
-  const float* Kq = a.K + a.k_off[Q];
   const int db = a.doc_ptr[j];
   const int n = a.doc_ptr[j + 1] - db;
 
-  float Kt[NE][NI];
-  float c[NE];
+  // Byte offset in the document of row k's word: e = ea + A v(k).
+  auto row_off = [&](int k) -> uint32_t {
+    return (uint32_t)(ea + A * (((k / R) ^ g) * R + k % R)) * 4u;
+  };
+  const auto rs_t = raw_rsrc(a.doc_t1 + db, (uint32_t)n * 4u);
+  const auto rs_c = raw_rsrc(a.doc_val + db, (uint32_t)n * 4u);
+  const auto rs_k = raw_rsrc(a.K + a.k_off[Q], (uint32_t)a.Vt * (uint32_t)ks * 4u);
+  const auto rs_m = raw_rsrc(a.mrow + (int64_t)Q * a.Vt, (uint32_t)a.Vt * 4u);
+
+  float Kt[P][NI];
+  float c[P];
+  uint32_t t1[P];
+#pragma unroll
+  for (int k = 0; k < P; ++k) {
+    t1[k] = buf_u32(rs_t, row_off(k));
+    c[k] = buf_f32(rs_c, row_off(k));
+  }
+  // Run starts: 16-byte runs of four slots, then at most one 8-byte run,
+  // then single slots (slot_of); offset of word t's run is
+  // (t + 1) ks4 + run_base with run_base = 4 i - ks4, or 2^31 for a run past q.
+  const uint32_t ks4 = (uint32_t)ks * 4u;
+  constexpr int NI4 = NI & ~3;
+  constexpr int R2 = (NI - NI4) & 2;
+  auto run_base = [&](int m) -> uint32_t {
+    const int i = slot_of<B, NI>(b, m);
+    return i < q ? (uint32_t)i * 4u - ks4 : 0x80000000u;
+  };
   // The row-minimum part of the distance, sum_e o_e s_e mrow_e, is
   // sum_e c_e mrow_e once the final half-step has set o = c / s: gathered with
-  // the tile, by the first query-word lane of each row, in FP64.
+  // the tile in FP64 (every lane forms its rows' sum; only the first
+  // query-word lane of each group keeps it).
   double cm = 0.0;
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    const int e = ea + A * k;
-    const bool ve = e < n;
-    int t = 0;
-    float cv = 0.f;
-    if (ve) {
-      t = a.doc_t[db + e];
-      cv = a.doc_val[db + e];
-      if (b == 0) cm = fma((double)cv, (double)a.mrow[(int64_t)Q * a.Vt + t], cm);
-    }
-    c[k] = cv;
-    const float* row = Kq + (int64_t)t * ks;
-    constexpr int NI4 = NI & ~3;
-    constexpr int R2 = (NI - NI4) & 2;
+  for (int k = 0; k < P; ++k) {
 #pragma unroll
     for (int m = 0; m < NI4; m += 4) {
-      const int i = slot_of<B, NI>(b, m);
-      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (ve && i < q) v = *reinterpret_cast<const float4*>(row + i);
-      Kt[k][m] = v.x;
-      Kt[k][m + 1] = (i + 1 < q) ? v.y : 0.f;
-      Kt[k][m + 2] = (i + 2 < q) ? v.z : 0.f;
-      Kt[k][m + 3] = (i + 3 < q) ? v.w : 0.f;
+      const auto v =
+          __builtin_amdgcn_raw_buffer_load_b128(rs_k, __umul24(t1[k], ks4) + run_base(m), 0, 0);
+      Kt[k][m] = __uint_as_float(v[0]);
+      Kt[k][m + 1] = __uint_as_float(v[1]);
+      Kt[k][m + 2] = __uint_as_float(v[2]);
+      Kt[k][m + 3] = __uint_as_float(v[3]);
     }
     if constexpr (R2 == 2) {
-      const int i = slot_of<B, NI>(b, NI4);
-      float2 v = make_float2(0.f, 0.f);
-      if (ve && i < q) v = *reinterpret_cast<const float2*>(row + i);
-      Kt[k][NI4] = v.x;
-      Kt[k][NI4 + 1] = (i + 1 < q) ? v.y : 0.f;
+      const auto v =
+          __builtin_amdgcn_raw_buffer_load_b64(rs_k, __umul24(t1[k], ks4) + run_base(NI4), 0, 0);
+      Kt[k][NI4] = __uint_as_float(v[0]);
+      Kt[k][NI4 + 1] = __uint_as_float(v[1]);
     }
 #pragma unroll
-    for (int m = NI4 + R2; m < NI; ++m) {
-      const int i = slot_of<B, NI>(b, m);
-      Kt[k][m] = (ve && i < q) ? row[i] : 0.f;
-    }
+    for (int m = NI4 + R2; m < NI; ++m)
+      Kt[k][m] = buf_f32(rs_k, __umul24(t1[k], ks4) + run_base(m));
+    cm = fma((double)c[k], (double)buf_f32(rs_m, t1[k] * 4u - 4u), cm);
   }
-  // Rows that are one query-word group divide each o_e once per quad
-  // (row_quotients) instead of in every lane.
-  constexpr bool kRowQuot = B == 16 && NE >= 3;
-  float cq[(NE + 3) / 4];
-  if constexpr (kRowQuot) row_quotient_c<NE>(c, cq, lane);
+  if (b != 0) cm = 0.0;
   float u[NI];
 #pragma unroll
   for (int m = 0; m < NI; ++m)
@@ -419,7 +491,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   // is not monotone) restarts once from the uniform start.  Slow but steady
   // progress is not a stall: such pairs are slow from any start.
   const bool warm_on = a.warm != nullptr;
-  bool seeded = warm_on;
   bool fell_back = !warm_on;
   float res_win = INFINITY, res_low = INFINITY;
   UpdateState<NO> st;
@@ -428,45 +499,20 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   int it = 0;
   float res = 0.f;
 
-  // One sweep: o = c / (K^T u) (or the seeded v), then the owned x = K o.
-  // Every partial sum starts at kFltMin instead of 0: a padded slot (c = 0,
-  // r = 0, kernel entries 0) then ends with a sum of a few kFltMin, so its
-  // quotient is exactly 0 without a select, while real sums (bounded below by
-  // the largest entry of a rescaled row, 1, times a scaling) are unchanged
-  // except at exact rounding ties.
-  auto sweep = [&](float (&xo)[NO]) {
-    float o[NE];
-    if (seeded) {
-#pragma unroll
-      for (int k = 0; k < NE; ++k) {
-        const int e = ea + A * k;
-        const float lv = (c[k] > 0.f) ? a.warm[(int64_t)Q * a.nnz + db + e] : 0.f;
-        o[k] = (c[k] > 0.f) ? __expf(fminf(fmaxf(lv, -60.f), 60.f)) : 0.f;
-      }
-    } else {
-      float s[NE];
-#pragma unroll
-      for (int k = 0; k < NE; ++k) {
-        float acc = kFltMin;
-#pragma unroll
-        for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
-        s[k] = acc;
-      }
-      if constexpr (kRowQuot) {
-        row_quotients<NE>(s, cq, o, lane);
-      } else {
-        group_sum_n<B>(s);
-#pragma unroll
-        for (int k = 0; k < NE; ++k) o[k] = fdiv(c[k], s[k]);
-      }
-    }
+  // x = K o for the owned slots.  Every partial sum starts at kFltMin
+  // instead of 0: a padded slot (c = 0, r = 0, kernel entries 0) then ends
+  // with a sum of a few kFltMin, so its quotient is exactly 0 without a
+  // select, while real sums (bounded below by the largest entry of a
+  // rescaled row, 1, times a scaling) are unchanged except at exact rounding
+  // ties.
+  auto x_side = [&](const float (&o)[P], float (&xo)[NO]) {
     float xs[NIP];
 #pragma unroll
     for (int m = 0; m < NIP; ++m) {
       float x = kFltMin;
       if (m < NI) {
 #pragma unroll
-        for (int k = 0; k < NE; ++k) x = fmaf(Kt[k][m], o[k], x);
+        for (int k = 0; k < P; ++k) x = fmaf(Kt[k][m], o[k], x);
       }
       xs[m] = x;
     }
@@ -507,6 +553,19 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       }
     }
   };
+  // One sweep from the current u: o = c / (K^T u), then the owned x = K o.
+  auto sweep = [&](float (&xo)[NO]) {
+    float s[P], o[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      float acc = kFltMin;
+#pragma unroll
+      for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
+      s[k] = acc;
+    }
+    row_quot<B, P, L>(s, c, o);
+    x_side(o, xo);
+  };
   // After an update: gather the scalings, count the iteration, and run the
   // stall check of a warm-started pair; true when the pair restarts.
   auto advance = [&]() -> bool {
@@ -515,10 +574,6 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 #pragma unroll
     for (int m = 0; m < NI; ++m) u[m] = ug[m];
     ++it;
-    if (seeded) {
-      seeded = false;
-      return false;
-    }
     res_low = fminf(res_low, res);
     if (!fell_back && (it & 63) == 0) {
       if (res_low >= res_win) return true;
@@ -526,6 +581,33 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     }
     return false;
   };
+
+  // The seeded half-step: v from the prediction (0 for rows with no weight),
+  // then the update with the seeded flag (no stop test, no acceleration
+  // history).
+  if (warm_on && a.max_iter > 0) {
+    const auto rs_w = raw_rsrc(a.warm + (int64_t)Q * a.nnz + db, (uint32_t)n * 4u);
+    float o[P];
+#pragma unroll
+    for (int k = 0; k < P; ++k) o[k] = buf_f32(rs_w, row_off(k));
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const float v = __expf(__builtin_amdgcn_fmed3f(o[k], -60.f, 60.f));
+      o[k] = mask_select(opaque_mask(c[k] > 0.f), v, 0.f);
+    }
+    float xo[NO];
+    x_side(o, xo);
+    bool stop, to_plain;
+    if (a.accel)
+      accel_update<B>(a, st, xo, ro, lro, uo, true, inv_q, it, stop, to_plain);
+    else
+      plain_update<B>(a, st, xo, ro, lro, uo, true, stop);
+    float ug[NIP];
+    rows_gather<NIP>(uo, ug);
+#pragma unroll
+    for (int m = 0; m < NI; ++m) u[m] = ug[m];
+    ++it;
+  }
 
   // The accelerated rule and the plain rules run in loops of their own (an
   // accelerated pair whose progress guard fires continues in the plain
@@ -541,8 +623,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
         float xo[NO];
         sweep(xo);
         bool stop, to_plain;
-        const float rv = accel_update<B>(a, st, xo, ro, lro, uo, seeded, inv_q, it, stop, to_plain);
-        if (!seeded) res = rv;
+        res = accel_update<B>(a, st, xo, ro, lro, uo, false, inv_q, it, stop, to_plain);
         restart = advance();
         if (stop) {  // u kept the verified point
           restart = false;
@@ -562,8 +643,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
         float xo[NO];
         sweep(xo);
         bool stop;
-        const float rv = plain_update<B>(a, st, xo, ro, lro, uo, seeded, stop);
-        if (!seeded) res = rv;
+        res = plain_update<B>(a, st, xo, ro, lro, uo, false, stop);
         restart = advance();
         if (stop) {  // the step is taken; a stall check due now is moot
           restart = false;
@@ -588,24 +668,32 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   // with M = mrow + ln(K) / lamda, o_e = c_e / s_e, s_e = sum_i u_i K_ei and
   // T_e = sum_i u_i K_ei log2 K_ei (every term <= 0, so the FP32 partials are
   // well conditioned; K + FLT_MIN keeps the padded entries' 0 * log2 finite
-  // and changes no normal K).  The products o_e T_e are summed in FP64.
+  // and changes no normal K).  The products o_e T_e of each lane's partial
+  // T_e are summed in FP64 (padded rows have o = 0).
   const double ln2_l = 0.6931471805599453 / (double)a.lamda;
-  double acc = 0.0;
+  float s[P], tt[P], o[P];
 #pragma unroll
-  for (int k = 0; k < NE; ++k) {
-    float s = 0.f, t = 0.f;
+  for (int k = 0; k < P; ++k) {
+    float sk = kFltMin, t = 0.f;
 #pragma unroll
     for (int m = 0; m < NI; ++m) {
       const float kv = Kt[k][m];
-      s = fmaf(kv, u[m], s);
+      sk = fmaf(kv, u[m], sk);
       t = fmaf(u[m] * kv, __builtin_amdgcn_logf(kv + kFltMin), t);
     }
-    s = group_sum<B>(s);
-    const int e = ea + A * k;
-    if (e < n) {
-      const float oe = fdiv(c[k], s);
-      if (a.duals) a.duals[(int64_t)Q * a.nnz + db + e] = __logf(oe);
-      acc += (double)(oe * t);
+    s[k] = sk;
+    tt[k] = t;
+  }
+  row_quot<B, P, L>(s, c, o);
+  double acc = 0.0;
+#pragma unroll
+  for (int k = 0; k < P; ++k) acc += (double)(o[k] * tt[k]);
+  if (a.duals) {
+    float* dq = a.duals + (int64_t)Q * a.nnz + db;
+#pragma unroll
+    for (int k = 0; k < P; ++k) {
+      const uint32_t e = row_off(k) / 4u;
+      if (b == 0 && e < (uint32_t)n) dq[e] = __logf(o[k]);
     }
   }
   acc = wave_sum_d(fma(acc, ln2_l, cm));
