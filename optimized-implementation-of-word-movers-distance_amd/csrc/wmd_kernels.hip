@@ -1021,6 +1021,7 @@ __global__ __launch_bounds__(64, kStreamWaves) void k_spmm_final(SolveArgs a) {
 // loop and runs concurrently with the other launches of the resident plan.
 constexpr int kLongThreads = 512;  // one thread per query word of the longest query
 constexpr int kLongWaves = kLongThreads / 64;
+constexpr int kLongRows = 8;      // document words in flight per wave (document-side half-step)
 
 // Block-wide sum of per-thread partials (every thread gets the same total:
 // the waves' sums added in wave order).
@@ -1075,46 +1076,48 @@ __global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
   // form).  One wave per document word, lanes over the query words (the
   // kernel row is read coalesced), four words in flight per wave.
   auto sddmm = [&]() {
-    for (int e0 = wv; e0 < n; e0 += 4 * kLongWaves) {
-      const float* row[4];
-      bool live[4];
+    for (int e0 = wv; e0 < n; e0 += kLongRows * kLongWaves) {
+      const float* row[kLongRows];
+      bool live[kLongRows];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+      for (int r = 0; r < kLongRows; ++r) {
         const int e = e0 + r * kLongWaves;
         live[r] = e < n;
         row[r] = Kq + (int64_t)(live[r] ? t_sh[e] : 0) * ks;
       }
-      float v[4];
+      float v[kLongRows];
       if constexpr (LOG) {
-        float mx[4] = {kLogZero, kLogZero, kLogZero, kLogZero};
+        float mx[kLongRows];
+#pragma unroll
+        for (int r = 0; r < kLongRows; ++r) mx[r] = kLogZero;
         for (int ii = lane; ii < q; ii += 64) {
           const float f = u_sh[ii];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) mx[r] = fmaxf(mx[r], row[r][ii] + f);
+          for (int r = 0; r < kLongRows; ++r) mx[r] = fmaxf(mx[r], row[r][ii] + f);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) mx[r] = bfly_red<1, 32, kMax>(mx[r]);
-        float sm[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int r = 0; r < kLongRows; ++r) mx[r] = bfly_red<1, 32, kMax>(mx[r]);
+        float sm[kLongRows] = {};
         for (int ii = lane; ii < q; ii += 64) {
           const float f = u_sh[ii];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) sm[r] += exp2f(row[r][ii] + f - mx[r]);
+          for (int r = 0; r < kLongRows; ++r) sm[r] += exp2f(row[r][ii] + f - mx[r]);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = mx[r] + __log2f(wave_sum(sm[r]));
+        for (int r = 0; r < kLongRows; ++r) v[r] = mx[r] + __log2f(wave_sum(sm[r]));
       } else {
-        float sm[4] = {0.f, 0.f, 0.f, 0.f};
+        float sm[kLongRows] = {};
         for (int ii = lane; ii < q; ii += 64) {
           const float u = u_sh[ii];
 #pragma unroll
-          for (int r = 0; r < 4; ++r) sm[r] = fmaf(row[r][ii], u, sm[r]);
+          for (int r = 0; r < kLongRows; ++r) sm[r] = fmaf(row[r][ii], u, sm[r]);
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = wave_sum(sm[r]);
+        for (int r = 0; r < kLongRows; ++r) v[r] = wave_sum(sm[r]);
       }
       if (lane == 0) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
+        for (int r = 0; r < kLongRows; ++r) {
           const int e = e0 + r * kLongWaves;
           if (live[r]) {
             const float c = a.doc_val[db + e];
@@ -1134,7 +1137,7 @@ __global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
     if constexpr (LOG) {
 #pragma unroll
       for (int c = 0; c < kC; ++c) p[c] = kLogZero;
-#pragma unroll 4
+#pragma unroll 8
       for (int e = wv; e < n; e += kLongWaves) {
         const float* row = Kq + (int64_t)t_sh[e] * ks;
         const float g = o_sh[e];
@@ -1162,7 +1165,7 @@ __global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
         p[c] = 0.f;
       }
       __syncthreads();
-#pragma unroll 4
+#pragma unroll 8
       for (int e = wv; e < n; e += kLongWaves) {
         const float* row = Kq + (int64_t)t_sh[e] * ks;
         const float g = o_sh[e];
@@ -1184,7 +1187,7 @@ __global__ __launch_bounds__(kLongThreads) void k_long(SolveArgs a) {
     } else {
 #pragma unroll
       for (int c = 0; c < kC; ++c) p[c] = 0.f;
-#pragma unroll 4
+#pragma unroll 8
       for (int e = wv; e < n; e += kLongWaves) {
         const float* row = Kq + (int64_t)t_sh[e] * ks;
         const float o = o_sh[e];

@@ -85,70 +85,78 @@ __device__ __forceinline__ float partner_mov(float v) {
   else return dpp_mov<0xB1>(v);                           // quad_perm [1,0,3,2]
 }
 
-// Rows and scatter levels of a tile: P >= NE rows (the rows past NE are
-// padding with c = 0 and kernel entries 0), L scatter levels with 2^L | P.
-// Cost per sweep in VALU instructions: the scatter adds and the broadcast
-// moves (P - R each), the butterflies and divisions of the R owned values,
-// and the two FMAs per padded tile entry (weighted up for their registers).
-struct RowPlan {
-  int P, L;
-};
-constexpr int row_plan_cost(int B, int NE, int NI, int P, int L) {
-  return 2 * (P - (P >> L)) + (ilog2(B) - L + 2) * (P >> L) + 3 * (P - NE) * NI;
+// Scatter levels of a tile of NE rows: the first PS = NE rounded down to a
+// multiple of 2^L rows are scattered (and stored in the permuted order), the
+// NE - PS rows after them are summed by butterflies and divided in every
+// lane.  Cost per sweep in VALU instructions: the scatter adds and the
+// broadcast moves (PS - R each), the butterflies and divisions of the R
+// owned values, and log2 B butterfly steps plus a division per tail row.
+constexpr int row_scatter_cost(int B, int NE, int L) {
+  const int ps = NE / (1 << L) * (1 << L), r = ps >> L;
+  return 2 * (ps - r) + (ilog2(B) - L + 2) * r + (ilog2(B) + 2) * (NE - ps);
 }
-constexpr RowPlan row_plan(int B, int NE, int NI) {
-  RowPlan best{NE, 0};
-  int cost = row_plan_cost(B, NE, NI, NE, 0);
+constexpr int row_scatter_levels(int B, int NE) {
+  int best = 0;
   for (int L = 1; L <= ilog2(B); ++L)
-    for (int P = NE; P <= NE + 3; ++P)
-      if (P % (1 << L) == 0 && row_plan_cost(B, NE, NI, P, L) < cost) {
-        cost = row_plan_cost(B, NE, NI, P, L);
-        best = RowPlan{P, L};
-      }
+    if (NE >= (1 << L) && row_scatter_cost(B, NE, L) < row_scatter_cost(B, NE, best)) best = L;
   return best;
 }
 
-template <int B, int P, int L, int LV>
-__device__ __forceinline__ void quot_scatter(float (&s)[P]) {
+// The scatter, butterfly and broadcast levels over the first P values of an
+// N-value array (P a multiple of 2^L).
+template <int B, int N, int P, int L, int LV>
+__device__ __forceinline__ void quot_scatter(float (&s)[N]) {
   if constexpr (LV < L) {
     constexpr int PAT = level_xor(B, LV), R = P >> L, D = PAT >> (ilog2(B) - L);
     constexpr int H = (P >> LV) / 2;
 #pragma unroll
     for (int j = 0; j < H; ++j) s[j] += partner_mov<PAT>(s[((j / R) ^ D) * R + j % R]);
-    quot_scatter<B, P, L, LV + 1>(s);
+    quot_scatter<B, N, P, L, LV + 1>(s);
   }
 }
 
-template <int B, int P, int L, int LV>
-__device__ __forceinline__ void quot_butterfly(float (&s)[P]) {
+template <int B, int N, int P, int L, int LV>
+__device__ __forceinline__ void quot_butterfly(float (&s)[N]) {
   if constexpr (LV < ilog2(B)) {
     constexpr int PAT = level_xor(B, LV);
 #pragma unroll
     for (int r = 0; r < (P >> L); ++r) s[r] += partner_mov<PAT>(s[r]);
-    quot_butterfly<B, P, L, LV + 1>(s);
+    quot_butterfly<B, N, P, L, LV + 1>(s);
   }
 }
 
-template <int B, int P, int L, int LV>
-__device__ __forceinline__ void quot_broadcast(float (&o)[P]) {
+template <int B, int N, int P, int L, int LV>
+__device__ __forceinline__ void quot_broadcast(float (&o)[N]) {
   if constexpr (LV >= 0) {
     constexpr int PAT = level_xor(B, LV), R = P >> L, D = PAT >> (ilog2(B) - L);
     constexpr int H = (P >> LV) / 2;
 #pragma unroll
     for (int j = 0; j < H; ++j) o[((j / R) ^ D) * R + j % R] = partner_mov<PAT>(o[j]);
-    quot_broadcast<B, P, L, LV - 1>(o);
+    quot_broadcast<B, N, P, L, LV - 1>(o);
   }
 }
 
-// o_k = c_k / (sum over the group of s_k) for every register row k, with
-// c and s in the lane's row order (block h in register block h ^ g).
-template <int B, int P, int L>
-__device__ __forceinline__ void row_quot(float (&s)[P], const float (&c)[P], float (&o)[P]) {
-  quot_scatter<B, P, L, 0>(s);
-  quot_butterfly<B, P, L, L>(s);
+// o_k = c_k / (sum over the group of s_k) for every register row k < NE,
+// with c and s in the lane's row order (rows below PS: block h in register
+// block h ^ g; the tail rows in document order).
+template <int B, int NE, int L>
+__device__ __forceinline__ void row_quot(float (&s)[NE], const float (&c)[NE], float (&o)[NE]) {
+  constexpr int PS = NE / (1 << L) * (1 << L);
+  if constexpr (PS > 0) {
+    quot_scatter<B, NE, PS, L, 0>(s);
+    quot_butterfly<B, NE, PS, L, L>(s);
 #pragma unroll
-  for (int r = 0; r < (P >> L); ++r) o[r] = fdiv(c[r], s[r]);
-  quot_broadcast<B, P, L, L - 1>(o);
+    for (int r = 0; r < (PS >> L); ++r) o[r] = fdiv(c[r], s[r]);
+    quot_broadcast<B, NE, PS, L, L - 1>(o);
+  }
+  if constexpr (PS < NE) {
+    float t[NE - PS];
+#pragma unroll
+    for (int k = PS; k < NE; ++k) t[k - PS] = s[k];
+    group_sum_n<B>(t);
+#pragma unroll
+    for (int k = PS; k < NE; ++k) o[k] = fdiv(c[k], t[k - PS]);
+  }
 }
 
 // Select between two values under a wave-uniform condition as a bitwise
@@ -382,8 +390,8 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   static_assert(B <= 16, "query-word groups stay inside a 16-lane row");
   constexpr int AW = 64 / B;
   constexpr int A = W * AW;
-  constexpr RowPlan kPlan = row_plan(B, NE, NI);
-  constexpr int P = kPlan.P, L = kPlan.L, R = P >> L;
+  constexpr int L = row_scatter_levels(B, NE);
+  constexpr int P = NE, PS = NE / (1 << L) * (1 << L), R = PS >> L;
   // After the scatter a lane owns NO = NIP / 4 of the x sums (NIP pads NI to
   // a multiple of 4): slot k <-> query-word slot own_slot<2>(k, rho).
   constexpr int NIP = (NI + 3) / 4 * 4;
@@ -411,10 +419,12 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   const int db = a.doc_ptr[j];
   const int n = a.doc_ptr[j + 1] - db;
 
-  // Byte offset in the document of row k's word: e = ea + A v(k).
-  auto row_off = [&](int k) -> uint32_t {
-    return (uint32_t)(ea + A * (((k / R) ^ g) * R + k % R)) * 4u;
+  // Byte offset in the document of row k's word: e = ea + A v(k), v(k) the
+  // row order of row_quot.
+  auto row_off_from = [](int ea_, int g_, int k) -> uint32_t {
+    return (uint32_t)(ea_ + A * (k < PS ? ((k / R) ^ g_) * R + k % R : k)) * 4u;
   };
+  auto row_off = [&](int k) -> uint32_t { return row_off_from(ea, g, k); };
   const auto rs_t = raw_rsrc(a.doc_t1 + db, (uint32_t)n * 4u);
   const auto rs_c = raw_rsrc(a.doc_val + db, (uint32_t)n * 4u);
   const auto rs_k = raw_rsrc(a.K + a.k_off[Q], (uint32_t)a.Vt * (uint32_t)ks * 4u);
@@ -563,7 +573,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       for (int m = 0; m < NI; ++m) acc = fmaf(Kt[k][m], u[m], acc);
       s[k] = acc;
     }
-    row_quot<B, P, L>(s, c, o);
+    row_quot<B, NE, L>(s, c, o);
     x_side(o, xo);
   };
   // After an update: gather the scalings, count the iteration, and run the
@@ -684,15 +694,19 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     s[k] = sk;
     tt[k] = t;
   }
-  row_quot<B, P, L>(s, c, o);
+  row_quot<B, NE, L>(s, c, o);
   double acc = 0.0;
 #pragma unroll
   for (int k = 0; k < P; ++k) acc += (double)(o[k] * tt[k]);
   if (a.duals) {
+    // The offsets are formed again from lane ids hidden from the optimiser,
+    // so the loop above does not keep the gather's offsets alive for this.
+    int ea_d = ea, g_d = g;
+    asm volatile("" : "+v"(ea_d), "+v"(g_d));
     float* dq = a.duals + (int64_t)Q * a.nnz + db;
 #pragma unroll
     for (int k = 0; k < P; ++k) {
-      const uint32_t e = row_off(k) / 4u;
+      const uint32_t e = row_off_from(ea_d, g_d, k) / 4u;
       if (b == 0 && e < (uint32_t)n) dq[e] = __logf(o[k]);
     }
   }

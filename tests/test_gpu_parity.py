@@ -463,11 +463,16 @@ def test_long_documents_routed_per_document(P, dg, oracle):
         wt, itt, rt = e.result()
         e.run(-10.0, max_iter=15, log_domain=True)
         wl = e.result()[0]
+        # Steady state of repeated runs: one summarized run of the timed
+        # configuration re-weights the launch plan with the measured
+        # iterations (the log-domain run above planned other shapes), then the
+        # timed runs reuse that plan; the same for the corpus without them.
         kw = dict(max_iter=20000, tol=1e-4, accel=True, summarize=False)
+        e.run(-10.0, **dict(kw, summarize=True))
         t_long = np.median([e.run(-10.0, **kw)["ms_total"] for _ in range(4)])
     with engine(P, base) as e:
         e.set_queries(qs.q_ptr, qs.words, qs.weights)
-        e.run(-10.0, **kw)
+        e.run(-10.0, **dict(kw, summarize=True))
         t_base = np.median([e.run(-10.0, **kw)["ms_total"] for _ in range(4)])
     sub = 40  # the 6 long documents and 34 others, for 8 of the queries
     b = int(corpus.doc_ptr[sub])
