@@ -36,9 +36,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
+# Units whose matrix-core kernels keep their accumulators in VGPRs (the
+# ReLU and output work reads them directly, with no AGPR copies).
+MFMA_VGPR_UNITS = ("wmd_mlp.hip",)
+
+
 def compile_command(src: str, obj: str) -> list[str]:
+    extra = (["-mllvm", "-amdgpu-mfma-vgpr-form"]
+             if os.path.basename(src) in MFMA_VGPR_UNITS else [])
     return [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", *extra,
             "-I", os.path.join(REPO, "include"), "-c", src, "-o", obj]
 
 

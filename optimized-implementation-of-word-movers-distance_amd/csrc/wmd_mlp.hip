@@ -42,6 +42,11 @@ __device__ __forceinline__ float relu(float x) { return fmaxf(x, 0.f); }
 
 // numpy.interp of x over ascending knots xc[0..q) with values yv: clamped
 // outside the knots, linear between the bracketing pair.
+// Linear interpolation between knots lo and lo + 1 (v_rcp_f32 quotient, 1 ulp).
+__device__ __forceinline__ float knot_lerp(const float* xc, const float* yv, int lo, float x) {
+  return yv[lo] + (x - xc[lo]) * (yv[lo + 1] - yv[lo]) * __builtin_amdgcn_rcpf(xc[lo + 1] - xc[lo]);
+}
+
 __device__ __forceinline__ float interp_knots(const float* xc, const float* yv, int q, float x) {
   if (x <= xc[0]) return yv[0];
   if (x >= xc[q - 1]) return yv[q - 1];
@@ -50,7 +55,7 @@ __device__ __forceinline__ float interp_knots(const float* xc, const float* yv, 
     const int mid = (lo + hi) >> 1;
     if (xc[mid] <= x) lo = mid; else hi = mid;
   }
-  return yv[lo] + (x - xc[lo]) * (yv[hi] - yv[lo]) / (xc[hi] - xc[lo]);
+  return knot_lerp(xc, yv, lo, x);
 }
 
 // interp_knots with the bracketing search started from a bucket table:
@@ -66,8 +71,7 @@ __device__ __forceinline__ float interp_bucketed(const float* xc, const float* y
   const int b = min(max((int)(x * (float)kKnotBuckets) - 1, 0), kKnotBuckets);
   int lo = bk[b];
   while (xc[lo + 1] <= x) ++lo;  // ends below q - 1: x < xc[q - 1]
-  const int hi = lo + 1;
-  return yv[lo] + (x - xc[lo]) * (yv[hi] - yv[lo]) / (xc[hi] - xc[lo]);
+  return knot_lerp(xc, yv, lo, x);
 }
 
 // Fill bk (kKnotBuckets + 1 entries) for knots xc[0 .. q); block-wide.
@@ -233,14 +237,16 @@ __global__ __launch_bounds__(256) void k_query_feats(const int32_t* q_ptr, const
 // consecutive nonzeros of one query (blockIdx.y); the query's sorted norm
 // table sits in LDS for the quantile lookups.  Each wave takes 64 samples at
 // a time: lane l computes sample l's 15 standardised features, which pass
-// through LDS (FP16, 16 per sample, the last 0) into the B-operand layout of
+// through LDS (FP16, 16 per sample, the last 1, the input of b1) into the B-operand layout of
 // v_mfma_f32_16x16x16_f16 (lane l holds rows 4 (l / 16) .. +3 of column
 // l % 16).  The hidden layers are computed transposed, H^T = W X^T (units
 // along M, samples along N), so a layer's accumulator tile, which has the
 // same lane layout, is the next layer's B operand after ReLU and conversion:
-//   layer 1: 2 unit tiles x 4 sample tiles, K = 16 (the features), b1 as C
-//   layer 2: 2 x 4 tiles, K = 32 (two steps), b2 as C
-//   output:  each lane's w3-weighted sum over its 4 units, summed over the
+//   layer 1: 2 unit tiles x 4 sample tiles, K = 16 (the features and 1);
+//            ReLU as packed FP16 maxima after the conversion
+//   layer 2: 2 x 4 tiles, K = 32 (two steps), b2 as C, the same ReLU
+//   output:  each lane's w3-weighted sum over its 4 FP16 units (packed FP16
+//            dot products, FP32 accumulation, w3 in FP16), summed over the
 //            four 16-lane groups (permlane swaps), + b3
 // With export set the kernel writes the 15 raw features instead.
 constexpr int kInferChunks = 8;
@@ -248,13 +254,15 @@ constexpr int kStageRow = 20;  // FP16 per staged sample (16 + 4 padding: 8-byte
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// ReLU of an accumulator tile in FP16: two packed conversions, then packed
+// maxima with 0 (fp16(max(d, 0)) = max(fp16(d), 0), rounding is monotonic).
 __device__ __forceinline__ half4_t relu_h4(float4_t d) {
-  half4_t h;
-  h[0] = (_Float16)relu(d[0]);
-  h[1] = (_Float16)relu(d[1]);
-  h[2] = (_Float16)relu(d[2]);
-  h[3] = (_Float16)relu(d[3]);
-  return h;
+  const half2_t z = {(_Float16)0.f, (_Float16)0.f};
+  const half2_t lo = __builtin_elementwise_max(half2_t{(_Float16)d[0], (_Float16)d[1]}, z);
+  const half2_t hi = __builtin_elementwise_max(half2_t{(_Float16)d[2], (_Float16)d[3]}, z);
+  return half4_t{lo[0], lo[1], hi[0], hi[1]};
 }
 
 template <bool EXPORT>
@@ -290,25 +298,30 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lg = lane >> 4;
     // This lane's operand fragments: A = weight rows (unit lr of each tile,
-    // inputs 4 lg .. +3 of each K step), C = b1 / b2 and the w3 factors of
-    // units 4 lg .. +3 of each tile.
+    // inputs 4 lg .. +3 of each K step; layer 1's input 15 is a constant 1
+    // whose weight is b1), C = b2 and the w3 factors of units 4 lg .. +3 of
+    // each tile.
     half4_t A1[2], A2[2][2];
-    float4_t C1[2], C2[2], W3[2];
+    float4_t C2[2];
+    half2_t W3[2][2];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
       const int unit = mt * 16 + lr;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int k = 4 * lg + i;
-        A1[mt][i] = (_Float16)(k < F ? p[kWarmW1 + unit * F + k] : 0.f);
+        A1[mt][i] = (_Float16)(k < F ? p[kWarmW1 + unit * F + k] : p[kWarmB1 + unit]);
         A2[mt][0][i] = (_Float16)p[kWarmW2 + unit * H + k];
         A2[mt][1][i] = (_Float16)p[kWarmW2 + unit * H + 16 + k];
-        C1[mt][i] = p[kWarmB1 + mt * 16 + 4 * lg + i];
         C2[mt][i] = p[kWarmB2 + mt * 16 + 4 * lg + i];
-        W3[mt][i] = p[kWarmW3 + mt * 16 + 4 * lg + i];
+        W3[mt][i >> 1][i & 1] = (_Float16)p[kWarmW3 + mt * 16 + 4 * lg + i];
       }
     }
     const float b3 = p[kWarmB3];
+    // standardisation z = (x - xm) xs as x xs + zoff, zoff = -xm xs
+    float zoff[F];
+#pragma unroll
+    for (int k = 0; k < F; ++k) zoff[k] = -p[kWarmXm + k] * p[kWarmXs + k];
     _Float16* st = stage[wv];
     // Every wave runs every chunk (the barriers stay block-uniform); lanes
     // past the last nonzero compute on the last one and store nothing.
@@ -320,7 +333,7 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
       half4_t zh[4];
 #pragma unroll
       for (int k = 0; k < 16; ++k)
-        zh[k >> 2][k & 3] = (_Float16)(k < F ? (x[k] - p[kWarmXm + k]) * p[kWarmXs + k] : 0.f);
+        zh[k >> 2][k & 3] = (_Float16)(k < F ? fmaf(x[k], p[kWarmXs + k], zoff[k]) : 1.f);
 #pragma unroll
       for (int v = 0; v < 4; ++v) *reinterpret_cast<half4_t*>(st + lane * kStageRow + 4 * v) = zh[v];
       __syncthreads();
@@ -334,7 +347,8 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
       for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
-          H1[mt][nt] = relu_h4(__builtin_amdgcn_mfma_f32_16x16x16f16(A1[mt], Bx[nt], C1[mt], 0, 0, 0));
+          H1[mt][nt] = relu_h4(__builtin_amdgcn_mfma_f32_16x16x16f16(A1[mt], Bx[nt], float4_t{},
+                                                                     0, 0, 0));
       float yp[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int mt = 0; mt < 2; ++mt) {
@@ -342,8 +356,11 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
         for (int nt = 0; nt < 4; ++nt) {
           float4_t d = __builtin_amdgcn_mfma_f32_16x16x16f16(A2[mt][0], H1[0][nt], C2[mt], 0, 0, 0);
           d = __builtin_amdgcn_mfma_f32_16x16x16f16(A2[mt][1], H1[1][nt], d, 0, 0, 0);
-#pragma unroll
-          for (int i = 0; i < 4; ++i) yp[nt] = fmaf(W3[mt][i], relu(d[i]), yp[nt]);
+          // the output layer on the FP16 layer-2 units: packed dot products
+          // with FP32 accumulation
+          const half4_t h2 = relu_h4(d);
+          yp[nt] = __builtin_amdgcn_fdot2(half2_t{h2[0], h2[1]}, W3[mt][0], yp[nt], false);
+          yp[nt] = __builtin_amdgcn_fdot2(half2_t{h2[2], h2[3]}, W3[mt][1], yp[nt], false);
         }
       }
 #pragma unroll

@@ -77,8 +77,9 @@ def _fp16(a):
 def test_inference_matches_oracle_forward(P, dg):
     """The matrix-core inference kernel against the FP64 forward of
     oracle/mlp_oracle.py with the kernel's FP16 roundings applied at the same
-    points (standardised inputs, W1, layer-1 outputs after ReLU, W2; biases,
-    w3 and every accumulation in full precision): agreement to FP32
+    points (standardised inputs, W1 and b1 (b1 enters as the weight of a
+    constant input), layer-1 and layer-2 outputs after ReLU, W2, w3; b2, b3
+    and every accumulation in full precision): agreement to FP32
     accumulation error pins the operand layouts and the wiring, where a
     transposed tile or a wrong lane group gives O(1) errors.  Against the
     exact FP64 forward the difference is the FP16 rounding itself."""
@@ -103,8 +104,9 @@ def test_inference_matches_oracle_forward(P, dg):
         got = e.warm_predictions(corpus.nnz).reshape(-1).astype(np.float64)
     xm, xs, W1, b1, W2, b2, w3, b3 = mo.unpack(p)
     z = (X.astype(np.float32) - p[L["xm"]:L["xs"]]) * p[L["xs"]:L["W1"]]
-    h1 = _fp16(np.maximum(_fp16(z) @ _fp16(W1).T + b1, 0.0))
-    emul = np.maximum(h1 @ _fp16(W2).T + b2, 0.0) @ w3 + b3
+    h1 = _fp16(np.maximum(_fp16(z) @ _fp16(W1).T + _fp16(b1), 0.0))
+    h2 = _fp16(np.maximum(h1 @ _fp16(W2).T + b2, 0.0))
+    emul = h2 @ _fp16(w3) + b3
     exact, _ = mo.forward(X, p)
     scale = max(1.0, float(np.abs(exact).max()))
     err_e = float(np.abs(got - emul).max()) / scale
