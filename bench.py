@@ -67,7 +67,7 @@ def sample_queries(qs):
     return [int(o[0]), int(o[len(o) // 2]), int(o[-1])]
 
 
-def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=30):
+def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=40, pairs_per_batch=128):
     """Train the warm-start network on converged duals of n_queries x n_docs
     seeded synthetic pairs over the same embeddings (not part of a step)."""
     tc, tq = dg.training_set(vecs, n_queries=n_queries, n_docs=n_docs, seed=101)
@@ -78,12 +78,13 @@ def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=30):
         e.run(lamda, max_iter=20000, tol=1e-6, export_duals=True, accel=True, summarize=False)
         t1 = time.perf_counter()
         e.mlp_set(P.mlp_init(seed=0))
-        losses = e.mlp_train(epochs=epochs, seed=0)
+        losses = e.mlp_train(epochs=epochs, seed=0, pairs_per_batch=pairs_per_batch)
         params = e.mlp_get()
     dt, dtt = time.perf_counter() - t0, time.perf_counter() - t1
     log(f"warm-start network trained on {tq.n * tc.N} pairs in {dtt:.2f}s ({dt:.1f}s with the "
         f"target solve), epoch losses {np.array2string(losses, precision=3)}")
     return {"params": params, "pairs": int(tq.n * tc.N), "epochs": epochs,
+            "pairs_per_batch": pairs_per_batch,
             "epoch_loss": [float(x) for x in losses], "train_seconds": round(dtt, 3),
             "seconds_with_targets": round(dt, 2)}
 
@@ -376,8 +377,8 @@ def main() -> int:
     ap.add_argument("--warm", choices=("uniform", "learned"), default="learned")
     ap.add_argument("--accel", choices=("none", "anderson"), default="anderson",
                     help="u-update acceleration")
-    ap.add_argument("--train-queries", type=int, default=1000)
-    ap.add_argument("--train-docs", type=int, default=20)
+    ap.add_argument("--train-queries", type=int, default=2000)
+    ap.add_argument("--train-docs", type=int, default=40)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="smallest-fitting tile shapes instead of autotuned ones")

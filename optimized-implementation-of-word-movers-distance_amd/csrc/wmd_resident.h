@@ -593,8 +593,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   };
 
   // The seeded half-step: v from the prediction (0 for rows with no weight),
-  // then the update with the seeded flag (no stop test, no acceleration
-  // history).
+  // then a plain step u = r / x (no stop test, no acceleration history).
+  // The accelerated rule starts from l = log2 u; it re-centres l at its
+  // first step (the plan ignores the scaling gauge), so l is not centred
+  // here.
   if (warm_on && a.max_iter > 0) {
     const auto rs_w = raw_rsrc(a.warm + (int64_t)Q * a.nnz + db, (uint32_t)n * 4u);
     float o[P];
@@ -607,11 +609,11 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     }
     float xo[NO];
     x_side(o, xo);
-    bool stop, to_plain;
-    if (a.accel)
-      accel_update<B>(a, st, xo, ro, lro, uo, true, inv_q, it, stop, to_plain);
-    else
-      plain_update<B>(a, st, xo, ro, lro, uo, true, stop);
+#pragma unroll
+    for (int k = 0; k < NO; ++k) {
+      uo[k] = fdiv(ro[k], xo[k]);  // padded slots: 0 / (a few kFltMin) = 0
+      st.l[k] = (ro[k] > 0.f) ? lro[k] - __builtin_amdgcn_logf(xo[k]) : 0.f;
+    }
     float ug[NIP];
     rows_gather<NIP>(uo, ug);
 #pragma unroll
