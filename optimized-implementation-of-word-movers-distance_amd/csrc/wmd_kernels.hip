@@ -33,6 +33,14 @@ namespace {
 // ~1e-7 (|a|^2 + |b|^2)).
 constexpr double kNearTau = 0.02;
 
+// Row-rescaled kernel entry from a distance d and its row minimum m:
+// exp(lamda (d - m)) as 2^(l2 (d - m)) with l2 = lamda log2(e) (one v_exp_f32;
+// the per-query and the shared distance stages both use it, so their outputs
+// stay bitwise equal).
+__device__ __forceinline__ float rescaled_kernel(float l2, float d, float m) {
+  return __builtin_amdgcn_exp2f(l2 * (d - m));
+}
+
 // ---------------------------------------------------------------------------
 // cdist + exp epilogue.  Block: 64 touched words x one query, query words in
 // chunks of 32, embedding components staged through LDS 32 at a time.  Thread
@@ -148,7 +156,7 @@ __global__ __launch_bounds__(256) void k_cdist(CdistArgs a) {
           if (i < q) {
             // Multi-chunk rows re-read the raw distance this thread stored.
             const float dd = (nqc == 1) ? dist[x][y] : Kq[(int64_t)t * ks + i];
-            kv = a.log_out ? l2 * (dd - m[x]) : expf(a.lamda * (dd - m[x]));
+            kv = a.log_out ? l2 * (dd - m[x]) : rescaled_kernel(l2, dd, m[x]);
           }
           Kq[(int64_t)t * ks + i] = kv;
         }
@@ -317,7 +325,7 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
         if (jo < q) {
           const float dd = Kq[(int64_t)t * ks + jo];
           const float m = __uint_as_float(smin[wy * 32 + i]);
-          kv = a.log_out ? l2 * (dd - m) : expf(a.lamda * (dd - m));
+          kv = a.log_out ? l2 * (dd - m) : rescaled_kernel(l2, dd, m);
         }
         Kq[(int64_t)t * ks + jo] = kv;
       }
@@ -427,16 +435,17 @@ __global__ __launch_bounds__(256) void k_cdist_mfma_raw(MfmaCdistArgs a) {
 // epilogue as the per-query kernels (bitwise-equal outputs).
 constexpr int kShareChunks = kShareMaxQ / 64;
 
-// One query's rows of the block; NC = its 64-word chunks (compile-time, so
-// the chunk values stay in registers and short queries do no padded work).
-template <int NC>
+// One query's rows of the block; NC = its 64-word chunks and LW the lanes
+// its row minimum spans (compile-time, so the chunk values stay in registers
+// and short queries do no padded work).  Slots past q read +inf, whose
+// kernel entry exp2(l2 (inf - m)) is the padding 0 (l2 < 0).
+template <int NC, int LW, bool LOG>
 __device__ __forceinline__ void share_query(const ShareArgs& a, const float* rows, int t0, int nr,
                                             int Q, int lane) {
   const int qb = a.q_ptr[Q];
   const int q = a.q_ptr[Q + 1] - qb;
   const int ks = a.k_stride[Q];
   const float l2 = a.lamda * 1.4426950408889634f;
-  const float pad = a.log_out ? kLogZero : 0.f;
   int sl[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
@@ -453,17 +462,38 @@ __device__ __forceinline__ void share_query(const ShareArgs& a, const float* row
       d[c] = sl[c] >= 0 ? rows[r * a.Sdp + sl[c]] : INFINITY;
       m = fminf(m, d[c]);
     }
-    m = bfly_red<1, 32, kMin>(m);
+    m = group_red<LW, kMin>(m);
+    float* row = Kq + (int64_t)t * ks;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int j = c * 64 + lane;
-      if (j < ks) {
-        float kv = pad;
-        if (j < q) kv = a.log_out ? l2 * (d[c] - m) : expf(a.lamda * (d[c] - m));
-        Kq[(int64_t)t * ks + j] = kv;
-      }
+      float kv;
+      if constexpr (LOG) kv = j < q ? l2 * (d[c] - m) : kLogZero;
+      else kv = rescaled_kernel(l2, d[c], m);
+      if (j < ks) row[j] = kv;
     }
     if (lane == 0) a.mrow[(int64_t)Q * a.Vt + t] = m;
+  }
+}
+
+template <bool LOG>
+__device__ __forceinline__ void share_query_any(const ShareArgs& a, const float* rows, int t0,
+                                                int nr, int Q, int lane) {
+  static_assert(kShareChunks == 8, "one case per chunk count");
+  const int ks = a.k_stride[Q];
+  switch ((ks + 63) >> 6) {
+    case 1:
+      if (ks <= 16) share_query<1, 16, LOG>(a, rows, t0, nr, Q, lane);
+      else if (ks <= 32) share_query<1, 32, LOG>(a, rows, t0, nr, Q, lane);
+      else share_query<1, 64, LOG>(a, rows, t0, nr, Q, lane);
+      break;
+    case 2: share_query<2, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    case 3: share_query<3, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    case 4: share_query<4, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    case 5: share_query<5, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    case 6: share_query<6, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    case 7: share_query<7, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    default: share_query<8, 64, LOG>(a, rows, t0, nr, Q, lane); break;
   }
 }
 
@@ -484,18 +514,10 @@ __global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  static_assert(kShareChunks == 8, "one case per chunk count");
-  for (int Q = w; Q < a.nq; Q += 4) {
-    switch ((a.k_stride[Q] + 63) >> 6) {
-      case 1: share_query<1>(a, rows, t0, nr, Q, lane); break;
-      case 2: share_query<2>(a, rows, t0, nr, Q, lane); break;
-      case 3: share_query<3>(a, rows, t0, nr, Q, lane); break;
-      case 4: share_query<4>(a, rows, t0, nr, Q, lane); break;
-      case 5: share_query<5>(a, rows, t0, nr, Q, lane); break;
-      case 6: share_query<6>(a, rows, t0, nr, Q, lane); break;
-      case 7: share_query<7>(a, rows, t0, nr, Q, lane); break;
-      default: share_query<8>(a, rows, t0, nr, Q, lane); break;
-    }
+  if (a.log_out) {
+    for (int Q = w; Q < a.nq; Q += 4) share_query_any<true>(a, rows, t0, nr, Q, lane);
+  } else {
+    for (int Q = w; Q < a.nq; Q += 4) share_query_any<false>(a, rows, t0, nr, Q, lane);
   }
 }
 
