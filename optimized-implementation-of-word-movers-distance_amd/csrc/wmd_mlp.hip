@@ -489,10 +489,22 @@ __global__ __launch_bounds__(kTrainThreads) void k_warm_train(WarmTrainArgs t) {
   }
   sd = block_sum_d(sd, red);
   const double mean = s1 > s0 ? sd / (double)(s1 - s0) : 0.0;
-  // Pass 2: per chunk of samples, the per-sample backward vectors go to LDS
-  // and every thread sums its parameters' gradients over the chunk.
-  constexpr int kPer = (kWarmParams + kTrainThreads - 1) / kTrainThreads;
-  float acc[kPer] = {};
+  // Pass 2: per chunk of samples, the per-sample backward vectors go to LDS.
+  // The weight gradients are GEMMs over the chunk's samples, G_W2 = dZ2^T H1
+  // (32 x 32) and G_W1 = dZ1^T Z (32 x 15), on the matrix cores:
+  // v_mfma_f32_16x16x4_f32 (FP32 operands, FP32 accumulation), K = 4 samples
+  // per instruction, operands read from the LDS rows in the instruction's
+  // layout (lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; the
+  // accumulator holds rows 4 (l / 16) .. +3 of column l % 16).  Rows past the
+  // chunk's last sample enter as 0.  The 97 vector gradients (b1, b2, w3, b3)
+  // are column sums on the vector ALU, two per thread.
+  static_assert(kTrainThreads == 2 * H && H == 32 && F <= 16, "one wave, two 16-unit tiles");
+  float4_t gw2[2][2] = {}, gw1[2] = {};
+  float gb = 0.f, gw = 0.f;
+  const int lane = threadIdx.x, li = lane & 15, lk = lane >> 4;
+  // b1 / b2 and w3 / b3 (thread H) columns of this thread
+  const int cb = lane < H ? kRowDz1 + lane : kRowDz2 + lane - H;
+  const int cw = lane < H ? kRowH2 + lane : kRowDy;
   double loss = 0.0;
   for (int64_t c0 = s0; c0 < s1; c0 += kTrainThreads) {
     const int cnt = (s1 - c0 < kTrainThreads) ? (int)(s1 - c0) : kTrainThreads;
@@ -526,42 +538,47 @@ __global__ __launch_bounds__(kTrainThreads) void k_warm_train(WarmTrainArgs t) {
       r[kRowDy] = dy;
     }
     __syncthreads();
+    for (int s4 = 0; s4 < cnt; s4 += 4) {
+      const int rr = s4 + lk;
+      const bool live = rr < cnt;
+      const float* r = rows + rr * kRow;
+      float a2[2], a1[2], b2[2];
 #pragma unroll
-    for (int m = 0; m < kPer; ++m) {
-      const int pi = threadIdx.x + m * kTrainThreads;
-      int ca = -1, cb = -1;  // gradient = sum over rows of row[ca] * (cb < 0 ? 1 : row[cb])
-      if (pi >= kWarmW1 && pi < kWarmB1) {
-        const int h = (pi - kWarmW1) / F, k = (pi - kWarmW1) % F;
-        ca = kRowDz1 + h; cb = kRowZ + k;
-      } else if (pi >= kWarmB1 && pi < kWarmW2) {
-        ca = kRowDz1 + (pi - kWarmB1);
-      } else if (pi >= kWarmW2 && pi < kWarmB2) {
-        const int o = (pi - kWarmW2) / H, h = (pi - kWarmW2) % H;
-        ca = kRowDz2 + o; cb = kRowH1 + h;
-      } else if (pi >= kWarmB2 && pi < kWarmW3) {
-        ca = kRowDz2 + (pi - kWarmB2);
-      } else if (pi >= kWarmW3 && pi < kWarmB3) {
-        ca = kRowH2 + (pi - kWarmW3);
-      } else if (pi == kWarmB3) {
-        ca = kRowDy;
+      for (int m = 0; m < 2; ++m) {
+        a2[m] = live ? r[kRowDz2 + 16 * m + li] : 0.f;
+        a1[m] = live ? r[kRowDz1 + 16 * m + li] : 0.f;
+        b2[m] = live ? r[kRowH1 + 16 * m + li] : 0.f;
       }
-      if (ca >= 0) {
-        float v = acc[m];
-        for (int rr = 0; rr < cnt; ++rr) {
-          const float* r = rows + rr * kRow;
-          v = fmaf(r[ca], cb >= 0 ? r[cb] : 1.f, v);
-        }
-        acc[m] = v;
+      const float b1 = (live && li < F) ? r[kRowZ + li] : 0.f;
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+          gw2[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a2[m], b2[n], gw2[m][n], 0, 0, 0);
+        gw1[m] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[m], b1, gw1[m], 0, 0, 0);
       }
+    }
+    for (int rr = 0; rr < cnt; ++rr) {
+      gb += rows[rr * kRow + cb];
+      if (lane <= H) gw += rows[rr * kRow + cw];
     }
     __syncthreads();
   }
   float* out = t.gpart + (int64_t)g * kWarmParams;
+  for (int pi = lane; pi < kWarmW1; pi += kTrainThreads) out[pi] = 0.f;  // standardisation
 #pragma unroll
-  for (int m = 0; m < kPer; ++m) {
-    const int pi = threadIdx.x + m * kTrainThreads;
-    if (pi < kWarmParams) out[pi] = acc[m];  // 0 for the standardisation slots
+  for (int m = 0; m < 2; ++m) {
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int o = 16 * m + 4 * lk + v;
+#pragma unroll
+      for (int n = 0; n < 2; ++n) out[kWarmW2 + o * H + 16 * n + li] = gw2[m][n][v];
+      if (li < F) out[kWarmW1 + o * F + li] = gw1[m][v];
+    }
   }
+  out[lane < H ? kWarmB1 + lane : kWarmB2 + lane - H] = gb;
+  if (lane < H) out[kWarmW3 + lane] = gw;
+  if (lane == H) out[kWarmB3] = gw;
   loss = block_sum_d(loss, red);
   if (threadIdx.x == 0) t.lpart[g] = loss;
 }
