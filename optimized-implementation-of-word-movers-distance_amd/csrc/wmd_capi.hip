@@ -113,6 +113,15 @@ struct ResidentLaunch {
 // Per-wave instruction estimate of one iteration of a pair in shape s with
 // NE document-word slots per lane: the two tile products, the B-lane
 // butterfly and quotient per slot row, and the per-slot update.
+// Tile rows NE of shape s on document class cls (the kernel's template
+// argument; equal for consecutive classes when the shape has more document
+// lanes than a class is wide).
+int tile_rows(int s, int cls) {
+  const wmd::TileShape& t = wmd::kShapes[s];
+  const int A = 64 * t.W / t.B;
+  return (wmd::kDocClassWidth * (cls + 1) + A - 1) / A;
+}
+
 double iter_cost(int s, int cls) {
   const wmd::TileShape& t = wmd::kShapes[s];
   const int A = 64 * t.W / t.B;
@@ -364,6 +373,10 @@ int shape_for(const wmd_context* c, int cls, int q, bool log_domain, int forced)
 // Until a run has measured them, every pair is weighted with this many
 // iterations.
 constexpr double kPriorIters = 20.0;
+// Largest share of a plan's estimated work one launch may carry, and the
+// largest share up to which consecutive classes merge into one launch.
+constexpr double kMaxLaunchShare = 0.02;
+constexpr double kMergeShare = 0.004;
 
 void add_launch(std::vector<ResidentLaunch>* plan, std::vector<int32_t>* flat,
                 const std::vector<int32_t>& queries, int s, int cls, int64_t d0, int64_t d1) {
@@ -386,16 +399,65 @@ int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
                   std::vector<ResidentLaunch>* plan) {
   std::vector<int32_t> flat;
   plan->clear();
+  // groups[cls][s]: the queries that run shape s on document class cls.
+  std::vector<std::vector<std::vector<int32_t>>> groups(
+      wmd::kNumDocClasses, std::vector<std::vector<int32_t>>(wmd::kNumShapes));
   for (int cls = 0; cls < wmd::kNumDocClasses; ++cls) {
-    const int64_t d0 = c->h_class_off[cls], d1 = c->h_class_off[cls + 1];
-    if (d1 == d0) continue;
-    std::vector<std::vector<int32_t>> groups(wmd::kNumShapes);
+    if (c->h_class_off[cls + 1] == c->h_class_off[cls]) continue;
     for (int Q = 0; Q < c->nq; ++Q) {
       const int q = c->h_q_ptr[Q + 1] - c->h_q_ptr[Q];
-      groups[shape_for(c, cls, q, log_domain, forced_shape)].push_back(Q);
+      groups[cls][shape_for(c, cls, q, log_domain, forced_shape)].push_back(Q);
     }
-    for (int s = 0; s < wmd::kNumShapes; ++s) add_launch(plan, &flat, groups[s], s, cls, d0, d1);
   }
+  // Consecutive classes whose tiles have the same row count NE and whose
+  // groups hold the same queries form one launch (their documents are
+  // contiguous in doc_perm) while the merged launch stays small
+  // (kMergeShare of the estimated work): a small launch's time is its
+  // slowest pair's latency, so fewer of them shorten the lanes' tails, while
+  // large launches stay apart for the lane balance.  The launch takes the
+  // last class's NE, which is the same.
+  auto group_work = [&](int cls, int s) {
+    return (double)groups[cls][s].size() *
+           (double)(c->h_class_off[cls + 1] - c->h_class_off[cls]) * kPriorIters *
+           iter_cost(s, cls);
+  };
+  double total_est = 0.0;
+  for (int cls = 0; cls < wmd::kNumDocClasses; ++cls)
+    for (int s = 0; s < wmd::kNumShapes; ++s) total_est += group_work(cls, s);
+  for (int s = 0; s < wmd::kNumShapes; ++s) {
+    for (int cls = 0; cls < wmd::kNumDocClasses;) {
+      int last = cls;
+      if (!groups[cls][s].empty()) {
+        double w = group_work(cls, s);
+        while (last + 1 < wmd::kNumDocClasses && tile_rows(s, last + 1) == tile_rows(s, cls) &&
+               groups[last + 1][s] == groups[cls][s] &&
+               w + group_work(last + 1, s) <= kMergeShare * total_est) {
+          ++last;
+          w += group_work(last, s);
+        }
+      }
+      add_launch(plan, &flat, groups[cls][s], s, last, c->h_class_off[cls],
+                 c->h_class_off[last + 1]);
+      cls = last + 1;
+    }
+  }
+  // A launch above kMaxLaunchShare of the estimated work is split into
+  // document ranges (same queries), so the four lanes can be balanced.
+  double total = 0.0;
+  for (const ResidentLaunch& l : *plan) total += l.work;
+  std::vector<ResidentLaunch> split;
+  for (const ResidentLaunch& l : *plan) {
+    const int64_t m = std::min<int64_t>(
+        l.d1 - l.d0, (int64_t)std::ceil(l.work / (kMaxLaunchShare * total)));
+    for (int64_t k = 0; k < std::max<int64_t>(m, 1); ++k) {
+      ResidentLaunch p = l;
+      p.d0 = l.d0 + (l.d1 - l.d0) * k / std::max<int64_t>(m, 1);
+      p.d1 = l.d0 + (l.d1 - l.d0) * (k + 1) / std::max<int64_t>(m, 1);
+      p.work = l.work * (double)(p.d1 - p.d0) / (double)(l.d1 - l.d0);
+      if (p.d1 > p.d0) split.push_back(p);
+    }
+  }
+  plan->swap(split);
   if (c->n_long > 0) {
     // Every query against the documents beyond the tiles, in one launch.
     ResidentLaunch l{kLongLaunch, -1, flat.size(), (int64_t)c->nq, 0, c->n_long, 0.0};
