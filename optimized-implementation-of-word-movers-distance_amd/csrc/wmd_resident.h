@@ -209,6 +209,8 @@ __device__ __forceinline__ void update_state_reset(UpdateState<N>& st, float ome
 // One accelerated u-update of the N slots this lane updates, from their
 // x = K o, r and log2 r; returns the row-marginal residual
 // sum_i |u_i x_i - r_i| of the current plan (every lane the same value).
+// reduce(v) replaces per-lane partials by their sums over every query slot
+// of the pair (the same bits in every lane of every wave).
 //   g = centre(log2 r - log2 x), f = g - l,
 //   l <- l + f - gamma (dl + df), gamma = (df . f) / (df . df),
 //   dl, df the changes since the previous accepted step; a step whose
@@ -221,11 +223,12 @@ __device__ __forceinline__ void update_state_reset(UpdateState<N>& st, float ome
 // point's plain update (2^(log2 r - log2 x) = r / x) and the pair continues
 // with plain steps.  Both outcomes of the acceptance test are formed and
 // blended, so the state never differs between paths.
-template <int B, int N>
+template <int N, typename Red>
 __device__ __forceinline__ float accel_update(const SolveArgs& a, UpdateState<N>& st,
                                               const float (&x)[N], const float (&r)[N],
                                               const float (&lr)[N], float (&u)[N], bool seeded,
-                                              float inv_q, int it, bool& stop, bool& to_plain) {
+                                              float inv_q, int it, bool& stop, bool& to_plain,
+                                              Red reduce) {
   float g[N];
   float red[2] = {0.f, 0.f};
 #pragma unroll
@@ -234,7 +237,7 @@ __device__ __forceinline__ float accel_update(const SolveArgs& a, UpdateState<N>
     g[n] = (r[n] > 0.f) ? lr[n] - __builtin_amdgcn_logf(x[n]) : 0.f;
     red[1] += g[n];
   }
-  slot_sum_n<B>(red);
+  reduce(red);
   const float res = wave_uniform(red[0]);
   stop = !seeded && a.tol > 0.f && res < a.tol;
   // Progress guard, due at the end of every 64-iteration window and taken at
@@ -262,7 +265,7 @@ __device__ __forceinline__ float accel_update(const SolveArgs& a, UpdateState<N>
     d[0] = fmaf(df[n], df[n], d[0]);
     d[1] = fmaf(df[n], f[n], d[1]);
   }
-  slot_sum_n<B>(d);
+  reduce(d);
   const float gam = (ok && st.prev) ? fdiv(wave_uniform(d[1]),
                                            fmaf(wave_uniform(d[0]), 1.f + 1e-6f, 1e-30f))
                                     : 0.f;
@@ -295,11 +298,11 @@ __device__ __forceinline__ float accel_update(const SolveArgs& a, UpdateState<N>
 //   plain      u = r / x
 //   omega > 1  u <- u (r / (u x))^omega = u exp2(omega (log2 r - log2 (u x)));
 //              a rise in the residual switches the pair to plain steps
-template <int B, int N>
+template <int N, typename Red>
 __device__ __forceinline__ float plain_update(const SolveArgs& a, UpdateState<N>& st,
                                               const float (&x)[N], const float (&r)[N],
                                               const float (&lr)[N], float (&u)[N], bool seeded,
-                                              bool& stop) {
+                                              bool& stop, Red reduce) {
   float ux[N];
   float part = 0.f;
 #pragma unroll
@@ -308,7 +311,7 @@ __device__ __forceinline__ float plain_update(const SolveArgs& a, UpdateState<N>
     part += fabsf(ux[n] - r[n]);  // padded slots: 0 * x - 0
   }
   float red[1] = {part};
-  slot_sum_n<B>(red);
+  reduce(red);
   const float res = wave_uniform(red[0]);
   if (st.sor && !seeded) {
     if (res > st.res_prev) st.sor = false;
@@ -396,7 +399,15 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   // a multiple of 4): slot k <-> query-word slot own_slot<2>(k, rho).
   constexpr int NIP = (NI + 3) / 4 * 4;
   constexpr int NO = NIP / 4;
-  __shared__ float xsh[(W > 1) ? 2 * W * NO * 64 : 1];
+  // For W > 1 every wave holds the same NO owned slots after its in-wave
+  // scatter; wave w totals and updates slot set w (NU = 1 per wave), the
+  // scalar sums of the update rules add the waves' partials through LDS, and
+  // the updated scalings reach every wave through LDS.
+  static_assert(W == 1 || NO <= W, "one owned slot set per wave");
+  constexpr int NU = (W == 1) ? NO : 1;
+  __shared__ float xsh[(W > 1) ? W * NO * 64 : 1];  // the waves' x partials
+  __shared__ float ush[(W > 1) ? NO * 64 : 1];      // the updated scalings
+  __shared__ float rsh[(W > 1) ? 2 * W * 2 : 1];    // scalar partials, two rounds
   __shared__ double dsh[(W > 1) ? W : 1];
 
   const int lane = threadIdx.x & 63;
@@ -480,19 +491,43 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 #pragma unroll
   for (int m = 0; m < NI; ++m)
     u[m] = (slot_of<B, NI>(b, m) < q) ? (float)q : 0.f;  // x = 1/q, u = 1/x (:188-191)
-  // The lane updates only the NO scalings it owns and gathers the rest; uo,
+  // The lane updates only the NU scalings it owns and gathers the rest; uo,
   // ro, lro are its owned u, r, log2 r (padded slots: r = 0, log2 r = -inf,
-  // which the update rules map to u = 0).
-  float uo[NO], ro[NO], lro[NO];
+  // which the update rules map to u = 0).  W > 1: slot set wv of the wave
+  // (none for wv >= NO: r = 0).
+  float uo[NU], ro[NU], lro[NU];
 #pragma unroll
-  for (int k = 0; k < NO; ++k) {
-    const int m = own_slot<2>(k, rho);
+  for (int k = 0; k < NU; ++k) {
+    const int ko = (W == 1) ? k : wv;
+    const int m = own_slot<2>(ko, rho);
     const int i = slot_of<B, NI>(b, m);
-    const bool live = m < NI && i < q;
+    const bool live = ko < NO && m < NI && i < q;
     ro[k] = live ? a.q_r[qb + i] : 0.f;
     lro[k] = __builtin_amdgcn_logf(ro[k]);
     uo[k] = live ? (float)q : 0.f;
   }
+  // Sum over every query slot of the pair (the update rules' reductions):
+  // the wave's slots, then for W > 1 the waves' partials in wave order.
+  int rround = 0;
+  auto reduce = [&](auto& v) {
+    constexpr int N2 = sizeof(v) / sizeof(float);
+    slot_sum_n<B>(v);
+    if constexpr (W > 1) {
+      float* rb = rsh + (rround++ & 1) * (W * 2);
+      if (lane == 0) {
+#pragma unroll
+        for (int n = 0; n < N2; ++n) rb[wv * 2 + n] = v[n];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int n = 0; n < N2; ++n) {
+        float t = rb[n];
+#pragma unroll
+        for (int w = 1; w < W; ++w) t += rb[w * 2 + n];
+        v[n] = t;
+      }
+    }
+  };
 
   // Learned warm start: the first pass takes v from the predicted log v
   // (one half-step, counted as an iteration) instead of from u = q.  A
@@ -503,7 +538,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   const bool warm_on = a.warm != nullptr;
   bool fell_back = !warm_on;
   float res_win = INFINITY, res_low = INFINITY;
-  UpdateState<NO> st;
+  UpdateState<NU> st;
   update_state_reset(st, a.omega);
   const float inv_q = 1.f / (float)q;  // centring of the accelerated update
   int it = 0;
@@ -515,7 +550,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   // select, while real sums (bounded below by the largest entry of a
   // rescaled row, 1, times a scaling) are unchanged except at exact rounding
   // ties.
-  auto x_side = [&](const float (&o)[P], float (&xo)[NO]) {
+  auto x_side = [&](const float (&o)[P], float (&xo)[NU]) {
     float xs[NIP];
 #pragma unroll
     for (int m = 0; m < NIP; ++m) {
@@ -527,44 +562,46 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       xs[m] = x;
     }
     // x_i for the owned slots: scatter across rows, then the in-row masks.
-    rows_reduce_scatter<NIP>(xs, xo);
-    bfly_sum_n<B, 8>(xo);
-    if constexpr (W > 1) {
-      // The waves of a pair own the same slots (ownership follows the row):
-      // add the waves' sums through LDS, in wave order, so every wave holds
-      // bitwise-equal totals.  Double buffering: the next write to this half
-      // happens two iterations later, after every wave has passed the next
-      // barrier.
-      float* buf = xsh + (it & 1) * (W * NO * 64);
+    float xw[NO];
+    rows_reduce_scatter<NIP>(xs, xw);
+    bfly_sum_n<B, 8>(xw);
+    if constexpr (W == 1) {
 #pragma unroll
-      for (int k = 0; k < NO; ++k) buf[(wv * NO + k) * 64 + lane] = xo[k];
+      for (int k = 0; k < NO; ++k) xo[k] = xw[k];
+    } else {
+      // Wave wv totals slot set wv over the waves' partials, in wave order
+      // (a wave without a slot set takes x = 1 for its r = 0).
+#pragma unroll
+      for (int k = 0; k < NO; ++k) xsh[(wv * NO + k) * 64 + lane] = xw[k];
       __syncthreads();
-      if constexpr (W >= 4) {
-        // Wave wv totals slots wv, wv + W, ... into wave 0's entries (no
-        // other wave touches those slots in between), then every wave reads
-        // the totals: W - 1 adds per slot once instead of in every wave.
-        for (int k = wv; k < NO; k += W) {
-          float x = buf[k * 64 + lane];
+      float x = 1.f;
+      if (wv < NO) {
+        x = xsh[wv * 64 + lane];
 #pragma unroll
-          for (int w = 1; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
-          buf[k * 64 + lane] = x;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < NO; ++k) xo[k] = buf[k * 64 + lane];
-      } else {
-#pragma unroll
-        for (int k = 0; k < NO; ++k) {
-          float x = buf[k * 64 + lane];
-#pragma unroll
-          for (int w = 1; w < W; ++w) x += buf[(w * NO + k) * 64 + lane];
-          xo[k] = x;
-        }
+        for (int w = 1; w < W; ++w) x += xsh[(w * NO + wv) * 64 + lane];
       }
+      xo[0] = x;
     }
   };
+  // Every lane's NI tile scalings from the owners' updated values.
+  auto gather_u = [&]() {
+    float uw[NO];
+    if constexpr (W == 1) {
+#pragma unroll
+      for (int k = 0; k < NO; ++k) uw[k] = uo[k];
+    } else {
+      if (wv < NO) ush[wv * 64 + lane] = uo[0];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < NO; ++k) uw[k] = ush[k * 64 + lane];
+    }
+    float ug[NIP];
+    rows_gather<NIP>(uw, ug);
+#pragma unroll
+    for (int m = 0; m < NI; ++m) u[m] = ug[m];
+  };
   // One sweep from the current u: o = c / (K^T u), then the owned x = K o.
-  auto sweep = [&](float (&xo)[NO]) {
+  auto sweep = [&](float (&xo)[NU]) {
     float s[P], o[P];
 #pragma unroll
     for (int k = 0; k < P; ++k) {
@@ -579,10 +616,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
   // After an update: gather the scalings, count the iteration, and run the
   // stall check of a warm-started pair; true when the pair restarts.
   auto advance = [&]() -> bool {
-    float ug[NIP];
-    rows_gather<NIP>(uo, ug);
-#pragma unroll
-    for (int m = 0; m < NI; ++m) u[m] = ug[m];
+    gather_u();
     ++it;
     res_low = fminf(res_low, res);
     if (!fell_back && (it & 63) == 0) {
@@ -607,17 +641,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
       const float v = __expf(__builtin_amdgcn_fmed3f(o[k], -60.f, 60.f));
       o[k] = mask_select(opaque_mask(c[k] > 0.f), v, 0.f);
     }
-    float xo[NO];
+    float xo[NU];
     x_side(o, xo);
 #pragma unroll
-    for (int k = 0; k < NO; ++k) {
+    for (int k = 0; k < NU; ++k) {
       uo[k] = fdiv(ro[k], xo[k]);  // padded slots: 0 / (a few kFltMin) = 0
       st.l[k] = (ro[k] > 0.f) ? lro[k] - __builtin_amdgcn_logf(xo[k]) : 0.f;
     }
-    float ug[NIP];
-    rows_gather<NIP>(uo, ug);
-#pragma unroll
-    for (int m = 0; m < NI; ++m) u[m] = ug[m];
+    gather_u();
     ++it;
   }
 
@@ -632,10 +663,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     bool plain = !a.accel;
     if (!plain) {
       while (it < a.max_iter) {
-        float xo[NO];
+        float xo[NU];
         sweep(xo);
         bool stop, to_plain;
-        res = accel_update<B>(a, st, xo, ro, lro, uo, false, inv_q, it, stop, to_plain);
+        res = accel_update(a, st, xo, ro, lro, uo, false, inv_q, it, stop, to_plain, reduce);
         restart = advance();
         if (stop) {  // u kept the verified point
           restart = false;
@@ -652,10 +683,10 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
     }
     if (plain && !restart && !done) {
       while (it < a.max_iter) {
-        float xo[NO];
+        float xo[NU];
         sweep(xo);
         bool stop;
-        res = plain_update<B>(a, st, xo, ro, lro, uo, false, stop);
+        res = plain_update(a, st, xo, ro, lro, uo, false, stop, reduce);
         restart = advance();
         if (stop) {  // the step is taken; a stall check due now is moot
           restart = false;
@@ -669,7 +700,7 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident(SolveArgs a)
 #pragma unroll
     for (int m = 0; m < NI; ++m) u[m] = (slot_of<B, NI>(b, m) < q) ? (float)q : 0.f;
 #pragma unroll
-    for (int k = 0; k < NO; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
+    for (int k = 0; k < NU; ++k) uo[k] = (ro[k] > 0.f) ? (float)q : 0.f;
     update_state_reset(st, a.omega);
     fell_back = true;
     if (lane == 0 && (W == 1 || wv == 0) && a.fallbacks) atomicAdd(a.fallbacks, 1u);
