@@ -373,9 +373,8 @@ int shape_for(const wmd_context* c, int cls, int q, bool log_domain, int forced)
 // Until a run has measured them, every pair is weighted with this many
 // iterations.
 constexpr double kPriorIters = 20.0;
-// Largest share of a plan's estimated work one launch may carry, and the
-// largest share up to which consecutive classes merge into one launch.
-constexpr double kMaxLaunchShare = 0.02;
+// Largest share of a plan's estimated work up to which consecutive classes
+// merge into one launch.
 constexpr double kMergeShare = 0.004;
 
 void add_launch(std::vector<ResidentLaunch>* plan, std::vector<int32_t>* flat,
@@ -441,23 +440,6 @@ int plan_resident(wmd_context* c, int forced_shape, bool log_domain,
       cls = last + 1;
     }
   }
-  // A launch above kMaxLaunchShare of the estimated work is split into
-  // document ranges (same queries), so the four lanes can be balanced.
-  double total = 0.0;
-  for (const ResidentLaunch& l : *plan) total += l.work;
-  std::vector<ResidentLaunch> split;
-  for (const ResidentLaunch& l : *plan) {
-    const int64_t m = std::min<int64_t>(
-        l.d1 - l.d0, (int64_t)std::ceil(l.work / (kMaxLaunchShare * total)));
-    for (int64_t k = 0; k < std::max<int64_t>(m, 1); ++k) {
-      ResidentLaunch p = l;
-      p.d0 = l.d0 + (l.d1 - l.d0) * k / std::max<int64_t>(m, 1);
-      p.d1 = l.d0 + (l.d1 - l.d0) * (k + 1) / std::max<int64_t>(m, 1);
-      p.work = l.work * (double)(p.d1 - p.d0) / (double)(l.d1 - l.d0);
-      if (p.d1 > p.d0) split.push_back(p);
-    }
-  }
-  plan->swap(split);
   if (c->n_long > 0) {
     // Every query against the documents beyond the tiles, in one launch.
     ResidentLaunch l{kLongLaunch, -1, flat.size(), (int64_t)c->nq, 0, c->n_long, 0.0};
