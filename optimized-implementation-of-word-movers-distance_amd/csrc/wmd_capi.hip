@@ -1495,6 +1495,7 @@ int wmd_context_run(wmd_context* c, double lamda, const wmd_options* opt_in, wmd
     sh.rows = std::max(1, std::min(16, 20 * 1024 / (c->Sdp * (int)sizeof(float))));
     sh.lamda = (float)lamda;
     sh.log_out = log_domain ? 1 : 0;
+    sh.sum_q = c->h_q_ptr[c->nq];
     HIP_TRY(wmd::launch_share_rescale(sh, c->stream));
     ++launches;
   }

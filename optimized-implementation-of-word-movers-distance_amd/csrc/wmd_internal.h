@@ -81,9 +81,15 @@ struct ShareArgs {
   int rows;                // touched words per block (rows x Sdp floats of LDS)
   float lamda;
   int log_out;
+  int sum_q;               // query words of the batch (entries of slot)
 };
 constexpr int kShareMaxQ = 512;          // longest query of the shared form
 constexpr int kShareLds = 64 * 1024;     // LDS bytes per block of the shared epilogue
+// LDS bytes of the shared epilogue's staged query table (per query 16 + 8
+// bytes, int16 slots rounded up to 16 bytes).
+constexpr size_t share_table_bytes(int nq, int sum_q) {
+  return (size_t)nq * 24 + (((size_t)sum_q * 2 + 15) & ~(size_t)15);
+}
 
 // Sinkhorn stage.  A pair is (query, document); pairs of one launch are
 // bucket_q[p / N] x (p % N), query-major.

@@ -439,20 +439,42 @@ constexpr int kShareChunks = kShareMaxQ / 64;
 // its row minimum spans (compile-time, so the chunk values stay in registers
 // and short queries do no padded work).  Slots past q read +inf, whose
 // kernel entry exp2(l2 (inf - m)) is the padding 0 (l2 < 0).
-template <int NC, int LW, bool LOG>
-__device__ __forceinline__ void share_query(const ShareArgs& a, const float* rows, int t0, int nr,
-                                            int Q, int lane) {
-  const int qb = a.q_ptr[Q];
-  const int q = a.q_ptr[Q + 1] - qb;
-  const int ks = a.k_stride[Q];
+// The batch's query table as one wave sees it: from global memory, or staged
+// in LDS (per query its first word, length, kernel-row stride and block
+// offset; the distinct-word slot of every query word as int16).
+struct ShareTable {
+  const int4* meta;        // LDS: nq x {qb, q, ks, 0}
+  const int64_t* koff;     // LDS: nq
+  const int16_t* slot;     // LDS: sum q
+};
+
+template <int NC, int LW, bool LOG, bool LT>
+__device__ __forceinline__ void share_query(const ShareArgs& a, const float* rows,
+                                            const ShareTable& tb, int t0, int nr, int Q,
+                                            int lane) {
+  int qb, q, ks;
+  int64_t koff;
+  if constexpr (LT) {
+    const int4 mq = tb.meta[Q];
+    qb = __builtin_amdgcn_readfirstlane(mq.x);
+    q = __builtin_amdgcn_readfirstlane(mq.y);
+    ks = __builtin_amdgcn_readfirstlane(mq.z);
+    koff = tb.koff[Q];
+  } else {
+    qb = a.q_ptr[Q];
+    q = a.q_ptr[Q + 1] - qb;
+    ks = a.k_stride[Q];
+    koff = a.k_off[Q];
+  }
   const float l2 = a.lamda * 1.4426950408889634f;
   int sl[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const int j = c * 64 + lane;
-    sl[c] = j < q ? a.slot[qb + j] : -1;
+    if constexpr (LT) sl[c] = j < q ? (int)tb.slot[qb + j] : -1;
+    else sl[c] = j < q ? a.slot[qb + j] : -1;
   }
-  float* Kq = a.K + a.k_off[Q];
+  float* Kq = a.K + koff;
   for (int r = 0; r < nr; ++r) {
     const int t = t0 + r;
     float d[NC];
@@ -476,28 +498,35 @@ __device__ __forceinline__ void share_query(const ShareArgs& a, const float* row
   }
 }
 
-template <bool LOG>
-__device__ __forceinline__ void share_query_any(const ShareArgs& a, const float* rows, int t0,
-                                                int nr, int Q, int lane) {
+template <bool LOG, bool LT>
+__device__ __forceinline__ void share_query_any(const ShareArgs& a, const float* rows,
+                                                const ShareTable& tb, int t0, int nr, int Q,
+                                                int lane) {
   static_assert(kShareChunks == 8, "one case per chunk count");
-  const int ks = a.k_stride[Q];
+  const int ks = LT ? __builtin_amdgcn_readfirstlane(tb.meta[Q].z) : a.k_stride[Q];
   switch ((ks + 63) >> 6) {
     case 1:
-      if (ks <= 16) share_query<1, 16, LOG>(a, rows, t0, nr, Q, lane);
-      else if (ks <= 32) share_query<1, 32, LOG>(a, rows, t0, nr, Q, lane);
-      else share_query<1, 64, LOG>(a, rows, t0, nr, Q, lane);
+      if (ks <= 16) share_query<1, 16, LOG, LT>(a, rows, tb, t0, nr, Q, lane);
+      else if (ks <= 32) share_query<1, 32, LOG, LT>(a, rows, tb, t0, nr, Q, lane);
+      else share_query<1, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane);
       break;
-    case 2: share_query<2, 64, LOG>(a, rows, t0, nr, Q, lane); break;
-    case 3: share_query<3, 64, LOG>(a, rows, t0, nr, Q, lane); break;
-    case 4: share_query<4, 64, LOG>(a, rows, t0, nr, Q, lane); break;
-    case 5: share_query<5, 64, LOG>(a, rows, t0, nr, Q, lane); break;
-    case 6: share_query<6, 64, LOG>(a, rows, t0, nr, Q, lane); break;
-    case 7: share_query<7, 64, LOG>(a, rows, t0, nr, Q, lane); break;
-    default: share_query<8, 64, LOG>(a, rows, t0, nr, Q, lane); break;
+    case 2: share_query<2, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
+    case 3: share_query<3, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
+    case 4: share_query<4, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
+    case 5: share_query<5, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
+    case 6: share_query<6, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
+    case 7: share_query<7, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
+    default: share_query<8, 64, LOG, LT>(a, rows, tb, t0, nr, Q, lane); break;
   }
 }
 
-__global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
+// LT: the query table is staged in LDS behind the rows (a batch whose table
+// fits, see share_table_bytes), so a wave's loop over its queries touches no
+// global memory but the Dd-row LDS reads and the output stores; 512 threads.
+// Otherwise 256 threads read the table from global memory.
+template <bool LT>
+__global__ __launch_bounds__(LT ? 512 : 256) void k_share_rescale(ShareArgs a) {
+  constexpr int kT = LT ? 512 : 256;
   extern __shared__ float4 lds4[];
   const float* rows = reinterpret_cast<const float*>(lds4);
   // XCD-aware tile order: blocks are dispatched to the 8 XCDs in turn, so
@@ -510,14 +539,31 @@ __global__ __launch_bounds__(256) void k_share_rescale(ShareArgs a) {
   const int nr = min(a.rows, a.Vt - t0);
   const float4* src = reinterpret_cast<const float4*>(a.Dd + (int64_t)t0 * a.Sdp);
   const int n4 = nr * (a.Sdp >> 2);
-  for (int k = threadIdx.x; k < n4; k += 256) lds4[k] = src[k];
+  for (int k = threadIdx.x; k < n4; k += kT) lds4[k] = src[k];
+  ShareTable tb{};
+  if constexpr (LT) {
+    // behind the rows: meta (16 B per query), koff (8 B per query), slots
+    int4* meta = reinterpret_cast<int4*>(lds4 + a.rows * (a.Sdp >> 2));
+    int64_t* koff = reinterpret_cast<int64_t*>(meta + a.nq);
+    int16_t* slot = reinterpret_cast<int16_t*>(koff + a.nq);
+    for (int Q = threadIdx.x; Q < a.nq; Q += kT) {
+      const int qb = a.q_ptr[Q];
+      meta[Q] = make_int4(qb, a.q_ptr[Q + 1] - qb, a.k_stride[Q], 0);
+      koff[Q] = a.k_off[Q];
+    }
+    for (int i = threadIdx.x; i < a.sum_q; i += kT) slot[i] = (int16_t)a.slot[i];
+    tb.meta = meta;
+    tb.koff = koff;
+    tb.slot = slot;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int kW = kT / 64;
   if (a.log_out) {
-    for (int Q = w; Q < a.nq; Q += 4) share_query_any<true>(a, rows, t0, nr, Q, lane);
+    for (int Q = w; Q < a.nq; Q += kW) share_query_any<true, LT>(a, rows, tb, t0, nr, Q, lane);
   } else {
-    for (int Q = w; Q < a.nq; Q += 4) share_query_any<false>(a, rows, t0, nr, Q, lane);
+    for (int Q = w; Q < a.nq; Q += kW) share_query_any<false, LT>(a, rows, tb, t0, nr, Q, lane);
   }
 }
 
@@ -1405,10 +1451,15 @@ hipError_t launch_cdist_mfma(const MfmaCdistArgs& a, int nq, hipStream_t s) {
 
 hipError_t launch_share_rescale(const ShareArgs& a, hipStream_t s) {
   if (a.Vt <= 0 || a.nq <= 0) return hipSuccess;
-  if (a.rows <= 0 || (a.Sdp & 63) != 0 || (size_t)a.rows * a.Sdp * sizeof(float) > kShareLds)
-    return hipErrorInvalidValue;
-  const size_t lds = (size_t)a.rows * a.Sdp * sizeof(float);
-  hipLaunchKernelGGL(k_share_rescale, dim3((a.Vt + a.rows - 1) / a.rows), dim3(256), lds, s, a);
+  const size_t rows_b = (size_t)a.rows * a.Sdp * sizeof(float);
+  if (a.rows <= 0 || (a.Sdp & 63) != 0 || rows_b > kShareLds) return hipErrorInvalidValue;
+  const size_t table_b = share_table_bytes(a.nq, a.sum_q);
+  const dim3 grid((a.Vt + a.rows - 1) / a.rows);
+  if (rows_b + table_b <= kShareLds && a.Sdp <= 32768) {  // int16 slots
+    hipLaunchKernelGGL(k_share_rescale<true>, grid, dim3(512), rows_b + table_b, s, a);
+  } else {
+    hipLaunchKernelGGL(k_share_rescale<false>, grid, dim3(256), rows_b, s, a);
+  }
   return hipGetLastError();
 }
 
