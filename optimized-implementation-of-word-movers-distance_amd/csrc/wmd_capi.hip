@@ -867,6 +867,7 @@ wmd::WarmInferArgs warm_args(wmd_context* c, double lamda) {
   a.q.qn = c->qn.p;
   a.q.q_ptr = c->q_ptr.p;
   a.q.nq = c->nq;
+  a.q.qmax = c->qmax;
   a.mrow = c->mrow.p;
   a.Vt = (int)c->Vt;
   a.lam_abs = (float)(-lamda);

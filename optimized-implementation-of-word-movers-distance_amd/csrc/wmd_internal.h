@@ -266,6 +266,7 @@ struct WarmQueries {
   const float* qn;         // sum q
   const int32_t* q_ptr;    // nq + 1
   int nq;
+  int qmax;                // longest query (sizes the inference kernel's LDS tables)
 };
 struct WarmInferArgs {
   WarmCorpus c;

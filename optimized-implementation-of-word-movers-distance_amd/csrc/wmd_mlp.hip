@@ -101,7 +101,8 @@ __device__ __forceinline__ void sample_feats(const WarmInferArgs& a, const float
   x[1] = n * n;
   x[2] = cdf;
   x[3] = T;
-  x[4] = a.lam_abs * (sqrtf(fmaf(T, T, n * n)) - mr);
+  // v_sqrt_f32 (1 ulp; training and inference both take the feature from here)
+  x[4] = a.lam_abs * (__builtin_amdgcn_sqrtf(fmaf(T, T, n * n)) - mr);
 }
 
 // All 15 raw features of a sample.
@@ -234,43 +235,232 @@ __global__ __launch_bounds__(256) void k_query_feats(const int32_t* q_ptr, const
 
 // ---------------------------------------------------------------------------
 // Inference.  Blocks of 256 threads (four waves) over kInferChunks x 256
-// consecutive nonzeros of one query (blockIdx.y); the query's sorted norm
-// table sits in LDS for the quantile lookups.  Each wave takes 64 samples at
-// a time: lane l computes sample l's 15 standardised features, which pass
-// through LDS (FP16, 16 per sample, the last 1, the input of b1) into the B-operand layout of
-// v_mfma_f32_16x16x16_f16 (lane l holds rows 4 (l / 16) .. +3 of column
-// l % 16).  The hidden layers are computed transposed, H^T = W X^T (units
-// along M, samples along N), so a layer's accumulator tile, which has the
-// same lane layout, is the next layer's B operand after ReLU and conversion:
-//   layer 1: 2 unit tiles x 4 sample tiles, K = 16 (the features and 1);
-//            ReLU as packed FP16 maxima after the conversion
-//   layer 2: 2 x 4 tiles, K = 32 (two steps), b2 as C, the same ReLU
-//   output:  each lane's w3-weighted sum over its 4 FP16 units (packed FP16
-//            dot products, FP32 accumulation, w3 in FP16), summed over the
-//            four 16-lane groups (permlane swaps), + b3
-// With export set the kernel writes the 15 raw features instead.
+// consecutive nonzeros and a group of up to kInferGroupMax consecutive
+// queries (blockIdx.y; the group size is a launch argument)
+// (blockIdx.y); the group's sorted norm tables and their bucket tables sit
+// in LDS for the quantile lookups.  Each wave takes 64 samples (nonzeros) at
+// a time: lane l loads nonzero l's word, document and mid-rank once and
+// standardises the eight features that do not depend on the query; then per
+// query of the group it adds the two that do (the matched norm T and the
+// rescale gap) and the query's five statistics, and the wave runs the
+// network on the 64 samples.
+//
+// The network's 16 inputs are taken in slot order: slots 0-7 the
+// query-independent features (raw 0, 1, 2-6, 12), slots 8-14 the
+// query-dependent ones (raw 13, 14, 7-11), slot 15 a constant 1 whose weight
+// is b1.  A staged sample row holds the 16 FP16 inputs (the first eight
+// written once per chunk, the last eight per query); lane l reads inputs
+// 8 (l / 32) .. +7 of sample l % 32 of a 32-sample tile, which is the
+// B-operand layout of v_mfma_f32_32x32x16_f16.  The hidden layers are
+// computed transposed, H^T = W X^T (units along M, samples along N):
+//   layer 1: one 32 x 32 tile per sample tile, K = 16
+//   layer 2: K = 32 in two steps whose B operands are the layer-1 tile's
+//            lane share after ReLU and FP16 conversion (the accumulator holds
+//            units 8 (r / 4) + 4 (l / 32) + r % 4 of sample l % 32 in its
+//            element r, so step s takes elements 8 s .. 8 s + 7 and the
+//            weight columns are permuted to match: input unit
+//            16 s + 8 (j / 4) + 4 (l / 32) + j % 4 at position j); b2 as C
+//   output:  each lane's w3-weighted sum over its 16 FP16 units (packed FP16
+//            dot products, FP32 accumulation, w3 in FP16), one permlane32
+//            swap adds the two lane halves of both sample tiles, + b3
+// ReLU is applied in FP16 (fp16(max(d, 0)) = max(fp16(d), 0): rounding is
+// monotonic) as packed conversions and packed maxima.
 constexpr int kInferChunks = 8;
-constexpr int kStageRow = 20;  // FP16 per staged sample (16 + 4 padding: 8-byte aligned rows)
-typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
-typedef float float4_t __attribute__((ext_vector_type(4)));
-
+constexpr int kInferGroupMax = 8;
+constexpr int kStageRow = 24;  // FP16 per staged sample (16 + 8 padding: 16-byte aligned rows)
+constexpr int kSlotFeat[15] = {0, 1, 2, 3, 4, 5, 6, 12, 13, 14, 7, 8, 9, 10, 11};
+constexpr size_t kStageBytes = 4 * 64 * kStageRow * sizeof(_Float16);
+constexpr size_t kBucketBytes = ((kKnotBuckets + 1) * sizeof(int16_t) + 15) & ~(size_t)15;
+// Dynamic LDS of an inference block: the four waves' stages, then per query
+// of the group its standardised statistics (16 bytes), its bucket table and
+// its two norm tables of qmax entries.
+constexpr size_t infer_lds_bytes(int qmax, int group) {
+  return kStageBytes +
+         (size_t)group * (16 + kBucketBytes + 2 * (((size_t)qmax + 3) & ~(size_t)3) * 4);
+}
+// Queries per block: the most (up to kInferGroupMax) whose tables keep a
+// block within 40 KB of LDS (four blocks, sixteen waves, per CU).
+inline int infer_group(int qmax) {
+  int g = kInferGroupMax;
+  while (g > 1 && infer_lds_bytes(qmax, g) > 40 * 1024) g >>= 1;
+  return g;
+}
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
 
-// ReLU of an accumulator tile in FP16: two packed conversions, then packed
-// maxima with 0 (fp16(max(d, 0)) = max(fp16(d), 0), rounding is monotonic).
-__device__ __forceinline__ half4_t relu_h4(float4_t d) {
+// ReLU of elements 8 s .. 8 s + 7 of an accumulator share, in FP16.
+__device__ __forceinline__ half8_t relu_h8(const float16_t& d, int s) {
   const half2_t z = {(_Float16)0.f, (_Float16)0.f};
-  const half2_t lo = __builtin_elementwise_max(half2_t{(_Float16)d[0], (_Float16)d[1]}, z);
-  const half2_t hi = __builtin_elementwise_max(half2_t{(_Float16)d[2], (_Float16)d[3]}, z);
-  return half4_t{lo[0], lo[1], hi[0], hi[1]};
+  half8_t h;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const half2_t v = __builtin_elementwise_max(
+        half2_t{(_Float16)d[8 * s + 2 * i], (_Float16)d[8 * s + 2 * i + 1]}, z);
+    h[2 * i] = v[0];
+    h[2 * i + 1] = v[1];
+  }
+  return h;
 }
 
-template <bool EXPORT>
-__global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
-  static_assert(H == 32 && F <= 16, "the matrix-core layout takes 32 units and 16 inputs");
+__device__ __forceinline__ uint32_t pack_h2(float lo, float hi) {
+  const half2_t h = {(_Float16)lo, (_Float16)hi};
+  return __builtin_bit_cast(uint32_t, h);
+}
+
+// Orders a wave's writes of its own LDS stage before its lanes read other
+// lanes' rows, and those reads before the next writes.  One wave's LDS
+// instructions execute in issue order, so a compiler barrier with a
+// wavefront-scope fence is all it takes; the block's other waves, which
+// stage elsewhere, run on without waiting.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a, int group) {
+  static_assert(H == 32 && F == 15, "the matrix-core layout takes 32 units and 15 inputs + 1");
+  extern __shared__ float4 lds4[];
+  _Float16* stage = reinterpret_cast<_Float16*>(lds4);
+  const int qcap = (a.q.qmax + 3) & ~3;
+  uint4* qd = reinterpret_cast<uint4*>(reinterpret_cast<char*>(lds4) + kStageBytes);
+  const auto bk_of = [&](int g) {
+    return reinterpret_cast<int16_t*>(qd + group) + g * (kBucketBytes / sizeof(int16_t));
+  };
+  float* tables = reinterpret_cast<float*>(reinterpret_cast<char*>(qd + group) +
+                                           group * kBucketBytes);
+  const int Q0 = blockIdx.y * group;
+  const int nG = min(group, a.q.nq - Q0);
+  for (int g = 0; g < nG; ++g) {
+    const int qb = a.q.q_ptr[Q0 + g], q = a.q.q_ptr[Q0 + g + 1] - qb;
+    float* sqc = tables + 2 * g * qcap;
+    for (int i = threadIdx.x; i < q; i += 256) {
+      sqc[i] = a.q.qc[qb + i];
+      sqc[qcap + i] = a.q.qn[qb + i];
+    }
+  }
+  const cfloat* p = as_const(a.params);
+  // standardisation z = (x - xm) xs as x xs + zoff, zoff = -xm xs
+  const auto zstd = [&](float x, int f) {
+    return fmaf(x, p[kWarmXs + f], -p[kWarmXm + f] * p[kWarmXs + f]);
+  };
+  // The group's query statistics, standardised: input slots 10-14 and 15
+  // (the constant 1) as three FP16 pairs per query, behind the query's T and
+  // gap pair in its half of the sample rows; the first word holds the query's
+  // length until the pair replaces it.
+  if (threadIdx.x < nG) {
+    const int Q = Q0 + threadIdx.x;
+    const float* qs = a.q.qstat + (int64_t)Q * 5;
+    qd[threadIdx.x] = make_uint4((uint32_t)(a.q.q_ptr[Q + 1] - a.q.q_ptr[Q]),
+                                 pack_h2(zstd(qs[0], 7), zstd(qs[1], 8)),
+                                 pack_h2(zstd(qs[2], 9), zstd(qs[3], 10)),
+                                 pack_h2(zstd(qs[4], 11), 1.f));
+  }
+  __syncthreads();
+  for (int g = 0; g < nG; ++g) {
+    const int q = a.q.q_ptr[Q0 + g + 1] - a.q.q_ptr[Q0 + g];
+    build_knot_buckets(tables + 2 * g * qcap, q, bk_of(g));
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * (256 * kInferChunks);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lr = lane & 31, hf = lane >> 5;
+  // This lane's operand fragments (row lr of the weight tiles, K positions
+  // of lane half hf): A1 = W1 at input slots 8 hf .. +7 (slot 15: b1),
+  // A2[s] = W2 in the permuted K order of step s, C2 = b2 and W3 = w3 of
+  // the units of accumulator elements r.
+  half8_t A1, A2[2];
+  float16_t C2;
+  half2_t W3[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    float w = p[kWarmB1 + lr];
+#pragma unroll
+    for (int k = 0; k < 15; ++k)  // slot 8 hf + j, selected without indexing by lane
+      if (8 * hf + j == k) w = p[kWarmW1 + lr * F + kSlotFeat[k]];
+    A1[j] = (_Float16)w;
+#pragma unroll
+    for (int st = 0; st < 2; ++st)
+      A2[st][j] = (_Float16)p[kWarmW2 + lr * H + 16 * st + 8 * (j >> 2) + 4 * hf + (j & 3)];
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int unit = 8 * (r >> 2) + 4 * hf + (r & 3);
+    C2[r] = p[kWarmB2 + unit];
+    W3[r >> 1][r & 1] = (_Float16)p[kWarmW3 + unit];
+  }
+  const float b3 = p[kWarmB3];
+  _Float16* row = stage + (wv * 64 + lane) * kStageRow;  // this lane's sample row
+  // Lanes past the last nonzero compute on the last one and store nothing
+  // (the wave's matrix instructions take all 64 lanes).
+  for (int ch = 0; ch < kInferChunks; ++ch) {
+    const int64_t pos = base + ch * 256 + threadIdx.x;
+    const bool valid = pos < a.c.nnz;
+    const int64_t pp = valid ? pos : a.c.nnz - 1;
+    const int t = a.c.doc_t[pp];
+    const int j = a.c.doc_of[pp];
+    const float cdf = a.c.dcdf[pp];
+    const float n = a.c.wnorm[t];
+    const float* ds = a.c.dstat + (int64_t)j * 5;
+    uint4 zi;
+    zi.x = pack_h2(zstd(n, 0), zstd(n * n, 1));
+    zi.y = pack_h2(zstd(ds[0], 2), zstd(ds[1], 3));
+    zi.z = pack_h2(zstd(ds[2], 4), zstd(ds[3], 5));
+    zi.w = pack_h2(zstd(ds[4], 6), zstd(cdf, 12));
+    *reinterpret_cast<uint4*>(row) = zi;
+#pragma unroll 1
+    for (int g = 0; g < nG; ++g) {
+      const int Q = Q0 + g;
+      uint4 zq = qd[g];
+      const int q = __builtin_amdgcn_readfirstlane((int)zq.x);
+      const float* sqc = tables + 2 * g * qcap;
+      // features 13 and 14 as in sample_feats
+      const float T = interp_bucketed(sqc, sqc + qcap, bk_of(g), q, cdf);
+      const float mr = a.mrow[(int64_t)Q * a.Vt + t];
+      const float gap = a.lam_abs * (__builtin_amdgcn_sqrtf(fmaf(T, T, n * n)) - mr);
+      zq.x = pack_h2(zstd(T, 13), zstd(gap, 14));
+      *reinterpret_cast<uint4*>(row + 8) = zq;
+      wave_lds_sync();
+      half8_t Bx[2];
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+        Bx[nt] = *reinterpret_cast<const half8_t*>(stage + (wv * 64 + nt * 32 + lr) * kStageRow +
+                                                   8 * hf);
+      wave_lds_sync();  // the stage is rewritten by the next query or chunk
+      float yp[2];
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt) {
+        const float16_t d1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A1, Bx[nt], float16_t{}, 0, 0, 0);
+        float16_t d2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A2[0], relu_h8(d1, 0), C2, 0, 0, 0);
+        d2 = __builtin_amdgcn_mfma_f32_32x32x16_f16(A2[1], relu_h8(d1, 1), d2, 0, 0, 0);
+        // the output layer on the FP16 layer-2 units: packed dot products
+        // with FP32 accumulation
+        float y = 0.f;
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+          const half8_t h2 = relu_h8(d2, st);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            y = __builtin_amdgcn_fdot2(half2_t{h2[2 * i], h2[2 * i + 1]}, W3[4 * st + i], y, false);
+        }
+        yp[nt] = y;
+      }
+      // lane l < 32: sample l of tile 0, both halves' units; lane l >= 32:
+      // sample l of the chunk (l - 32 of tile 1)
+      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(yp[0]),
+                                                       __float_as_uint(yp[1]), false, false);
+      const float y = __uint_as_float(sw[0]) + __uint_as_float(sw[1]);
+      if (valid) a.pred[(int64_t)Q * a.c.nnz + pos] = y + b3;
+    }
+  }
+}
+
+// The 15 raw features of every (query, nonzero) sample: one query per block
+// (blockIdx.y), kInferChunks x 256 nonzeros.
+__global__ __launch_bounds__(256) void k_warm_export(WarmInferArgs a) {
   __shared__ float sqc[kWarmQueryMax], sqn[kWarmQueryMax];
   __shared__ int16_t sbk[kKnotBuckets + 1];
-  __shared__ __attribute__((aligned(16))) _Float16 stage[4][64 * kStageRow];
   const int Q = blockIdx.y;
   const int qb = a.q.q_ptr[Q], q = a.q.q_ptr[Q + 1] - qb;
   if (q > kWarmQueryMax) return;  // refused on the host; never index past the tables
@@ -281,94 +471,15 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a) {
   __syncthreads();
   build_knot_buckets(sqc, q, sbk);
   __syncthreads();
-  const cfloat* p = as_const(a.params);
   const int64_t base = (int64_t)blockIdx.x * (256 * kInferChunks);
-  if constexpr (EXPORT) {
-    for (int ch = 0; ch < kInferChunks; ++ch) {
-      const int64_t pos = base + ch * 256 + threadIdx.x;
-      if (pos >= a.c.nnz) break;
-      float x[F];
-      all_feats(a, sqc, sqn, q, Q, pos, x, sbk);
-      float* o = a.feats + ((int64_t)Q * a.c.nnz + pos) * F;
+  for (int ch = 0; ch < kInferChunks; ++ch) {
+    const int64_t pos = base + ch * 256 + threadIdx.x;
+    if (pos >= a.c.nnz) break;
+    float x[F];
+    all_feats(a, sqc, sqn, q, Q, pos, x, sbk);
+    float* o = a.feats + ((int64_t)Q * a.c.nnz + pos) * F;
 #pragma unroll
-      for (int k = 0; k < F; ++k) o[k] = x[k];
-    }
-    return;
-  } else {
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int lr = lane & 15, lg = lane >> 4;
-    // This lane's operand fragments: A = weight rows (unit lr of each tile,
-    // inputs 4 lg .. +3 of each K step; layer 1's input 15 is a constant 1
-    // whose weight is b1), C = b2 and the w3 factors of units 4 lg .. +3 of
-    // each tile.
-    half4_t A1[2], A2[2][2];
-    float4_t C2[2];
-    half2_t W3[2][2];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      const int unit = mt * 16 + lr;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = 4 * lg + i;
-        A1[mt][i] = (_Float16)(k < F ? p[kWarmW1 + unit * F + k] : p[kWarmB1 + unit]);
-        A2[mt][0][i] = (_Float16)p[kWarmW2 + unit * H + k];
-        A2[mt][1][i] = (_Float16)p[kWarmW2 + unit * H + 16 + k];
-        C2[mt][i] = p[kWarmB2 + mt * 16 + 4 * lg + i];
-        W3[mt][i >> 1][i & 1] = (_Float16)p[kWarmW3 + mt * 16 + 4 * lg + i];
-      }
-    }
-    const float b3 = p[kWarmB3];
-    // standardisation z = (x - xm) xs as x xs + zoff, zoff = -xm xs
-    float zoff[F];
-#pragma unroll
-    for (int k = 0; k < F; ++k) zoff[k] = -p[kWarmXm + k] * p[kWarmXs + k];
-    _Float16* st = stage[wv];
-    // Every wave runs every chunk (the barriers stay block-uniform); lanes
-    // past the last nonzero compute on the last one and store nothing.
-    for (int ch = 0; ch < kInferChunks; ++ch) {
-      const int64_t pos = base + ch * 256 + threadIdx.x;
-      const bool valid = pos < a.c.nnz;
-      float x[F];
-      all_feats(a, sqc, sqn, q, Q, valid ? pos : a.c.nnz - 1, x, sbk);
-      half4_t zh[4];
-#pragma unroll
-      for (int k = 0; k < 16; ++k)
-        zh[k >> 2][k & 3] = (_Float16)(k < F ? fmaf(x[k], p[kWarmXs + k], zoff[k]) : 1.f);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) *reinterpret_cast<half4_t*>(st + lane * kStageRow + 4 * v) = zh[v];
-      __syncthreads();
-      half4_t Bx[4];
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt)
-        Bx[nt] = *reinterpret_cast<const half4_t*>(st + (nt * 16 + lr) * kStageRow + 4 * lg);
-      __syncthreads();  // the stage is rewritten by the next chunk
-      half4_t H1[2][4];
-#pragma unroll
-      for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt)
-          H1[mt][nt] = relu_h4(__builtin_amdgcn_mfma_f32_16x16x16f16(A1[mt], Bx[nt], float4_t{},
-                                                                     0, 0, 0));
-      float yp[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int mt = 0; mt < 2; ++mt) {
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) {
-          float4_t d = __builtin_amdgcn_mfma_f32_16x16x16f16(A2[mt][0], H1[0][nt], C2[mt], 0, 0, 0);
-          d = __builtin_amdgcn_mfma_f32_16x16x16f16(A2[mt][1], H1[1][nt], d, 0, 0, 0);
-          // the output layer on the FP16 layer-2 units: packed dot products
-          // with FP32 accumulation
-          const half4_t h2 = relu_h4(d);
-          yp[nt] = __builtin_amdgcn_fdot2(half2_t{h2[0], h2[1]}, W3[mt][0], yp[nt], false);
-          yp[nt] = __builtin_amdgcn_fdot2(half2_t{h2[2], h2[3]}, W3[mt][1], yp[nt], false);
-        }
-      }
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) yp[nt] = bfly_red<16, 32>(yp[nt]);
-      // lane l = 16 lg + lr holds sample l in sample tile lg
-      const float y = lg == 0 ? yp[0] : lg == 1 ? yp[1] : lg == 2 ? yp[2] : yp[3];
-      if (valid) a.pred[(int64_t)Q * a.c.nnz + pos] = y + b3;
-    }
+    for (int k = 0; k < F; ++k) o[k] = x[k];
   }
 }
 
@@ -649,10 +760,17 @@ hipError_t launch_query_feats(const int32_t* q_ptr, const int32_t* q_words, cons
 
 hipError_t launch_warm_infer(const WarmInferArgs& a, hipStream_t s) {
   if (a.q.nq <= 0 || a.c.nnz <= 0) return hipSuccess;
+  if (a.q.qmax <= 0 || a.q.qmax > kWarmQueryMax) return hipErrorInvalidValue;
   const int64_t per = 256 * kInferChunks;
-  const dim3 grid((unsigned)((a.c.nnz + per - 1) / per), (unsigned)a.q.nq);
-  if (a.feats) hipLaunchKernelGGL(k_warm_infer<true>, grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL(k_warm_infer<false>, grid, dim3(256), 0, s, a);
+  const unsigned xb = (unsigned)((a.c.nnz + per - 1) / per);
+  if (a.feats) {
+    hipLaunchKernelGGL(k_warm_export, dim3(xb, (unsigned)a.q.nq), dim3(256), 0, s, a);
+  } else {
+    const int group = infer_group(a.q.qmax);
+    const unsigned yb = (unsigned)((a.q.nq + group - 1) / group);
+    hipLaunchKernelGGL(k_warm_infer, dim3(xb, yb), dim3(256), infer_lds_bytes(a.q.qmax, group), s,
+                       a, group);
+  }
   return hipGetLastError();
 }
 
