@@ -214,6 +214,25 @@ __global__ __launch_bounds__(256) void k_split_rows(SplitArgs a) {
   if (lane == 0) a.norms[row] = ss;
 }
 
+// Squared distance of a near pair (d^2 below kNearTau (|a|^2 + |b|^2)) as
+// FP64 direct differences of the FP32 embeddings.  Near pairs are rare (a
+// query word and the same or an almost equal corpus word): the epilogues
+// mark them in a per-lane bit mask and redo them after the common path.
+__device__ __forceinline__ double near_d2(const float* x, const float* y, int D) {
+  double s = 0.0;
+  for (int k = 0; k < D; ++k) {
+    const double df = (double)x[k] - (double)y[k];
+    s += df * df;
+  }
+  return s;
+}
+
+// The matrix-core distance from its FP64 square: the FP64 root rounded to
+// FP32.  Every matrix-core kernel takes its distances from here.
+__device__ __forceinline__ float mfma_dist(double d2) {
+  return (float)sqrt(d2 > 0.0 ? d2 : 0.0);
+}
+
 __device__ __forceinline__ bf16x8 load_bf16x8(const uint16_t* p, bool valid) {
   u16x8 v = {};
   if (valid) v = *reinterpret_cast<const u16x8*>(p);
@@ -285,30 +304,37 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc, 0, 0, 0);
     }
     const int jo = qc + wx * 32 + (lane & 31);  // output column of this lane
+    const bool jov = jo < q;
+    const double nbj = jov ? a.nq[qb + jo] : 0.0;
+    uint32_t near = 0;  // bit r: a near pair, left out here and redone below
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
       const int t = t0 + wy * 32 + i;
       float d = INFINITY;
-      if (t < a.Vt && jo < q) {
-        const double na = a.nt[t], nb = a.nq[qb + jo];
-        double d2 = na + nb - 2.0 * (double)acc[r];
-        if (d2 < kNearTau * (na + nb)) {
-          const float* x = a.E + (int64_t)a.touched[t] * a.D;
-          const float* y = a.E + (int64_t)a.q_words[qb + jo] * a.D;
-          double s = 0.0;
-          for (int k = 0; k < a.D; ++k) {
-            const double df = (double)x[k] - (double)y[k];
-            s += df * df;
-          }
-          d2 = s;
+      if (t < a.Vt && jov) {
+        const double na = a.nt[t];
+        const double d2 = na + nbj - 2.0 * (double)acc[r];
+        if (d2 < kNearTau * (na + nbj)) {
+          near |= 1u << r;
+        } else {
+          d = mfma_dist(d2);
+          Kq[(int64_t)t * ks + jo] = d;  // raw distance, rescaled below
         }
-        d = (float)sqrt(d2 > 0.0 ? d2 : 0.0);
-        Kq[(int64_t)t * ks + jo] = d;  // raw distance, rescaled below
       }
       // Row minimum over this wave's 32 columns, then across waves/chunks.
       const float mn = bfly_red<1, 16, kMin>(d);
       if ((lane & 31) == 0) atomicMin(&smin[wy * 32 + i], __float_as_uint(mn));
+    }
+    while (near) {
+      const int r = __builtin_ctz(near);
+      near &= near - 1;
+      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+      const int t = t0 + wy * 32 + i;
+      const float dn = mfma_dist(near_d2(a.E + (int64_t)a.touched[t] * a.D,
+                                         a.E + (int64_t)a.q_words[qb + jo] * a.D, a.D));
+      Kq[(int64_t)t * ks + jo] = dn;
+      atomicMin(&smin[wy * 32 + i], __float_as_uint(dn));
     }
   }
   // (Raw-mode launches run k_cdist_mfma_raw instead.)
@@ -400,29 +426,38 @@ __global__ __launch_bounds__(256) void k_cdist_mfma_raw(MfmaCdistArgs a) {
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a2, b0, acc[c], 0, 0, 0);
     }
   }
+  // Row by row (the row's norm and output row once), then the chunks'
+  // columns of this lane; near pairs are marked and redone after.
+  int jo[kRawChunks];
+  double nbq[kRawChunks];
 #pragma unroll
   for (int c = 0; c < kRawChunks; ++c) {
-    const int jo = c * 64 + wx * 32 + (lane & 31);  // output column of this lane
+    jo[c] = c * 64 + wx * 32 + (lane & 31);
+    nbq[c] = jo[c] < q ? a.nq[qb + jo[c]] : 0.0;
+  }
+  uint64_t near = 0;  // bit 4 r + c
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-      const int t = t0 + wy * 32 + i;
-      if (t < a.Vt && jo < q) {
-        const double na = a.nt[t], nb = a.nq[qb + jo];
-        double d2 = na + nb - 2.0 * (double)acc[c][r];
-        if (d2 < kNearTau * (na + nb)) {
-          const float* x = a.E + (int64_t)a.touched[t] * a.D;
-          const float* y = a.E + (int64_t)a.q_words[qb + jo] * a.D;
-          double s = 0.0;
-          for (int k = 0; k < a.D; ++k) {
-            const double df = (double)x[k] - (double)y[k];
-            s += df * df;
-          }
-          d2 = s;
-        }
-        Kq[(int64_t)t * ks + jo] = (float)sqrt(d2 > 0.0 ? d2 : 0.0);
-      }
+  for (int r = 0; r < 16; ++r) {
+    const int t = t0 + wy * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (t >= a.Vt) continue;
+    const double na = a.nt[t];
+    float* row = Kq + (int64_t)t * ks;
+#pragma unroll
+    for (int c = 0; c < kRawChunks; ++c) {
+      if (jo[c] >= q) continue;
+      const double d2 = na + nbq[c] - 2.0 * (double)acc[c][r];
+      if (d2 < kNearTau * (na + nbq[c])) near |= 1ull << (4 * r + c);
+      row[jo[c]] = mfma_dist(d2);
     }
+  }
+  while (near) {
+    const int bit = __builtin_ctzll(near);
+    near &= near - 1;
+    const int r = bit >> 2, c = bit & 3;
+    const int t = t0 + wy * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int j = c * 64 + wx * 32 + (lane & 31);
+    Kq[(int64_t)t * ks + j] = mfma_dist(near_d2(a.E + (int64_t)a.touched[t] * a.D,
+                                                a.E + (int64_t)a.q_words[qb + j] * a.D, a.D));
   }
 }
 
