@@ -38,14 +38,14 @@ struct SplitArgs {
   int rows;
   int D;
   int Dp;                  // D rounded up to 16 (zero padding)
-  uint16_t* out;           // [3][rows][Dp] BF16 bits: hi, mid, lo
+  uint16_t* out;           // [3][Dp / 16][rows][16] BF16 bits: hi, mid, lo
   double* norms;           // rows: FP64 |x|^2
 };
 
 // Matrix-core distance stage: same outputs as CdistArgs' stage.
 struct MfmaCdistArgs {
-  const uint16_t* Es;      // [3][Vt][Dp] touched-word planes
-  const uint16_t* Qs;      // [3][Sq][Dp] query-word planes (query-major, as q_words)
+  const uint16_t* Es;      // [3][Dp / 16][Vt][16] touched-word planes (k-step major)
+  const uint16_t* Qs;      // [3][Dp / 16][Sq][16] query-word planes (rows query-major, as q_words)
   const double* nt;        // Vt
   const double* nq;        // Sq
   const float* E;          // V x D (direct FP64 recomputation of near pairs)

@@ -189,15 +189,18 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// One wave per row: three BF16 planes [3][rows][Dp] (zero past D) and the
-// FP64 squared norm of each row.
+// One wave per row: three BF16 planes (zero past D) and the FP64 squared
+// norm of each row.  Planes are k-step major, [3][Dp / 16][rows][16]: the 16
+// values of one k-step of consecutive rows are contiguous, so a 32-row
+// operand fragment of the matrix kernels is one 1 KB run (eight whole cache
+// lines) instead of 32 B from each of 32 rows.
 __global__ __launch_bounds__(256) void k_split_rows(SplitArgs a) {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (row >= a.rows) return;
   const float* src = a.E + (int64_t)a.ids[row] * a.D;
   const size_t plane = (size_t)a.rows * a.Dp;
-  uint16_t* dst = a.out + (size_t)row * a.Dp;
+  uint16_t* dst = a.out + (size_t)row * 16;
   double ss = 0.0;
   for (int k = lane; k < a.Dp; k += 64) {
     const float x = (k < a.D) ? src[k] : 0.f;
@@ -205,9 +208,10 @@ __global__ __launch_bounds__(256) void k_split_rows(SplitArgs a) {
     const float r1 = x - bf16_to_f32(hi);
     const uint16_t mid = bf16_rne(r1);
     const uint16_t lo = bf16_rne(r1 - bf16_to_f32(mid));
-    dst[k] = hi;
-    dst[plane + k] = mid;
-    dst[2 * plane + k] = lo;
+    const size_t o = (size_t)(k >> 4) * a.rows * 16 + (k & 15);
+    dst[o] = hi;
+    dst[plane + o] = mid;
+    dst[2 * plane + o] = lo;
     ss += (double)x * (double)x;
   }
   ss = bfly_sum_d<1, 32>(ss);
@@ -282,20 +286,22 @@ __global__ __launch_bounds__(256) void k_cdist_mfma(MfmaCdistArgs a) {
   const int ta = t0 + wy * 32 + (lane & 31);  // A row this lane supplies
   const bool tav = ta < a.Vt;
   const int kh = (lane >> 5) * 8;
-  const uint16_t* ap = a.Es + (size_t)(tav ? ta : 0) * Dp + kh;
+  // planes k-step major: row x of k-step k0 / 16 at (k0 / 16) rows 16 + 16 x
+  const uint16_t* ap = a.Es + (size_t)(tav ? ta : 0) * 16 + kh;
 
   for (int qc = 0; qc < q; qc += 64) {
     const int jb = qc + wx * 32 + (lane & 31);  // B column (query word) this lane supplies
     const bool jbv = jb < q;
-    const uint16_t* bp = a.Qs + (size_t)(qb + (jbv ? jb : 0)) * Dp + kh;
+    const uint16_t* bp = a.Qs + (size_t)(qb + (jbv ? jb : 0)) * 16 + kh;
     f32x16 acc = {};
     for (int k0 = 0; k0 < Dp; k0 += 16) {
-      const bf16x8 a0 = load_bf16x8(ap + k0, tav);
-      const bf16x8 a1 = load_bf16x8(ap + tplane + k0, tav);
-      const bf16x8 a2 = load_bf16x8(ap + 2 * tplane + k0, tav);
-      const bf16x8 b0 = load_bf16x8(bp + k0, jbv);
-      const bf16x8 b1 = load_bf16x8(bp + qplane + k0, jbv);
-      const bf16x8 b2 = load_bf16x8(bp + 2 * qplane + k0, jbv);
+      const size_t ka = (size_t)k0 * a.Vt, kb = (size_t)k0 * a.Sq;
+      const bf16x8 a0 = load_bf16x8(ap + ka, tav);
+      const bf16x8 a1 = load_bf16x8(ap + tplane + ka, tav);
+      const bf16x8 a2 = load_bf16x8(ap + 2 * tplane + ka, tav);
+      const bf16x8 b0 = load_bf16x8(bp + kb, jbv);
+      const bf16x8 b1 = load_bf16x8(bp + qplane + kb, jbv);
+      const bf16x8 b2 = load_bf16x8(bp + 2 * qplane + kb, jbv);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc, 0, 0, 0);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc, 0, 0, 0);
@@ -397,27 +403,29 @@ __global__ __launch_bounds__(256) void k_cdist_mfma_raw(MfmaCdistArgs a) {
   const int ta = t0 + wy * 32 + (lane & 31);
   const bool tav = ta < a.Vt;
   const int kh = (lane >> 5) * 8;
-  const uint16_t* ap = a.Es + (size_t)(tav ? ta : 0) * Dp + kh;
+  // planes k-step major, as in k_cdist_mfma
+  const uint16_t* ap = a.Es + (size_t)(tav ? ta : 0) * 16 + kh;
   const uint16_t* bp[kRawChunks];
   bool bv[kRawChunks];
 #pragma unroll
   for (int c = 0; c < kRawChunks; ++c) {
     const int jb = c * 64 + wx * 32 + (lane & 31);
     bv[c] = jb < q;
-    bp[c] = a.Qs + (size_t)(qb + (bv[c] ? jb : 0)) * Dp + kh;
+    bp[c] = a.Qs + (size_t)(qb + (bv[c] ? jb : 0)) * 16 + kh;
   }
   f32x16 acc[kRawChunks];
 #pragma unroll
   for (int c = 0; c < kRawChunks; ++c) acc[c] = f32x16{};
   for (int k0 = 0; k0 < Dp; k0 += 16) {
-    const bf16x8 a0 = load_bf16x8(ap + k0, tav);
-    const bf16x8 a1 = load_bf16x8(ap + tplane + k0, tav);
-    const bf16x8 a2 = load_bf16x8(ap + 2 * tplane + k0, tav);
+    const size_t ka = (size_t)k0 * a.Vt, kb = (size_t)k0 * a.Sq;
+    const bf16x8 a0 = load_bf16x8(ap + ka, tav);
+    const bf16x8 a1 = load_bf16x8(ap + tplane + ka, tav);
+    const bf16x8 a2 = load_bf16x8(ap + 2 * tplane + ka, tav);
 #pragma unroll
     for (int c = 0; c < kRawChunks; ++c) {
-      const bf16x8 b0 = load_bf16x8(bp[c] + k0, bv[c]);
-      const bf16x8 b1 = load_bf16x8(bp[c] + qplane + k0, bv[c]);
-      const bf16x8 b2 = load_bf16x8(bp[c] + 2 * qplane + k0, bv[c]);
+      const bf16x8 b0 = load_bf16x8(bp[c] + kb, bv[c]);
+      const bf16x8 b1 = load_bf16x8(bp[c] + qplane + kb, bv[c]);
+      const bf16x8 b2 = load_bf16x8(bp[c] + 2 * qplane + kb, bv[c]);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[c], 0, 0, 0);
       acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[c], 0, 0, 0);
