@@ -74,7 +74,12 @@ def _fp16(a):
     return np.asarray(a, dtype=np.float32).astype(np.float16).astype(np.float64)
 
 
-def test_inference_matches_oracle_forward(P, dg):
+# query lengths: the longest sets how many queries an inference block takes
+# (eight up to about 380 words, four at 500), and the second set leaves a
+# partial last group
+@pytest.mark.parametrize("lengths", [[1, 2, 7, 40, 130, 300], [1, 2, 7, 40, 130, 450, 500]],
+                         ids=["groups_of_8", "groups_of_4"])
+def test_inference_matches_oracle_forward(P, dg, lengths):
     """The matrix-core inference kernel against the FP64 forward of
     oracle/mlp_oracle.py with the kernel's FP16 roundings applied at the same
     points (standardised inputs, W1 and b1 (b1 enters as the weight of a
@@ -84,9 +89,9 @@ def test_inference_matches_oracle_forward(P, dg):
     transposed tile or a wrong lane group gives O(1) errors.  Against the
     exact FP64 forward the difference is the FP16 rounding itself."""
     from oracle import mlp_oracle as mo
-    say("[case] warm-start inference (FP16 matrix cores) vs FP64 forward")
+    say(f"[case] warm-start inference (FP16 matrix cores) vs FP64 forward, q = {lengths}")
     corpus, _, _ = dg.workload("batched", seed=5, N=300, n_queries=1)
-    qs = dg.make_queries(np.random.default_rng(6), corpus.V, [1, 2, 7, 40, 130, 300])
+    qs = dg.make_queries(np.random.default_rng(6), corpus.V, lengths)
     rng = np.random.default_rng(7)
     p = P.mlp_init(seed=4)
     L = P.mlp_layout()

@@ -272,6 +272,32 @@ def test_shared_distances_match_per_query_bitwise(P, dg):
             f"{st_on['ms_cdist']:.2f} ms shared ({st_on['distinct']} distinct of {len(words)})")
 
 
+def test_shared_distances_query_table_beyond_lds(P, dg):
+    # The shared epilogue stages the batch's query table (24 B per query and
+    # 2 B per query word) in LDS behind the distance row (rows of about 20 KB)
+    # when the two fit in 64 KB; this batch's table does not, so the epilogue
+    # reads it from global memory.  Outputs stay bitwise equal to the
+    # per-query distance stage.
+    corpus, _, _ = dg.workload("batched", seed=14, N=200, n_queries=1)
+    pool = np.random.default_rng(4).choice(corpus.V, size=900, replace=False)
+    lens = [512] * 40 + [300] * 8
+    q_ptr, words, weights = _overlapping_queries(lens, pool, seed=5)
+    sdp = (len(np.unique(words)) + 63) // 64 * 64
+    rows_b = max(1, min(16, 20 * 1024 // (sdp * 4))) * sdp * 4
+    assert rows_b + 24 * len(lens) + 2 * len(words) > 64 * 1024
+    say(f"[case] shared distances, query table beyond LDS: {len(lens)} queries, {len(words)} words")
+    with engine(P, corpus) as e:
+        e.set_queries(q_ptr, words, weights)
+        for mfma in (True, False):
+            st_off = e.run(-10.0, max_iter=15, share="off", mfma_cdist=mfma)
+            w_off = e.result()[0]
+            st_on = e.run(-10.0, max_iter=15, share="on", mfma_cdist=mfma)
+            w_on = e.result()[0]
+            assert st_off["shared"] == 0 and st_on["shared"] == 1
+            assert np.array_equal(w_on, w_off)
+            assert np.all(np.isfinite(w_on))
+
+
 def test_shared_distances_automatic_choice_and_limits(P, dg):
     say("[case] shared distances: automatic choice and limits")
     corpus, _, _ = dg.workload("batched", seed=13, N=400, n_queries=1)
