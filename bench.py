@@ -67,7 +67,7 @@ def sample_queries(qs):
     return [int(o[0]), int(o[len(o) // 2]), int(o[-1])]
 
 
-def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=40, pairs_per_batch=128):
+def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=40, pairs_per_batch=512):
     """Train the warm-start network on converged duals of n_queries x n_docs
     seeded synthetic pairs over the same embeddings (not part of a step)."""
     tc, tq = dg.training_set(vecs, n_queries=n_queries, n_docs=n_docs, seed=101)
@@ -377,8 +377,8 @@ def main() -> int:
     ap.add_argument("--warm", choices=("uniform", "learned"), default="learned")
     ap.add_argument("--accel", choices=("none", "anderson"), default="anderson",
                     help="u-update acceleration")
-    ap.add_argument("--train-queries", type=int, default=2000)
-    ap.add_argument("--train-docs", type=int, default=40)
+    ap.add_argument("--train-queries", type=int, default=12000)
+    ap.add_argument("--train-docs", type=int, default=80)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="smallest-fitting tile shapes instead of autotuned ones")
