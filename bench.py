@@ -67,7 +67,7 @@ def sample_queries(qs):
     return [int(o[0]), int(o[len(o) // 2]), int(o[-1])]
 
 
-def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=40, pairs_per_batch=512):
+def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=40, pairs_per_batch=1024):
     """Train the warm-start network on converged duals of n_queries x n_docs
     seeded synthetic pairs over the same embeddings (not part of a step)."""
     tc, tq = dg.training_set(vecs, n_queries=n_queries, n_docs=n_docs, seed=101)
