@@ -439,7 +439,12 @@ def main() -> int:
     tune_ms = tune_changed = None
     if not args.no_autotune:
         default_table = eng.get_tuning()
-        tune_ms = eng.autotune(lamda, iters=16, mfma_cdist=mfma, accel=args.accel == "anderson")
+        # candidates timed at about the iterations the run's pairs take (the learned
+        # start's mean is near 7, the uniform start's near 20), so the per-pair gather
+        # and final pass weigh in as they do in the run
+        tune_iters = 8 if args.warm == "learned" else 16
+        tune_ms = eng.autotune(lamda, iters=tune_iters, mfma_cdist=mfma,
+                               accel=args.accel == "anderson")
         tune_changed = int((eng.get_tuning() != default_table).sum())
         log(f"tile shapes autotuned in {tune_ms:.0f} ms ({tune_changed} table entries differ "
             f"from the smallest-fitting shape)")
