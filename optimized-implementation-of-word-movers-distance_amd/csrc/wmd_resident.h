@@ -843,7 +843,35 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
   bool prev = false, check_due = false;
   float res_best = INFINITY, res_mark = INFINITY;
   const float inv_q = 1.f / (float)q;
+  // Absorption (stabilised log-domain iteration): once a potential's
+  // magnitude passes kAbsorb, the row half-step's g and the current f are
+  // added into the tile's logits (L <- L + f_i + g_e) and both restart from
+  // 0 (the plan is unchanged: scaling the kernel's rows and columns and the
+  // scalings by the inverse factors is the same plan; the column half-step
+  // then returns lx + f, so the update, the residual and the accelerated
+  // rule's differences read the same in the shifted coordinates).  Without
+  // it the exponent arguments L + f + g carry potentials of a few hundred,
+  // whose FP32 rounding (about 1e-5) puts the residual's floor near 1e-4 at
+  // lamda = 50; after it they carry only the change since the last
+  // absorption.  The accelerated rule's centred potentials are f itself, so
+  // they restart from 0 too.  The final pass reads M's logits from the
+  // kernel block again (the tile's are shifted).
+  // Plain steps check every 16th iteration only: from the uniform start the
+  // pairs they leave above 1e-4 at the iteration cap at lamda = 50 are slow,
+  // not at a rounding floor (the same pairs and residuals with and without
+  // it), while the accelerated rule's extrapolation, and the plain steps of
+  // a pair its progress guard hands over, are what the floor stalls.
+  constexpr float kAbsorb = 16.f;
   while (it < a.max_iter) {
+    bool absorb = false;
+    if (!plain || (it & 15) == 0) {
+      float fm = 0.f;
+#pragma unroll
+      for (int m = 0; m < NI; ++m)
+        if (r[m] > 0.f) fm = fmaxf(fm, fabsf(f[m]));
+      // f is the same on every wave of the pair, so this is uniform
+      absorb = group_red<B, kMax>(fm) > kAbsorb;
+    }
 #pragma unroll
     for (int k = 0; k < NE; ++k) {
       float mx = kLogZero;
@@ -855,6 +883,23 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
       for (int m = 0; m < NI; ++m) s += exp2f(L[k][m] + f[m] - mx);
       s = group_sum<B>(s);
       g[k] = (lc[k] > 0.5f * kLogZero) ? lc[k] - mx - __log2f(s) : kLogZero;
+    }
+    if (absorb) {
+#pragma unroll
+      for (int k = 0; k < NE; ++k) {
+        const bool ve = lc[k] > 0.5f * kLogZero;
+#pragma unroll
+        for (int m = 0; m < NI; ++m)
+          if (ve && r[m] > 0.f && L[k][m] > 0.5f * kLogZero) L[k][m] += f[m] + g[k];
+        if (ve) g[k] = 0.f;
+      }
+#pragma unroll
+      for (int m = 0; m < NI; ++m) {
+        if (r[m] > 0.f) {
+          f[m] = 0.f;
+          al[m] = 0.f;
+        }
+      }
     }
     float lx[NI];
 #pragma unroll
@@ -992,12 +1037,14 @@ __global__ __launch_bounds__(W == 1 ? 256 : 64 * W) void k_resident_log(SolveArg
     const int e = ea + A * k;
     if (e < n) {
       const double ge = (double)lc[k] - (double)mx - (double)__log2f(s);
-      const double mr = (double)a.mrow[(int64_t)Q * a.Vt + a.doc_t[db + e]];
+      const int t = a.doc_t[db + e];
+      const double mr = (double)a.mrow[(int64_t)Q * a.Vt + t];
+      const float* row = Lq + (int64_t)t * ks;  // the logits as loaded
 #pragma unroll
       for (int m = 0; m < NI; ++m) {
         if (r[m] > 0.f && L[k][m] > 0.5f * kLogZero) {
           const double lp = (double)L[k][m] + (double)f[m] + ge;
-          acc += exp(lp * ln2) * (mr + (double)L[k][m] * m_per_l);
+          acc += exp(lp * ln2) * (mr + (double)row[b + B * m] * m_per_l);
         }
       }
     }
