@@ -615,7 +615,7 @@ def test_benchmarked_configurations_full_size(P, dg, oracle):
     captured graph, uniform and learned starts, Anderson steps to residual
     1e-4; plus the plain-step uniform start.  The shortest, median and longest
     queries against the first 400 documents: within 1e-4 of converged FP64
-    (the north star's bound), and the plain steps within 1e-4 of the FP64
+    (the stated error bound), and the plain steps within 1e-4 of the FP64
     oracle stopped by the same rule (sinkhorn_wmd_fused.cpp:245-393 iterated
     to the residual)."""
     corpus, qs, p = dg.workload("batched", seed=0)
