@@ -409,18 +409,25 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a, int group) 
     zi.z = pack_h2(zstd(ds[2], 4), zstd(ds[3], 5));
     zi.w = pack_h2(zstd(ds[4], 6), zstd(cdf, 12));
     *reinterpret_cast<uint4*>(row) = zi;
-#pragma unroll 1
-    for (int g = 0; g < nG; ++g) {
-      const int Q = Q0 + g;
+    // The query-dependent half of the sample's input row: features 13 and 14
+    // as in sample_feats, behind the query's statistics.  Formed one query
+    // ahead, so the quantile search and the row-minimum load overlap the
+    // previous query's network.
+    const auto query_half = [&](int g) {
       uint4 zq = qd[g];
       const int q = __builtin_amdgcn_readfirstlane((int)zq.x);
       const float* sqc = tables + 2 * g * qcap;
-      // features 13 and 14 as in sample_feats
       const float T = interp_bucketed(sqc, sqc + qcap, bk_of(g), q, cdf);
-      const float mr = a.mrow[(int64_t)Q * a.Vt + t];
+      const float mr = a.mrow[(int64_t)(Q0 + g) * a.Vt + t];
       const float gap = a.lam_abs * (__builtin_amdgcn_sqrtf(fmaf(T, T, n * n)) - mr);
       zq.x = pack_h2(zstd(T, 13), zstd(gap, 14));
-      *reinterpret_cast<uint4*>(row + 8) = zq;
+      return zq;
+    };
+    uint4 zq_next = query_half(0);
+#pragma unroll 1
+    for (int g = 0; g < nG; ++g) {
+      const int Q = Q0 + g;
+      *reinterpret_cast<uint4*>(row + 8) = zq_next;
       wave_lds_sync();
       half8_t Bx[2];
 #pragma unroll
@@ -428,6 +435,7 @@ __global__ __launch_bounds__(256) void k_warm_infer(WarmInferArgs a, int group) 
         Bx[nt] = *reinterpret_cast<const half8_t*>(stage + (wv * 64 + nt * 32 + lr) * kStageRow +
                                                    8 * hf);
       wave_lds_sync();  // the stage is rewritten by the next query or chunk
+      if (g + 1 < nG) zq_next = query_half(g + 1);
       float yp[2];
 #pragma unroll
       for (int nt = 0; nt < 2; ++nt) {
