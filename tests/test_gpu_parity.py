@@ -249,10 +249,11 @@ def test_shared_distances_match_per_query_bitwise(P, dg):
     corpus, _, _ = dg.workload("batched", seed=12, N=1500, n_queries=1)
     pool = np.random.default_rng(3).choice(corpus.V, size=900, replace=False)
     log_cap = max(cq for (_, cq, _), (_, lg) in zip(P.shapes(), P.api.shape_domains()) if lg)
-    lengths = [5, 40, 64, 65, 130, 300, 512]
+    # one-chunk queries of up to 16, 32 and 64 words (their row minima span 16, 32, 64 lanes)
+    lengths = [5, 16, 20, 32, 40, 64, 65, 130, 300, 512]
     cases = [("mfma", True, False, RES, lengths), ("fp32", False, False, RES, lengths),
              ("mfma log", True, True, RES, [q for q in lengths if q <= log_cap]),
-             ("mfma fused", True, False, FUS, [5, 40, 65])]
+             ("mfma fused", True, False, FUS, [5, 20, 40, 65])]
     for name, mfma, log_domain, variant, lens in cases:
         say(f"[case] shared distances {name} q={lens}")
         q_ptr, words, weights = _overlapping_queries(lens, pool, seed=len(lens))
