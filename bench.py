@@ -68,7 +68,7 @@ def sample_queries(qs):
 
 
 def train_warm_start(P, dg, vecs, lamda, n_queries, n_docs, epochs=40, pairs_per_batch=1024,
-                     seeds=(0, 1, 2)):
+                     seeds=(0, 1, 2, 3, 4, 5)):
     """Train the warm-start network on converged duals of n_queries x n_docs
     seeded synthetic pairs over the same embeddings (not part of a step), once
     per initialisation seed, and keep the network with the fewest mean
