@@ -407,6 +407,9 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="smallest-fitting tile shapes instead of autotuned ones")
+    ap.add_argument("--tune-iters", type=int, default=0,
+                    help="fixed iterations of the autotuner's candidate launches (default: 8 for "
+                         "the learned start, 16 for the uniform start)")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the solver kernels directly instead of replaying a HIP graph "
                          "(per-dispatch counter runs)")
@@ -467,7 +470,7 @@ def main() -> int:
         # candidates timed at about the iterations the run's pairs take (the learned
         # start's mean is near 7, the uniform start's near 20), so the per-pair gather
         # and final pass weigh in as they do in the run
-        tune_iters = 8 if args.warm == "learned" else 16
+        tune_iters = args.tune_iters or (8 if args.warm == "learned" else 16)
         tune_ms = eng.autotune(lamda, iters=tune_iters, mfma_cdist=mfma,
                                accel=args.accel == "anderson")
         tune_changed = int((eng.get_tuning() != default_table).sum())
